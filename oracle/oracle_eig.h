@@ -1,0 +1,363 @@
+// Oracle: canonical lowest-eigenpair solver and the CDFCI subspace update.
+// TEST INFRASTRUCTURE ONLY (checker for the HIP path, never shipped).
+//
+// The reference calls LAPACK dsyevr (RANGE='I', IL=IU=1) and then runs one
+// step of inverse iteration in __float128 on an LU factorisation
+// (src/solver.h:333-609).  LAPACK is an external, unversioned dependency
+// (Makefile:25), so this restatement fixes one deterministic algorithm that
+// the device solver reproduces operation for operation:
+//   1. Householder tridiagonalisation (LAPACK dsytd2 conventions),
+//   2. Sturm-count multisection (256 points per round) for the lowest
+//      eigenvalue of the tridiagonal matrix,
+//   3. three steps of inverse iteration on the tridiagonal matrix in double
+//      (dgtsv-style elimination with partial pivoting),
+//   4. the reference's extended-precision refinement step, here one
+//      inverse-iteration solve with shift lambda - 1e-14 (solver.h:439) in
+//      double-double, back-transformed with the Householder vectors in
+//      double-double and normalised.
+// Reductions use the canonical 64-lane order of oracle_num.h.
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "oracle_num.h"
+
+namespace oc {
+
+struct EigPair {
+  double lambda = 0.0;  // bisection eigenvalue of the tridiagonal form
+  std::vector<dd> q;    // refined, normalised eigenvector (q[0] >= 0)
+};
+
+static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
+static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
+static inline bool dd_ge(dd a, dd b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
+
+// Tridiagonal solve (T - shift) y = b, T = tridiag(e, d, e), dgtsv-style.
+static void tri_solve_d(int n, const double* d, const double* e, double shift, double* b) {
+  const double tiny = 1e-290;
+  std::vector<double> dg(n), du(n > 1 ? n - 1 : 1), dl(n > 1 ? n - 1 : 1), du2(n > 2 ? n - 2 : 1, 0.0);
+  for (int i = 0; i < n; ++i) dg[i] = d[i] - shift;
+  for (int i = 0; i + 1 < n; ++i) {
+    du[i] = e[i];
+    dl[i] = e[i];
+  }
+  for (int i = 0; i + 1 < n; ++i) {
+    if (std::fabs(dg[i]) >= std::fabs(dl[i])) {
+      if (dg[i] == 0.0) dg[i] = tiny;
+      double fact = dl[i] / dg[i];
+      dg[i + 1] = dg[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+      if (i < n - 2) du2[i] = 0.0;
+    } else {
+      double fact = dg[i] / dl[i];
+      double ndgi = dl[i];
+      double ndui = dg[i + 1];
+      double ndgi1 = du[i] - fact * dg[i + 1];
+      if (i < n - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -fact * du[i + 1];
+      }
+      dg[i] = ndgi;
+      du[i] = ndui;
+      dg[i + 1] = ndgi1;
+      double bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = bi - fact * bi1;
+    }
+  }
+  if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
+  b[n - 1] = b[n - 1] / dg[n - 1];
+  if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
+  for (int i = n - 3; i >= 0; --i) {
+    double t = b[i] - du[i] * b[i + 1];
+    t = t - du2[i] * b[i + 2];
+    b[i] = t / dg[i];
+  }
+}
+
+static void tri_solve_dd(int n, const double* d, const double* e, double shift, dd* b) {
+  const dd tiny{1e-290, 0.0};
+  std::vector<dd> dg(n), du(n > 1 ? n - 1 : 1), dl(n > 1 ? n - 1 : 1),
+      du2(n > 2 ? n - 2 : 1, dd{0.0, 0.0});
+  for (int i = 0; i < n; ++i) dg[i] = two_sum(d[i], -shift);
+  for (int i = 0; i + 1 < n; ++i) {
+    du[i] = dd_make(e[i]);
+    dl[i] = dd_make(e[i]);
+  }
+  for (int i = 0; i + 1 < n; ++i) {
+    if (dd_ge(dd_abs(dg[i]), dd_abs(dl[i]))) {
+      if (dd_is_zero(dg[i])) dg[i] = tiny;
+      dd fact = dd_div(dl[i], dg[i]);
+      dg[i + 1] = dd_sub(dg[i + 1], dd_mul(fact, du[i]));
+      b[i + 1] = dd_sub(b[i + 1], dd_mul(fact, b[i]));
+      if (i < n - 2) du2[i] = dd{0.0, 0.0};
+    } else {
+      dd fact = dd_div(dg[i], dl[i]);
+      dd ndgi = dl[i];
+      dd ndui = dg[i + 1];
+      dd ndgi1 = dd_sub(du[i], dd_mul(fact, dg[i + 1]));
+      if (i < n - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = dd_neg(dd_mul(fact, du[i + 1]));
+      }
+      dg[i] = ndgi;
+      du[i] = ndui;
+      dg[i + 1] = ndgi1;
+      dd bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = dd_sub(bi, dd_mul(fact, bi1));
+    }
+  }
+  if (dd_is_zero(dg[n - 1])) dg[n - 1] = tiny;
+  b[n - 1] = dd_div(b[n - 1], dg[n - 1]);
+  if (n >= 2) b[n - 2] = dd_div(dd_sub(b[n - 2], dd_mul(du[n - 2], b[n - 1])), dg[n - 2]);
+  for (int i = n - 3; i >= 0; --i) {
+    dd t = dd_sub(b[i], dd_mul(du[i], b[i + 1]));
+    t = dd_sub(t, dd_mul(du2[i], b[i + 2]));
+    b[i] = dd_div(t, dg[i]);
+  }
+}
+
+static int sturm_count(int n, const double* d, const double* e2, double pivmin, double x) {
+  double q = d[0] - x;
+  if (std::fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// Lowest eigenvalue of tridiag(e, d, e) by 256-point multisection.
+static double tri_lowest(int n, const double* d, const double* e) {
+  if (n == 1) return d[0];
+  std::vector<double> e2(n - 1);
+  double emax = 1.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    e2[i] = e[i] * e[i];
+    emax = std::max(emax, e2[i]);
+  }
+  double pivmin = 2.2250738585072014e-308 * emax;
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    double r = 0.0;
+    if (i > 0) r = r + std::fabs(e[i - 1]);
+    if (i < n - 1) r = r + std::fabs(e[i]);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  double pad = (hi - lo) * 1e-6 + 1e-200;
+  lo = lo - pad;
+  hi = hi + pad;
+  for (int round = 0; round < 16; ++round) {
+    double w = (hi - lo) / 257.0;
+    int j = 256;
+    double xs[256];
+    for (int t = 0; t < 256; ++t) xs[t] = lo + (double)(t + 1) * w;
+    for (int t = 0; t < 256; ++t)
+      if (sturm_count(n, d, e2.data(), pivmin, xs[t]) >= 1) {
+        j = t;
+        break;
+      }
+    double nlo = (j == 0) ? lo : xs[j - 1];
+    double nhi = (j == 256) ? hi : xs[j];
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo + 0.5 * (hi - lo);
+}
+
+// A: n x n symmetric, row-major, both triangles filled with identical bits.
+static EigPair lowest_eigpair(int n, std::vector<double> A) {
+  EigPair out;
+  out.q.assign(n, dd{0.0, 0.0});
+  if (n == 1) {
+    out.lambda = A[0];
+    out.q[0] = dd{1.0, 0.0};
+    return out;
+  }
+  std::vector<double> d(n), e(n - 1), tau(n, 0.0);
+  std::vector<std::vector<double>> vs(n);
+  std::vector<double> v(n), p(n), w(n);
+  for (int s = 0; s <= n - 3; ++s) {
+    int m = n - s - 1;
+    double alpha = A[(s + 1) * n + s];
+    double xnorm2 = red64(m - 1, [&](int i) {
+      double t = A[(s + 2 + i) * n + s];
+      return t * t;
+    });
+    d[s] = A[s * n + s];
+    if (xnorm2 == 0.0) {
+      tau[s] = 0.0;
+      e[s] = alpha;
+      continue;
+    }
+    double beta = -std::copysign(std::sqrt(alpha * alpha + xnorm2), alpha);
+    double t = (beta - alpha) / beta;
+    double scal = 1.0 / (alpha - beta);
+    v[0] = 1.0;
+    for (int i = 1; i < m; ++i) v[i] = A[(s + 1 + i) * n + s] * scal;
+    tau[s] = t;
+    e[s] = beta;
+    vs[s].assign(v.begin(), v.begin() + m);
+    for (int r = 0; r < m; ++r) {
+      double acc = 0.0;
+      const double* row = &A[(s + 1 + r) * n + (s + 1)];
+      for (int c = 0; c < m; ++c) acc = acc + row[c] * v[c];
+      p[r] = t * acc;
+    }
+    double K = (-0.5 * t) * red64(m, [&](int i) { return p[i] * v[i]; });
+    for (int r = 0; r < m; ++r) w[r] = p[r] + K * v[r];
+    for (int r = 0; r < m; ++r) {
+      double* row = &A[(s + 1 + r) * n + (s + 1)];
+      for (int c = 0; c < m; ++c) row[c] = row[c] - (v[r] * w[c] + w[r] * v[c]);
+    }
+  }
+  d[n - 2] = A[(n - 2) * n + (n - 2)];
+  d[n - 1] = A[(n - 1) * n + (n - 1)];
+  e[n - 2] = A[(n - 1) * n + (n - 2)];
+
+  double lam = tri_lowest(n, d.data(), e.data());
+  out.lambda = lam;
+
+  // Inverse iteration on T in double.
+  std::vector<double> y(n, 1.0);
+  for (int it = 0; it < 3; ++it) {
+    tri_solve_d(n, d.data(), e.data(), lam, y.data());
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(y[i]));
+    for (int i = 0; i < n; ++i) y[i] = y[i] / mx;
+  }
+
+  // Extended-precision refinement step (shift lambda - 1e-14, solver.h:439).
+  std::vector<dd> q(n);
+  for (int i = 0; i < n; ++i) q[i] = dd_make(y[i]);
+  tri_solve_dd(n, d.data(), e.data(), lam - 1e-14, q.data());
+  for (int s = n - 3; s >= 0; --s) {
+    if (tau[s] == 0.0) continue;
+    int m = n - s - 1;
+    const std::vector<double>& vv = vs[s];
+    dd r = red64_dd(m, [&](int i) { return dd_mul_d(q[s + 1 + i], vv[i]); });
+    dd t = dd_mul_d(r, tau[s]);
+    for (int i = 0; i < m; ++i) q[s + 1 + i] = dd_sub(q[s + 1 + i], dd_mul_d(t, vv[i]));
+  }
+  dd nrm2 = red64_dd(n, [&](int i) { return dd_mul(q[i], q[i]); });
+  dd nrm = dd_sqrt(nrm2);
+  for (int i = 0; i < n; ++i) q[i] = dd_div(q[i], nrm);
+  int lead = 0;
+  while (lead < n - 1 && dd_is_zero(q[lead])) ++lead;
+  if (dd_lt0(q[lead]))
+    for (int i = 0; i < n; ++i) q[i] = dd_neg(q[i]);
+  out.q = q;
+  return out;
+}
+
+struct EigStats {
+  double max_residual = 0.0;
+  double curr_max_abs_evec = 0.0;
+  long sum_iterations = 0;
+  long sum_inv_iter = 0;
+  long curr_inv_iter = 0;
+  double sum_diff_inv_iter = 0.0;
+  double max_diff_inv_iter = 0.0;
+  double curr_diff_inv_iter = 0.0;
+};
+
+enum { EIG_OK = 0, EIG_NONNEG = 1 };
+
+// The CDFCI subspace update (src/solver.h:333-609), canonical form.
+// Hs: k x k row-major; only Hs[i*k+j] with j >= i is read (the reference's
+// dsymv/dsyevr read the 'L' triangle of the column-major view, which is the
+// upper triangle of the row-major array).  Returns dx; updates xx and sc.
+static int eig_update(int k, const double* Hs, const double* x, const double* z, dd& xx, dd& sc,
+                      dd xz, double* dx, EigStats& st) {
+  auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
+  double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
+  for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
+  for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
+  std::vector<double> Hx(k);
+  for (int r = 0; r < k; ++r) {
+    double acc = 0.0;
+    for (int c = 0; c < k; ++c) acc = acc + H(r, c) * x[c];
+    Hx[r] = acc;
+  }
+  for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
+  double yy = dd_add_d(xx, -xTx).hi;
+  double y1 = std::sqrt(yy);
+
+  if (std::fabs(yy) > 1e-14) {
+    int n = k + 1;
+    std::vector<double> A((size_t)n * n);
+    dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+    A[0] = dd_div_d(a00, yy).hi;
+    for (int i = 0; i < k; ++i) {
+      double b = (z[i] - Hx[i]) / y1;
+      A[(i + 1) * n] = b;
+      A[i + 1] = b;
+    }
+    for (int i = 0; i < k; ++i)
+      for (int j = i; j < k; ++j) {
+        A[(i + 1) * n + (j + 1)] = Hs[i * k + j];
+        A[(j + 1) * n + (i + 1)] = Hs[i * k + j];
+      }
+    EigPair ep = lowest_eigpair(n, A);
+    double lam = ep.lambda;
+    std::vector<double> ev(n), dl(n), Ae(n);
+    for (int i = 0; i < n; ++i) {
+      ev[i] = ep.q[i].hi;
+      dl[i] = ep.q[i].lo;
+    }
+    for (int r = 0; r < n; ++r) {
+      double acc = 0.0;
+      for (int c = 0; c < n; ++c) acc = acc + A[r * n + c] * ev[c];
+      Ae[r] = acc;
+    }
+    double res2 = red64(n, [&](int i) {
+      double t = Ae[i] + (-lam) * ev[i];
+      return t * t;
+    });
+    st.max_residual = std::max(st.max_residual, std::sqrt(res2));
+    double mev = 0.0;
+    for (int i = 1; i < n; ++i) mev = std::max(mev, std::fabs(ev[i]));
+    st.curr_max_abs_evec = mev;
+    dd t1 = red64_dd(n, [&](int i) { return two_prod(ev[i], Ae[i]); });
+    dd t2 = red64_dd(n, [&](int i) { return two_prod(dl[i], Ae[i]); });
+    dd rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    dd lam_q = dd_neg(rq);
+    if (!(lam_q.hi > 0.0)) return EIG_NONNEG;
+    dd gam = dd_div_d(ep.q[0], y1);
+    dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
+    for (int i = 0; i < k; ++i) dx[i] = dd_add_d(dd_div(ep.q[i + 1], gam), -x[i]).hi;
+    double sc_d = ev[0] / y1 * std::sqrt(-lam);
+    dd diff = dd_add_d(dd_abs(sc_q), -std::fabs(sc_d));
+    double ad = std::fabs(diff.hi);
+    st.sum_iterations += 1;
+    st.sum_inv_iter += 1;
+    st.curr_inv_iter = 1;
+    st.sum_diff_inv_iter += ad;
+    st.max_diff_inv_iter = std::max(st.max_diff_inv_iter, ad);
+    st.curr_diff_inv_iter = ad;
+    sc = sc_q;
+    xx = dd_div(dd_div(lam_q, sc_q), sc_q);
+    return EIG_OK;
+  }
+  // First iteration: only the reference determinant carries weight.
+  std::vector<double> A((size_t)k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) A[i * k + j] = H(i, j);
+  EigPair ep = lowest_eigpair(k, A);
+  double lam = ep.lambda;
+  if (!(lam < 0.0)) return EIG_NONNEG;
+  double r = std::sqrt(-lam);
+  for (int i = 0; i < k; ++i) dx[i] = ep.q[i].hi * r - x[i];
+  xx = dd_make(-lam);
+  sc = dd_make(1.0);
+  return EIG_OK;
+}
+
+}  // namespace oc
