@@ -1,0 +1,146 @@
+/*
+ * mcdfci.h -- C ABI of the MI355X coordinate-descent FCI (CDFCI) library.
+ *
+ * The reference (ninotreve/mcdfci) is a header-only C++ program with no
+ * plugin or FFI surface; its hot path is the solver loop of
+ * src/solver.h:672-893 over the interfaces
+ *   Hamiltonian<N>        src/hamiltonian.h:25-54
+ *   WaveFunction<N>       src/wavefunction.h:163-221
+ *   CoordinatePick<N>     src/solver.h:94-101
+ *   CoordinateUpdate<N>   src/solver.h:230-250
+ * Each entry point below names the reference code it replaces.  Everything
+ * runs on one HIP device; host arrays belong to the caller and are copied.
+ *
+ * Determinants cross this boundary in the reference's interleaved format:
+ * spin-orbital 2p is alpha of spatial orbital p and 2p+1 is beta
+ * (src/input.h:242-243); bit s%64 of word s/64 is set when spin-orbital s is
+ * occupied (src/determinant.h:24-100).  Arrays of determinants hold
+ * `nwords` uint64 words per determinant (mcdfci_info).
+ *
+ * Every call returns a status (MCDFCI_OK = 0); mcdfci_last_error(ctx)
+ * gives the message.  Calls on one context are not re-entrant.
+ */
+#ifndef MCDFCI_H
+#define MCDFCI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MCDFCI_OK = 0,
+  MCDFCI_E_INPUT = 1,       /* invalid argument or file (std::invalid_argument in the reference) */
+  MCDFCI_E_OOM = 2,         /* device allocation failed */
+  MCDFCI_E_TABLE_FULL = 3,  /* std::overflow_error of src/wavefunction.h:594-599 */
+  MCDFCI_E_HIP = 4,         /* HIP runtime error */
+  MCDFCI_E_NUMERIC = 5,     /* subspace eigenvalue >= 0: sqrt(-lambda) undefined (src/solver.h:524,591) */
+  MCDFCI_E_CANDIDATES = 6,  /* fewer than k distinct candidates (src/solver.h:218-223 reads past the end) */
+  MCDFCI_E_UNSUPPORTED = 7  /* a configuration outside what the device path implements */
+};
+
+enum { MCDFCI_PREC_FP64 = 0, MCDFCI_PREC_MIXED = 1 };
+
+typedef struct mcdfci_ctx mcdfci_ctx;
+
+/* Options.  Meanings and defaults follow src/run_wrapper.h:64-87. */
+typedef struct {
+  int num_of_coordinate; /* k */
+  int estimate;          /* M: per-column candidate list length */
+  double z_threshold;    /* compression gate |dz*scale| > z_threshold (src/wavefunction.h:518) */
+  double max_memory;     /* GB for the wavefunction table (src/run_wrapper.h:203-213) */
+  double threshold;      /* hamiltonian.threshold: table entries need |v| > threshold */
+  int precision;         /* MCDFCI_PREC_FP64 or MCDFCI_PREC_MIXED */
+  int device;            /* HIP device ordinal */
+} mcdfci_opts;
+
+/* Spin-restricted Hamiltonian in spatial orbitals (what an RHF FCIDUMP holds). */
+typedef struct {
+  int norb;                      /* spatial orbitals */
+  int nelec;
+  int ms2;
+  double core;
+  const double* h;               /* norb*norb one-electron integrals */
+  const double* eri;             /* norb^4 chemists' (pq|rs), fully 8-fold symmetric */
+  const double* orbital_energy;  /* norb entries, or NULL (HF = lowest orbitals) */
+} mcdfci_ham;
+
+typedef struct {
+  double energy;        /* xz / xx (src/wavefunction.h:195-198) */
+  double dx_max;        /* max |dx| of the last iteration */
+  int64_t x_size;       /* |x|_0 */
+  int64_t z_size;       /* |z|_0 */
+  int64_t cols_size;    /* |H[I]|_0: sum of the last iteration's column lengths */
+  int64_t updates;      /* upds */
+  double scale;         /* scale (hi part) */
+  double xx;            /* ||x||^2 (hi part) */
+  double xz;            /* x.z */
+  int64_t iterations;   /* completed iterations after initialisation */
+  /* eig statistics of the [LOG] line (src/solver.h:787-798) */
+  double max_residual;
+  double curr_max_abs_evec;
+  int64_t sum_iterations;
+  int64_t sum_inv_iter;
+  int64_t curr_inv_iter;
+  double sum_diff_inv_iter;
+  double curr_diff_inv_iter;
+  double max_diff_inv_iter;
+  int64_t table_slots;  /* device table capacity (slots) */
+  double max_size;      /* the reference's max_wavefunction_size */
+} mcdfci_report;
+
+const char* mcdfci_last_error(const mcdfci_ctx* ctx);
+
+/* Replaces Fcidump + HamiltonianMolecule<N>(fci, threshold) construction
+ * (src/input.h:88-152, src/hamiltonian.h:98-109) and the table sizing of
+ * src/run_wrapper.h:181,203-213. */
+int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** out);
+int mcdfci_create_from_fcidump(const char* path, const mcdfci_opts* opts, mcdfci_ctx** out);
+int mcdfci_destroy(mcdfci_ctx* ctx);
+
+/* info[0..5] = {nwords, spin-orbitals, nelec, ms2, column enumeration size, k} */
+int mcdfci_info(const mcdfci_ctx* ctx, int64_t* info);
+
+/* HamiltonianMolecule::get_hartree_fock (src/hamiltonian.h:469-492). */
+int mcdfci_hartree_fock(mcdfci_ctx* ctx, uint64_t* det);
+
+/* Outer CDFCI::solve initialisation (src/solver.h:853-873): x = e_ref,
+ * z = H[:,ref], candidates = top-M of that column.  ref_det NULL -> HF. */
+int mcdfci_init(mcdfci_ctx* ctx, const uint64_t* ref_det);
+
+/* n_iter iterations of the inner loop (src/solver.h:701-759) on the device. */
+int mcdfci_step(mcdfci_ctx* ctx, int64_t n_iter);
+
+/* The report row (src/solver.h:761-798). Synchronises the device. */
+int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
+
+/* Phase times in seconds (pick, submatrix, columns, eig, update_x, update_z),
+ * the step_timer of src/solver.h:683,704-754; device phases are measured
+ * with HIP events when timing is enabled (mcdfci_set_timing). */
+int mcdfci_set_timing(mcdfci_ctx* ctx, int enable);
+int mcdfci_timers(mcdfci_ctx* ctx, double* t6);
+
+/* Test hooks. */
+/* HamiltonianMolecule::get_column (src/hamiltonian.h:412-418): writes the
+ * column of det (diagonal first, then the rest sorted by determinant). */
+int mcdfci_column(mcdfci_ctx* ctx, const uint64_t* det, uint64_t* dets, double* vals, int64_t cap,
+                  int64_t* n);
+/* CDFCI::get_hamiltonian_submatrix (src/solver.h:637-670); H is k*k
+ * row-major, entries with j >= i are filled (the triangle the update reads). */
+int mcdfci_submatrix(mcdfci_ctx* ctx, int k, const uint64_t* dets, double* H);
+/* CoordinateUpdateEig::operator() (src/solver.h:333-609) on given inputs.
+ * s[6] = {xx.hi, xx.lo, scale.hi, scale.lo, xz.hi, xz.lo} in, xx/scale out. */
+int mcdfci_eig_update(mcdfci_ctx* ctx, int k, const double* H, const double* x, const double* z,
+                      double* s, double* dx);
+/* Contents of the wavefunction table, sorted by determinant
+ * (WaveFunctionCuckoo::fetch_val over all entries, src/wavefunction.h:631-651). */
+int mcdfci_table_dump(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);
+/* The candidate list passed to the next pick (sub_xz, src/wavefunction.h:567-589). */
+int mcdfci_candidates(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCDFCI_H */

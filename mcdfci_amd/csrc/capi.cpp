@@ -1,0 +1,718 @@
+// Host engine behind the C ABI of include/mcdfci.h: builds the device
+// integral tables, owns the HBM wavefunction table and iteration buffers,
+// and sequences the kernels of one CDFCI iteration on one HIP stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mcdfci.h"
+#include "device.h"
+#include "fcidump.h"
+#include "kernels.h"
+
+using namespace mc;
+
+namespace {
+
+std::string g_err;
+
+int binom2(int n) { return n < 2 ? 0 : n * (n - 1) / 2; }
+
+}  // namespace
+
+struct mcdfci_ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  SpatialHam ham;
+  mcdfci_opts opts{};
+  DevHam dh{};
+  // integral tables (one allocation each)
+  double* d_eri = nullptr;
+  float* d_eri32 = nullptr;
+  double* d_h1 = nullptr;
+  float* d_h132 = nullptr;
+  double* d_ss = nullptr;
+  double* d_so = nullptr;
+  float* d_ss32 = nullptr;
+  float* d_so32 = nullptr;
+  uint8_t* d_sok = nullptr;
+  double* d_d1 = nullptr;
+  double* d_d2 = nullptr;
+  // wavefunction table
+  Slot* tab = nullptr;
+  uint64_t nslots = 0;
+  double max_size = 0.0;
+  int table_pow = 0;
+  // solver state and iteration buffers
+  DevState* st = nullptr;
+  DevState* st_scratch = nullptr;
+  int k = 1, M = 1;
+  uint64_t* pdet = nullptr;
+  uint32_t* pslot = nullptr;
+  double* px = nullptr;
+  double* pz = nullptr;
+  double* dx = nullptr;
+  double* Hs = nullptr;
+  double* Aglob = nullptr;
+  uint64_t* vis = nullptr;
+  int vis_words = 0;
+  int nchunks = 0;
+  StageEnt* stage = nullptr;
+  int* stage_cnt = nullptr;
+  int64_t* part = nullptr;
+  int64_t* col_xz = nullptr;
+  CandEnt* cand = nullptr;
+  int* cand_cnt = nullptr;
+  size_t iter_bytes = 0;
+  bool ready = false;
+  bool timing = false;
+  double timers[6] = {0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[7] = {};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(mcdfci_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHECK(c, x)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) return fail((c), MCDFCI_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(mcdfci_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, MCDFCI_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                     hipGetErrorString(e));
+  }
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return MCDFCI_OK;
+}
+
+void dfree(mcdfci_ctx* c, void* p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+template <class T>
+int upload(mcdfci_ctx* c, T** p, const std::vector<T>& v) {
+  int rc = dalloc(c, p, v.size());
+  if (rc) return rc;
+  HIPCHECK(c, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return MCDFCI_OK;
+}
+
+// Enumeration layout of the (na, nb) sector.
+void set_sector(DevHam& H, int na, int nb) {
+  H.na = na;
+  H.nb = nb;
+  H.nva = H.M - na;
+  H.nvb = H.M - nb;
+  H.ne = na + nb;
+  H.cpa_o = binom2(na);
+  H.cpb_o = binom2(nb);
+  H.cpa_v = binom2(H.nva);
+  H.cpb_v = binom2(H.nvb);
+  H.n_sa = na * H.nva;
+  H.n_sb = nb * H.nvb;
+  H.n_daa = H.cpa_o * H.cpa_v;
+  H.n_dbb = H.cpb_o * H.cpb_v;
+  H.n_dab = na * nb * H.nva * H.nvb;
+  H.off_sb = 1 + H.n_sa;
+  H.off_daa = H.off_sb + H.n_sb;
+  H.off_dbb = H.off_daa + H.n_daa;
+  H.off_dab = H.off_dbb + H.n_dbb;
+  H.n_enum = H.off_dab + H.n_dab;
+}
+
+void det_sector(uint64_t det, int& na, int& nb) {
+  na = __builtin_popcountll(det & 0x5555555555555555ULL);
+  nb = __builtin_popcountll(det & 0xAAAAAAAAAAAAAAAAULL);
+}
+
+int build_tables(mcdfci_ctx* c) {
+  const SpatialHam& S = c->ham;
+  const int M = S.norb, ns = 2 * M;
+  const size_t M2 = (size_t)M * M, M3 = M2 * M, M4 = M2 * M2;
+  auto E = [&](int p, int q, int r, int s) { return S.eri[(((size_t)p * M + q) * M + r) * M + s]; };
+  std::vector<double> ss(M3), so(M3), d1(ns), d2((size_t)ns * ns, 0.0);
+  std::vector<uint8_t> sok(M2);
+  for (int p = 0; p < M; ++p)
+    for (int r = 0; r < M; ++r) {
+      double mx = std::fabs(S.h[p * M + r]);
+      for (int k = 0; k < M; ++k) {
+        double a = E(p, r, k, k);
+        double b = E(p, k, k, r);
+        ss[((size_t)p * M + r) * M + k] = a - b;
+        so[((size_t)p * M + r) * M + k] = a;
+        mx = std::max(mx, std::fabs(a - b));
+        mx = std::max(mx, std::fabs(a));
+      }
+      sok[p * M + r] = mx > c->opts.threshold ? 1 : 0;
+    }
+  for (int i = 0; i < ns; ++i) d1[i] = S.h[(i >> 1) * M + (i >> 1)];
+  for (int i = 0; i < ns; ++i)
+    for (int j = i + 1; j < ns; ++j) {
+      int p = i >> 1, q = j >> 1;
+      double v = E(p, p, q, q);
+      if ((i & 1) == (j & 1)) v = v - E(p, q, q, p);
+      d2[(size_t)i * ns + j] = v;
+    }
+  std::vector<float> eri32(M4), h132(M2), ss32(M3), so32(M3);
+  for (size_t i = 0; i < M4; ++i) eri32[i] = (float)S.eri[i];
+  for (size_t i = 0; i < M2; ++i) h132[i] = (float)S.h[i];
+  for (size_t i = 0; i < M3; ++i) {
+    ss32[i] = (float)ss[i];
+    so32[i] = (float)so[i];
+  }
+  int rc;
+  if ((rc = upload(c, &c->d_eri, S.eri))) return rc;
+  if ((rc = upload(c, &c->d_eri32, eri32))) return rc;
+  if ((rc = upload(c, &c->d_h1, S.h))) return rc;
+  if ((rc = upload(c, &c->d_h132, h132))) return rc;
+  if ((rc = upload(c, &c->d_ss, ss))) return rc;
+  if ((rc = upload(c, &c->d_so, so))) return rc;
+  if ((rc = upload(c, &c->d_ss32, ss32))) return rc;
+  if ((rc = upload(c, &c->d_so32, so32))) return rc;
+  if ((rc = upload(c, &c->d_sok, sok))) return rc;
+  if ((rc = upload(c, &c->d_d1, d1))) return rc;
+  if ((rc = upload(c, &c->d_d2, d2))) return rc;
+  DevHam& H = c->dh;
+  H.M = M;
+  H.ns = ns;
+  H.mixed = c->opts.precision == MCDFCI_PREC_MIXED ? 1 : 0;
+  H.core = S.core;
+  H.thr = c->opts.threshold;
+  H.eri = c->d_eri;
+  H.eri32 = c->d_eri32;
+  H.h1 = c->d_h1;
+  H.h132 = c->d_h132;
+  H.s_same = c->d_ss;
+  H.s_opp = c->d_so;
+  H.s_same32 = c->d_ss32;
+  H.s_opp32 = c->d_so32;
+  H.single_ok = c->d_sok;
+  H.d1 = c->d_d1;
+  H.d2 = c->d_d2;
+  int na = (S.nelec + S.ms2) / 2, nb = (S.nelec - S.ms2) / 2;
+  set_sector(H, na, nb);
+  return MCDFCI_OK;
+}
+
+// Table sizing of src/run_wrapper.h:203-213: the largest power of two 2^p
+// with 2^p * (8N + 16) bytes below max_memory GB; the reference's capacity
+// is floor(0.79 * 2^p) and it stops when z_size > 0.79 * capacity.
+void table_size(double max_memory, int nwords, uint64_t& slots, double& max_size, int& pw) {
+  const int bytes = nwords * 8 + 16;
+  uint64_t size = 8;
+  while ((double)(size * (uint64_t)bytes) < max_memory * 1073741824.0) size *= 2;
+  size /= 2;
+  slots = size;
+  max_size = (double)(uint64_t)(0.79 * (double)size);
+  pw = 0;
+  while ((1ULL << pw) < size) ++pw;
+}
+
+int alloc_iteration(mcdfci_ctx* c) {
+  for (void* p : {(void*)c->pdet, (void*)c->pslot, (void*)c->px, (void*)c->pz, (void*)c->dx, (void*)c->Hs,
+                  (void*)c->Aglob, (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
+                  (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt})
+    dfree(c, p);
+  const int k = c->k, M = c->M;
+  const DevHam& H = c->dh;
+  c->nchunks = (H.n_enum + kChunk - 1) / kChunk;
+  c->vis_words = c->nchunks * (kChunk / 64);
+  const int mw = std::min(M, kChunk);
+  if ((long long)c->nchunks * mw > kSortCap)
+    return fail(c, MCDFCI_E_UNSUPPORTED,
+                "estimate * column chunks exceeds the per-column merge capacity (" + std::to_string(kSortCap) +
+                    "); reduce estimate");
+  if ((long long)k * M > kSortCap)
+    return fail(c, MCDFCI_E_UNSUPPORTED,
+                "num_of_coordinate * estimate exceeds the pick capacity (" + std::to_string(kSortCap) + ")");
+  int rc;
+  if ((rc = dalloc(c, &c->pdet, k))) return rc;
+  if ((rc = dalloc(c, &c->pslot, k))) return rc;
+  if ((rc = dalloc(c, &c->px, k))) return rc;
+  if ((rc = dalloc(c, &c->pz, k))) return rc;
+  if ((rc = dalloc(c, &c->dx, k))) return rc;
+  if ((rc = dalloc(c, &c->Hs, (size_t)k * k))) return rc;
+  if ((rc = dalloc(c, &c->Aglob, (size_t)(k + 1) * (k + 1)))) return rc;
+  if ((rc = dalloc(c, &c->vis, (size_t)k * c->vis_words))) return rc;
+  if ((rc = dalloc(c, &c->stage, (size_t)k * c->nchunks * mw))) return rc;
+  if ((rc = dalloc(c, &c->stage_cnt, (size_t)k * c->nchunks))) return rc;
+  if ((rc = dalloc(c, &c->part, (size_t)k * c->nchunks * 4))) return rc;
+  if ((rc = dalloc(c, &c->col_xz, (size_t)2 * k))) return rc;
+  if ((rc = dalloc(c, &c->cand, (size_t)k * M))) return rc;
+  if ((rc = dalloc(c, &c->cand_cnt, k))) return rc;
+  HIPCHECK(c, hipMemset(c->cand_cnt, 0, sizeof(int) * k));
+  return MCDFCI_OK;
+}
+
+int check_state_error(mcdfci_ctx* c, const DevState& s) {
+  switch (s.error) {
+    case ERR_NONE:
+      return MCDFCI_OK;
+    case ERR_TABLE_FULL:
+      return fail(c, MCDFCI_E_TABLE_FULL,
+                  "The hash table is full. Please increase max_wavefunction_size or z_threshold.");
+    case ERR_NUMERIC:
+      return fail(c, MCDFCI_E_NUMERIC,
+                  "The lowest subspace eigenvalue is not negative, so sqrt(-lambda) is undefined. The "
+                  "Hamiltonian (core energy included) must have a negative lowest eigenvalue.");
+    case ERR_CANDIDATES:
+      return fail(c, MCDFCI_E_CANDIDATES,
+                  "Fewer than num_of_coordinate distinct candidate determinants are available.");
+    case ERR_PROBE:
+      return fail(c, MCDFCI_E_TABLE_FULL, "The hash table has no free slot left.");
+    case ERR_CAP:
+      return fail(c, MCDFCI_E_UNSUPPORTED, "candidate list exceeds the device sort capacity.");
+    default:
+      return fail(c, MCDFCI_E_HIP, "device error " + std::to_string(s.error));
+  }
+}
+
+void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
+  hipStream_t s = c->stream;
+  launch_scatter(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, k,
+                 c->nchunks);
+  launch_pass2(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, c->M,
+               k, c->nchunks, c->stage, c->stage_cnt, c->part);
+  launch_merge(s, c->st, c->pdet, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
+               c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration);
+}
+
+void launch_iteration(mcdfci_ctx* c) {
+  hipStream_t s = c->stream;
+  const int k = c->k;
+  if (c->timing) (void)hipEventRecord(c->ev[0], s);
+  launch_pick(s, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz);
+  if (c->timing) (void)hipEventRecord(c->ev[1], s);
+  launch_submatrix(s, c->dh, c->st, c->pdet, k, c->Hs, k);
+  if (c->timing) (void)hipEventRecord(c->ev[2], s);
+  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, c->Aglob, c->tab, c->pslot, 0);
+  if (c->timing) (void)hipEventRecord(c->ev[3], s);
+  launch_update_z(c, k, 1);
+  if (c->timing) {
+    (void)hipEventRecord(c->ev[4], s);
+    (void)hipEventSynchronize(c->ev[4]);
+    float ms;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->timers[0] += ms * 1e-3;
+    (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+    c->timers[1] += ms * 1e-3;
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+    c->timers[3] += ms * 1e-3;
+    (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
+    c->timers[5] += ms * 1e-3;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mcdfci_last_error(const mcdfci_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** out) {
+  if (!ham || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
+  *out = nullptr;
+  if (ham->norb <= 0 || !ham->h || !ham->eri) return fail(nullptr, MCDFCI_E_INPUT, "invalid Hamiltonian");
+  if (ham->norb > kMaxOrb)
+    return fail(nullptr, MCDFCI_E_UNSUPPORTED,
+                "the device path supports up to " + std::to_string(kMaxOrb) + " spatial orbitals");
+  if (opts->num_of_coordinate < 1 || opts->num_of_coordinate > kMaxK)
+    return fail(nullptr, MCDFCI_E_UNSUPPORTED, "num_of_coordinate must be in [1, " + std::to_string(kMaxK) + "]");
+  if (opts->estimate < 1) return fail(nullptr, MCDFCI_E_INPUT, "estimate must be positive");
+  if (!(opts->max_memory > 0.0))
+    return fail(nullptr, MCDFCI_E_INPUT,
+                "solver:cdfci:max_memory is invalid. It is the maximum memory (GB) allowed for the "
+                "wavefunction. Please set a positive number.");
+  if (opts->threshold < 0.0) return fail(nullptr, MCDFCI_E_UNSUPPORTED, "hamiltonian:threshold must be >= 0");
+  mcdfci_ctx* c = new mcdfci_ctx();
+  c->opts = *opts;
+  c->k = opts->num_of_coordinate;
+  c->M = opts->estimate;
+  c->device = opts->device;
+  SpatialHam& S = c->ham;
+  S.norb = ham->norb;
+  S.nelec = ham->nelec;
+  S.ms2 = ham->ms2;
+  S.core = ham->core;
+  const size_t n2 = (size_t)S.norb * S.norb;
+  S.h.assign(ham->h, ham->h + n2);
+  S.eri.assign(ham->eri, ham->eri + n2 * n2);
+  if (ham->orbital_energy)
+    for (int p = 0; p < S.norb; ++p) {
+      S.orb_e.push_back({2 * p, ham->orbital_energy[p]});
+      S.orb_e.push_back({2 * p + 1, ham->orbital_energy[p]});
+    }
+  int rc = MCDFCI_OK;
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) rc = fail(c, MCDFCI_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  if (!rc) {
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = fail(c, MCDFCI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if (!rc) rc = build_tables(c);
+  if (!rc) {
+    table_size(opts->max_memory, 1, c->nslots, c->max_size, c->table_pow);
+    rc = dalloc(c, &c->tab, c->nslots);
+  }
+  if (!rc) rc = dalloc(c, &c->st, 1);
+  if (!rc) rc = dalloc(c, &c->st_scratch, 1);
+  if (!rc)
+    for (int i = 0; i < 7 && !rc; ++i)
+      if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(c, MCDFCI_E_HIP, "hipEventCreate failed");
+  if (rc) {
+    g_err = c->err;
+    mcdfci_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MCDFCI_OK;
+}
+
+int mcdfci_create_from_fcidump(const char* path, const mcdfci_opts* opts, mcdfci_ctx** out) {
+  if (!path) return fail(nullptr, MCDFCI_E_INPUT, "null path");
+  SpatialHam S;
+  try {
+    S = read_fcidump(path, nullptr);
+  } catch (const std::exception& ex) {
+    return fail(nullptr, MCDFCI_E_INPUT, ex.what());
+  }
+  std::vector<double> oe;
+  mcdfci_ham h{};
+  h.norb = S.norb;
+  h.nelec = S.nelec;
+  h.ms2 = S.ms2;
+  h.core = S.core;
+  h.h = S.h.data();
+  h.eri = S.eri.data();
+  h.orbital_energy = nullptr;
+  int rc = mcdfci_create(&h, opts, out);
+  if (rc == MCDFCI_OK) (*out)->ham.orb_e = S.orb_e;  // keep per-spin-orbital energies as read
+  return rc;
+}
+
+int mcdfci_destroy(mcdfci_ctx* c) {
+  if (!c) return MCDFCI_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MCDFCI_OK;
+}
+
+int mcdfci_info(const mcdfci_ctx* c, int64_t* info) {
+  if (!c || !info) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
+  info[0] = 1;
+  info[1] = 2 * c->ham.norb;
+  info[2] = c->ham.nelec;
+  info[3] = c->ham.ms2;
+  info[4] = c->dh.n_enum;
+  info[5] = c->k;
+  return MCDFCI_OK;
+}
+
+int mcdfci_hartree_fock(mcdfci_ctx* c, uint64_t* det) {
+  if (!c || !det) return fail(c, MCDFCI_E_INPUT, "null argument");
+  uint64_t d[2];
+  hartree_fock(c->ham, d, nullptr);
+  det[0] = d[0];
+  return MCDFCI_OK;
+}
+
+int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
+  if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
+  HIPCHECK(c, hipSetDevice(c->device));
+  uint64_t det;
+  if (ref_det) {
+    det = ref_det[0];
+  } else {
+    uint64_t d[2];
+    hartree_fock(c->ham, d, nullptr);
+    det = d[0];
+  }
+  const int ns = 2 * c->ham.norb;
+  if (ns < 64 && (det >> ns) != 0) return fail(c, MCDFCI_E_INPUT, "reference determinant has bits beyond NORB");
+  int na, nb;
+  det_sector(det, na, nb);
+  if (na + nb != c->ham.nelec)
+    return fail(c, MCDFCI_E_INPUT,
+                "Reference determinant is invalid. The number of valid orbitals does not match the number of "
+                "electrons.");
+  set_sector(c->dh, na, nb);
+  int rc = alloc_iteration(c);
+  if (rc) return rc;
+  HIPCHECK(c, hipMemsetAsync(c->tab, 0, c->nslots * sizeof(Slot), c->stream));
+  DevState s{};
+  s.z_thr = c->opts.z_threshold;
+  s.max_size = c->max_size;
+  HIPCHECK(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  launch_init_ref(c->stream, c->st, det, c->tab, c->nslots - 1, c->pdet, c->pslot, c->px, c->pz, c->dx);
+  launch_submatrix(c->stream, c->dh, c->st, c->pdet, 1, c->Hs, c->k);
+  launch_update_z(c, 1, 0);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  for (double& t : c->timers) t = 0.0;
+  c->ready = true;
+  return check_state_error(c, s);
+}
+
+int mcdfci_step(mcdfci_ctx* c, int64_t n) {
+  if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
+  if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  HIPCHECK(c, hipSetDevice(c->device));
+  for (int64_t i = 0; i < n; ++i) launch_iteration(c);
+  HIPCHECK(c, hipGetLastError());
+  return MCDFCI_OK;
+}
+
+int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {
+  if (!c || !r) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  i128 xz = (((i128)s.xz_hi) << 64) | (i128)s.xz_lo;
+  dd xx{s.xx_hi, s.xx_lo};
+  std::memset(r, 0, sizeof(*r));
+  r->energy = dd_div(fix_to_dd(xz), xx).hi;
+  r->dx_max = s.dx_max;
+  r->x_size = s.x_size;
+  r->z_size = s.z_size;
+  r->cols_size = s.cols_size;
+  r->updates = s.updates;
+  r->scale = s.sc_hi;
+  r->xx = s.xx_hi;
+  r->xz = fix_to_d(xz);
+  r->iterations = s.iterations;
+  r->max_residual = s.max_residual;
+  r->curr_max_abs_evec = s.curr_max_abs_evec;
+  r->sum_iterations = s.sum_iterations;
+  r->sum_inv_iter = s.sum_inv_iter;
+  r->curr_inv_iter = s.curr_inv_iter;
+  r->sum_diff_inv_iter = s.sum_diff_inv;
+  r->curr_diff_inv_iter = s.curr_diff_inv;
+  r->max_diff_inv_iter = s.max_diff_inv;
+  r->table_slots = (int64_t)c->nslots;
+  r->max_size = c->max_size;
+  return check_state_error(c, s);
+}
+
+int mcdfci_set_timing(mcdfci_ctx* c, int enable) {
+  if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
+  c->timing = enable != 0;
+  return MCDFCI_OK;
+}
+
+int mcdfci_timers(mcdfci_ctx* c, double* t6) {
+  if (!c || !t6) return fail(c, MCDFCI_E_INPUT, "null argument");
+  for (int i = 0; i < 6; ++i) t6[i] = c->timers[i];
+  return MCDFCI_OK;
+}
+
+int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* vals, int64_t cap, int64_t* n) {
+  if (!c || !det || !n) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DevHam H = c->dh;
+  int na, nb;
+  det_sector(det[0], na, nb);
+  set_sector(H, na, nb);
+  uint64_t* dd_ = nullptr;
+  double* dh_ = nullptr;
+  double* dv = nullptr;
+  unsigned long long* cnt = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &dd_, 1))) return rc;
+  if ((rc = dalloc(c, &dh_, 1))) return rc;
+  const long long ncap = H.n_enum;
+  uint64_t* od = nullptr;
+  if ((rc = dalloc(c, &od, ncap))) return rc;
+  if ((rc = dalloc(c, &dv, ncap))) return rc;
+  if ((rc = dalloc(c, &cnt, 1))) return rc;
+  HIPCHECK(c, hipMemcpy(dd_, det, 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemset(cnt, 0, 8));
+  launch_submatrix(c->stream, H, nullptr, dd_, 1, dh_, 1);
+  double diag;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&diag, dh_, 8, hipMemcpyDeviceToHost));
+  launch_column_dump(c->stream, H, det[0], diag, od, dv, cnt, ncap);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  unsigned long long m;
+  HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> hd(m);
+  std::vector<double> hv(m);
+  HIPCHECK(c, hipMemcpy(hd.data(), od, m * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hv.data(), dv, m * 8, hipMemcpyDeviceToHost));
+  for (void* p : {(void*)dd_, (void*)dh_, (void*)od, (void*)dv, (void*)cnt}) dfree(c, p);
+  std::vector<size_t> order(m);
+  for (size_t i = 0; i < m; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    bool da = hd[a] == det[0], db = hd[b] == det[0];
+    if (da != db) return da;
+    return hd[a] < hd[b];
+  });
+  *n = (int64_t)m;
+  for (size_t i = 0; i < m && (int64_t)i < cap; ++i) {
+    dets[i] = hd[order[i]];
+    vals[i] = hv[order[i]];
+  }
+  return MCDFCI_OK;
+}
+
+int mcdfci_submatrix(mcdfci_ctx* c, int k, const uint64_t* dets, double* Hout) {
+  if (!c || !dets || !Hout || k < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  uint64_t* d = nullptr;
+  double* h = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &d, k))) return rc;
+  if ((rc = dalloc(c, &h, (size_t)k * k))) return rc;
+  HIPCHECK(c, hipMemcpy(d, dets, 8 * (size_t)k, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemset(h, 0, 8 * (size_t)k * k));
+  launch_submatrix(c->stream, c->dh, nullptr, d, k, h, k);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(Hout, h, 8 * (size_t)k * k, hipMemcpyDeviceToHost));
+  dfree(c, d);
+  dfree(c, h);
+  return MCDFCI_OK;
+}
+
+int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, const double* z, double* s,
+                      double* dxo) {
+  if (!c || !Hin || !x || !z || !s || !dxo || k < 1 || k > kMaxK) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  double *h = nullptr, *dxx = nullptr, *dz = nullptr, *ddx = nullptr, *A = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &h, (size_t)k * k))) return rc;
+  if ((rc = dalloc(c, &dxx, k))) return rc;
+  if ((rc = dalloc(c, &dz, k))) return rc;
+  if ((rc = dalloc(c, &ddx, k))) return rc;
+  if ((rc = dalloc(c, &A, (size_t)(k + 1) * (k + 1)))) return rc;
+  HIPCHECK(c, hipMemcpy(h, Hin, 8 * (size_t)k * k, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(dxx, x, 8 * (size_t)k, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(dz, z, 8 * (size_t)k, hipMemcpyHostToDevice));
+  DevState st{};
+  st.xx_hi = s[0];
+  st.xx_lo = s[1];
+  st.sc_hi = s[2];
+  st.sc_lo = s[3];
+  i128 xz = to_fix(s[4]) + to_fix(s[5]);
+  st.xz_hi = (int64_t)(xz >> 64);
+  st.xz_lo = (uint64_t)xz;
+  HIPCHECK(c, hipMemcpy(c->st_scratch, &st, sizeof(st), hipMemcpyHostToDevice));
+  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, A, nullptr, nullptr, 1);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&st, c->st_scratch, sizeof(st), hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(dxo, ddx, 8 * (size_t)k, hipMemcpyDeviceToHost));
+  for (void* p : {(void*)h, (void*)dxx, (void*)dz, (void*)ddx, (void*)A}) dfree(c, p);
+  s[0] = st.xx_hi;
+  s[1] = st.xx_lo;
+  s[2] = st.sc_hi;
+  s[3] = st.sc_lo;
+  if (st.error == ERR_NUMERIC) return fail(c, MCDFCI_E_NUMERIC, "non-negative subspace eigenvalue");
+  return MCDFCI_OK;
+}
+
+int mcdfci_table_dump(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n) {
+  if (!c || !n) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  long long ub = (long long)std::min<uint64_t>(c->nslots, (uint64_t)(s.z_size + s.x_size + 16 + c->k));
+  uint64_t* od = nullptr;
+  double* ox = nullptr;
+  int64_t* oz = nullptr;
+  unsigned long long* cnt = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &od, ub))) return rc;
+  if ((rc = dalloc(c, &ox, ub))) return rc;
+  if ((rc = dalloc(c, &oz, 2 * ub))) return rc;
+  if ((rc = dalloc(c, &cnt, 1))) return rc;
+  HIPCHECK(c, hipMemset(cnt, 0, 8));
+  launch_table_compact(c->stream, c->tab, c->nslots, od, ox, oz, cnt, ub);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  unsigned long long m;
+  HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
+  if ((long long)m > ub) m = ub;
+  std::vector<uint64_t> hd(m);
+  std::vector<double> hx(m);
+  std::vector<int64_t> hz(2 * m);
+  HIPCHECK(c, hipMemcpy(hd.data(), od, m * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hx.data(), ox, m * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hz.data(), oz, m * 16, hipMemcpyDeviceToHost));
+  for (void* p : {(void*)od, (void*)ox, (void*)oz, (void*)cnt}) dfree(c, p);
+  std::vector<size_t> order(m);
+  for (size_t i = 0; i < m; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return hd[a] < hd[b]; });
+  *n = (int64_t)m;
+  for (size_t i = 0; i < m && (int64_t)i < cap; ++i) {
+    size_t j = order[i];
+    if (dets) dets[i] = hd[j];
+    if (x) x[i] = hx[j];
+    if (z) z[i] = fix_to_d(fix_join(hz[2 * j], hz[2 * j + 1]));
+  }
+  return MCDFCI_OK;
+}
+
+int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n) {
+  if (!c || !n) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  std::vector<int> cnt(c->k);
+  std::vector<CandEnt> ce((size_t)c->k * c->M);
+  HIPCHECK(c, hipMemcpy(cnt.data(), c->cand_cnt, sizeof(int) * c->k, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(ce.data(), c->cand, sizeof(CandEnt) * ce.size(), hipMemcpyDeviceToHost));
+  int64_t m = 0;
+  for (int t = 0; t < s.k_prev; ++t)
+    for (int i = 0; i < cnt[t]; ++i) {
+      const CandEnt& e = ce[(size_t)t * c->M + i];
+      Slot sl;
+      HIPCHECK(c, hipMemcpy(&sl, c->tab + e.slot, sizeof(Slot), hipMemcpyDeviceToHost));
+      if (m < cap) {
+        if (dets) dets[m] = e.det;
+        if (x) x[m] = sl.x;
+        if (z) z[m] = fix_to_d(fix_join(sl.za, sl.zb));
+      }
+      ++m;
+    }
+  *n = m;
+  return MCDFCI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Device-side data layout shared by the kernels and the host engine.
+//
+// HBM layout (one context):
+//   table      : 2^p slots of {key, x, zA, zB} (32 B, 4 slots per 128 B line),
+//                open addressing with bucketed linear probing; key 0 = empty.
+//                z = (zA * 2^32 + zB) * 2^-72 exactly; x = -0.0 marks a slot
+//                inserted during the current iteration (cleared in pass 2).
+//   integrals  : dense spatial (pq|rs) [M^4] and h [M^2] in fp64 and fp32,
+//                single-excitation sums S_same/S_opp [M^3], diagonal terms
+//                per spin-orbital; all read through L2/MALL.
+//   iteration  : picked dets/slots/x/z, dx, H_sub (k x k), per-column
+//                visibility bitmaps, per-chunk top-M staging lists and
+//                exact partial sums, the candidate lists.
+#pragma once
+
+#include <cstdint>
+
+#include "num.h"
+
+namespace mc {
+
+constexpr int kWG = 256;               // threads per workgroup
+constexpr int kEPT = 4;                // column elements per thread
+constexpr int kChunk = kWG * kEPT;     // column elements per workgroup
+constexpr int kSortCap = 4096;         // entries sorted by one workgroup
+constexpr int kMaxK = 256;             // coordinates per iteration
+constexpr int kMaxOrb = 32;            // spatial orbitals (one 64-bit word)
+constexpr uint64_t kMark = 0x8000000000000000ULL;  // bits of -0.0
+
+struct Slot {
+  uint64_t key;
+  double x;
+  int64_t za;
+  int64_t zb;
+};
+
+// Integral tables and column enumeration layout for one (n_alpha, n_beta)
+// sector.  Column element index e:
+//   0                      diagonal
+//   1 + [0, n_sa)          alpha singles   (occ o, virt u) -> o*nva + u
+//   off_sb + [0, n_sb)     beta singles
+//   off_daa + [0, n_daa)   alpha-alpha doubles (occ pair, virt pair)
+//   off_dbb + [0, n_dbb)   beta-beta doubles
+//   off_dab + [0, n_dab)   alpha-beta doubles ((oa*nb + ob)*nva + va)*nvb + vb
+struct DevHam {
+  int M, ns, na, nb, nva, nvb, ne;
+  int cpa_o, cpb_o, cpa_v, cpb_v;  // pair counts C(na,2), C(nb,2), C(nva,2), C(nvb,2)
+  int n_sa, n_sb, n_daa, n_dbb, n_dab;
+  int off_sb, off_daa, off_dbb, off_dab, n_enum;
+  int mixed;
+  double core, thr;
+  const double* eri;
+  const float* eri32;
+  const double* h1;
+  const float* h132;
+  const double* s_same;  // [p][r][k] = (pr|kk) - (pk|kr)
+  const double* s_opp;   // [p][r][k] = (pr|kk)
+  const float* s_same32;
+  const float* s_opp32;
+  const uint8_t* single_ok;  // [p][r]
+  const double* d1;          // [spin-orbital] h_ii
+  const double* d2;          // [i*ns + j] <ij||ij>, i < j
+};
+
+// Device-resident solver state (scalars and counters).
+struct DevState {
+  double xx_hi, xx_lo, sc_hi, sc_lo;
+  int64_t xz_hi;   // x.z as a fixed-point 128-bit value
+  uint64_t xz_lo;
+  long long x_size, z_size, updates, cols_size, iterations;
+  long long it_new, it_upd, it_cols;  // counters of the running iteration
+  int k_cur;       // columns of the running update
+  int k_prev;      // columns whose candidates feed the next pick
+  int error;
+  unsigned int ticket;
+  double z_thr;
+  double max_size;
+  double dx_max;
+  double max_residual, curr_max_abs_evec, sum_diff_inv, max_diff_inv, curr_diff_inv;
+  long long sum_iterations, sum_inv_iter, curr_inv_iter;
+};
+
+enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9 };
+
+struct StageEnt {
+  uint64_t val;   // bits of |x*xss+z| (non-negative double)
+  uint64_t det;
+  uint32_t slot;  // table slot, or 0xffffffff when absent
+  uint32_t pad;
+};
+
+struct CandEnt {
+  uint64_t det;
+  uint32_t slot;
+  uint32_t pad;
+};
+
+}  // namespace mc

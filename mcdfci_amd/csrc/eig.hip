@@ -1,0 +1,455 @@
+// Subspace update of the CDFCI iteration (src/solver.h:333-609) and
+// update_x (src/wavefunction.h:425-451) in one workgroup.
+//
+// The (k+1)-dimensional problem [alpha b^T; b H_sub] is solved for its
+// lowest eigenpair by
+//   1. Householder tridiagonalisation (dsytd2 conventions; the trailing
+//      block is updated column-per-thread, which keeps it bitwise symmetric),
+//   2. 256-point Sturm multisection per round for the lowest eigenvalue,
+//   3. three inverse-iteration solves on the tridiagonal matrix (double),
+//   4. the extended-precision refinement step of the reference
+//      (src/solver.h:437-503) as one double-double inverse-iteration solve
+//      with shift lambda - 1e-14, back-transformed in double-double.
+// Sequential sums run in ascending order; parallel sums use the 64-lane
+// order (strided partials, then xor butterfly), evaluated redundantly by
+// every wave so that all threads hold the same bits.
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace mc {
+
+constexpr int kEigLdsN = 90;  // A lives in LDS up to this dimension
+constexpr int kNMax = kMaxK + 1;
+
+template <class F>
+__device__ __forceinline__ double red64(int n, F term) {
+  const int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 64) acc = acc + term(i);
+  for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+  return acc;
+}
+
+template <class F>
+__device__ __forceinline__ dd red64_dd(int n, F term) {
+  const int lane = threadIdx.x & 63;
+  dd acc{0.0, 0.0};
+  for (int i = lane; i < n; i += 64) acc = dd_add(acc, term(i));
+  for (int off = 32; off >= 1; off >>= 1) {
+    dd o{__shfl_xor(acc.hi, off, 64), __shfl_xor(acc.lo, off, 64)};
+    acc = dd_add(acc, o);
+  }
+  return acc;
+}
+
+struct EigLds {
+  double x[kMaxK], z[kMaxK], Hx[kMaxK];
+  double v[kNMax], p[kNMax], w[kNMax], col[kNMax];
+  double d[kNMax], e[kNMax], tau[kNMax], y[kNMax];
+  double t0[kNMax], t1[kNMax], t2[kNMax], t3[kNMax];
+  dd q[kNMax];
+  dd u0[kNMax], u1[kNMax], u2[kNMax], u3[kNMax];
+  double ev[kNMax], dl[kNMax], Ae[kNMax];
+  double sc[16];
+  int iv[8];
+  int wmin[4];
+};
+
+__device__ void tri_solve_d(int n, const double* d, const double* e, double shift, double* b, double* dg,
+                            double* du, double* dl, double* du2) {
+  const double tiny = 1e-290;
+  for (int i = 0; i < n; ++i) dg[i] = d[i] - shift;
+  for (int i = 0; i + 1 < n; ++i) {
+    du[i] = e[i];
+    dl[i] = e[i];
+  }
+  for (int i = 0; i + 2 < n; ++i) du2[i] = 0.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (__builtin_fabs(dg[i]) >= __builtin_fabs(dl[i])) {
+      if (dg[i] == 0.0) dg[i] = tiny;
+      double fact = dl[i] / dg[i];
+      dg[i + 1] = dg[i + 1] - fact * du[i];
+      b[i + 1] = b[i + 1] - fact * b[i];
+      if (i < n - 2) du2[i] = 0.0;
+    } else {
+      double fact = dg[i] / dl[i];
+      double ndgi = dl[i];
+      double ndui = dg[i + 1];
+      double ndgi1 = du[i] - fact * dg[i + 1];
+      if (i < n - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -fact * du[i + 1];
+      }
+      dg[i] = ndgi;
+      du[i] = ndui;
+      dg[i + 1] = ndgi1;
+      double bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = bi - fact * bi1;
+    }
+  }
+  if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
+  b[n - 1] = b[n - 1] / dg[n - 1];
+  if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
+  for (int i = n - 3; i >= 0; --i) {
+    double t = b[i] - du[i] * b[i + 1];
+    t = t - du2[i] * b[i + 2];
+    b[i] = t / dg[i];
+  }
+}
+
+__device__ void tri_solve_dd(int n, const double* d, const double* e, double shift, dd* b, dd* dg, dd* du,
+                             dd* dl, dd* du2) {
+  const dd tiny{1e-290, 0.0};
+  for (int i = 0; i < n; ++i) dg[i] = two_sum(d[i], -shift);
+  for (int i = 0; i + 1 < n; ++i) {
+    du[i] = mkdd(e[i]);
+    dl[i] = mkdd(e[i]);
+  }
+  for (int i = 0; i + 2 < n; ++i) du2[i] = dd{0.0, 0.0};
+  for (int i = 0; i + 1 < n; ++i) {
+    if (dd_ge(dd_abs(dg[i]), dd_abs(dl[i]))) {
+      if (dd_zero_p(dg[i])) dg[i] = tiny;
+      dd fact = dd_div(dl[i], dg[i]);
+      dg[i + 1] = dd_sub(dg[i + 1], dd_mul(fact, du[i]));
+      b[i + 1] = dd_sub(b[i + 1], dd_mul(fact, b[i]));
+      if (i < n - 2) du2[i] = dd{0.0, 0.0};
+    } else {
+      dd fact = dd_div(dg[i], dl[i]);
+      dd ndgi = dl[i];
+      dd ndui = dg[i + 1];
+      dd ndgi1 = dd_sub(du[i], dd_mul(fact, dg[i + 1]));
+      if (i < n - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = dd_neg(dd_mul(fact, du[i + 1]));
+      }
+      dg[i] = ndgi;
+      du[i] = ndui;
+      dg[i + 1] = ndgi1;
+      dd bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = dd_sub(bi, dd_mul(fact, bi1));
+    }
+  }
+  if (dd_zero_p(dg[n - 1])) dg[n - 1] = tiny;
+  b[n - 1] = dd_div(b[n - 1], dg[n - 1]);
+  if (n >= 2) b[n - 2] = dd_div(dd_sub(b[n - 2], dd_mul(du[n - 2], b[n - 1])), dg[n - 2]);
+  for (int i = n - 3; i >= 0; --i) {
+    dd t = dd_sub(b[i], dd_mul(du[i], b[i + 1]));
+    t = dd_sub(t, dd_mul(du2[i], b[i + 2]));
+    b[i] = dd_div(t, dg[i]);
+  }
+}
+
+__device__ __forceinline__ int sturm_count(int n, const double* d, const double* e2, double pivmin, double x) {
+  double q = d[0] - x;
+  if (__builtin_fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (__builtin_fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// Lowest eigenpair of the symmetric n x n matrix in A (destroyed).  Leaves
+// lambda in L.sc[0] and the normalised dd eigenvector in L.q.
+__device__ void lowest_eigpair(EigLds& L, double* A, int n) {
+  const int tid = threadIdx.x;
+  const int bs = blockDim.x;
+  if (n == 1) {
+    if (tid == 0) {
+      L.sc[0] = A[0];
+      L.q[0] = dd{1.0, 0.0};
+    }
+    __syncthreads();
+    return;
+  }
+  for (int s = 0; s <= n - 3; ++s) {
+    const int m = n - s - 1;
+    for (int i = tid; i < m; i += bs) L.col[i] = A[s * n + s + 1 + i];
+    __syncthreads();
+    const double alpha = L.col[0];
+    const double xnorm2 = red64(m - 1, [&](int i) {
+      double t = L.col[1 + i];
+      return t * t;
+    });
+    if (tid == 0) L.d[s] = A[s * n + s];
+    if (xnorm2 == 0.0) {
+      if (tid == 0) {
+        L.tau[s] = 0.0;
+        L.e[s] = alpha;
+      }
+      __syncthreads();
+      continue;
+    }
+    const double beta = -__builtin_copysign(__builtin_sqrt(alpha * alpha + xnorm2), alpha);
+    const double tt = (beta - alpha) / beta;
+    const double scl = 1.0 / (alpha - beta);
+    for (int i = tid; i < m; i += bs) {
+      double vi = (i == 0) ? 1.0 : L.col[i] * scl;
+      L.v[i] = vi;
+      A[s * n + s + 1 + i] = vi;
+    }
+    if (tid == 0) {
+      L.tau[s] = tt;
+      L.e[s] = beta;
+    }
+    __syncthreads();
+    for (int r = tid; r < m; r += bs) {
+      double acc = 0.0;
+      const double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
+      for (int c = 0; c < m; ++c) acc = acc + cp[(size_t)c * n] * L.v[c];
+      L.p[r] = tt * acc;
+    }
+    __syncthreads();
+    const double K = (-0.5 * tt) * red64(m, [&](int i) { return L.p[i] * L.v[i]; });
+    for (int r = tid; r < m; r += bs) L.w[r] = L.p[r] + K * L.v[r];
+    __syncthreads();
+    for (int r = tid; r < m; r += bs) {
+      const double vr = L.v[r], wr = L.w[r];
+      double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
+      for (int c = 0; c < m; ++c) {
+        double a = cp[(size_t)c * n];
+        cp[(size_t)c * n] = a - (L.v[c] * wr + L.w[c] * vr);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    L.d[n - 2] = A[(n - 2) * n + (n - 2)];
+    L.d[n - 1] = A[(n - 1) * n + (n - 1)];
+    L.e[n - 2] = A[(n - 1) * n + (n - 2)];
+    // Gershgorin interval and pivot guard for the Sturm counts.
+    double emax = 1.0;
+    for (int i = 0; i + 1 < n; ++i) {
+      L.t0[i] = L.e[i] * L.e[i];
+      emax = fmax(emax, L.t0[i]);
+    }
+    double lo = L.d[0], hi = L.d[0];
+    for (int i = 0; i < n; ++i) {
+      double r = 0.0;
+      if (i > 0) r = r + __builtin_fabs(L.e[i - 1]);
+      if (i < n - 1) r = r + __builtin_fabs(L.e[i]);
+      lo = fmin(lo, L.d[i] - r);
+      hi = fmax(hi, L.d[i] + r);
+    }
+    double pad = (hi - lo) * 1e-6 + 1e-200;
+    L.sc[1] = lo - pad;
+    L.sc[2] = hi + pad;
+    L.sc[3] = 2.2250738585072014e-308 * emax;
+    L.iv[0] = 0;
+  }
+  __syncthreads();
+  // Multisection: thread t evaluates the count at lo + (t+1) * (hi-lo)/257.
+  for (int round = 0; round < 16; ++round) {
+    const double lo = L.sc[1], hi = L.sc[2];
+    const double w = (hi - lo) / 257.0;
+    bool hit = false;
+    if (tid < 256) {
+      const double xt = lo + (double)(tid + 1) * w;
+      hit = sturm_count(n, L.d, L.t0, L.sc[3], xt) >= 1;
+    }
+    unsigned long long bal = __ballot(hit);
+    if ((tid & 63) == 0) L.wmin[tid >> 6] = bal ? (tid + __ffsll((long long)bal) - 1) : 256;
+    __syncthreads();
+    if (tid == 0) {
+      int j = 256;
+      for (int q = 0; q < (int)(bs >> 6); ++q) j = min(j, L.wmin[q]);
+      double nlo = (j == 0) ? lo : lo + (double)j * w;
+      double nhi = (j == 256) ? hi : lo + (double)(j + 1) * w;
+      if (nlo == lo && nhi == hi) {
+        L.iv[0] = 1;
+      } else {
+        L.sc[1] = nlo;
+        L.sc[2] = nhi;
+      }
+    }
+    __syncthreads();
+    if (L.iv[0]) break;
+  }
+  if (tid == 0) {
+    const double lo = L.sc[1], hi = L.sc[2];
+    const double lam = lo + 0.5 * (hi - lo);
+    L.sc[0] = lam;
+    for (int i = 0; i < n; ++i) L.y[i] = 1.0;
+    for (int it = 0; it < 3; ++it) {
+      tri_solve_d(n, L.d, L.e, lam, L.y, L.t0, L.t1, L.t2, L.t3);
+      double mx = 0.0;
+      for (int i = 0; i < n; ++i) mx = fmax(mx, __builtin_fabs(L.y[i]));
+      for (int i = 0; i < n; ++i) L.y[i] = L.y[i] / mx;
+    }
+    for (int i = 0; i < n; ++i) L.q[i] = mkdd(L.y[i]);
+    tri_solve_dd(n, L.d, L.e, lam - 1e-14, L.q, L.u0, L.u1, L.u2, L.u3);
+  }
+  __syncthreads();
+  // Back-transform q <- H_0 ... H_{n-3} q in double-double.
+  for (int s = n - 3; s >= 0; --s) {
+    const double ts = L.tau[s];
+    if (ts == 0.0) continue;
+    const int m = n - s - 1;
+    const double* vs = A + (size_t)s * n + s + 1;
+    const dd r = red64_dd(m, [&](int i) { return dd_mul_d(L.q[s + 1 + i], vs[i]); });
+    const dd t = dd_mul_d(r, ts);
+    __syncthreads();
+    for (int i = tid; i < m; i += bs) L.q[s + 1 + i] = dd_sub(L.q[s + 1 + i], dd_mul_d(t, vs[i]));
+    __syncthreads();
+  }
+  const dd nrm2 = red64_dd(n, [&](int i) { return dd_mul(L.q[i], L.q[i]); });
+  const dd nrm = dd_sqrt(nrm2);
+  __syncthreads();
+  for (int i = tid; i < n; i += bs) L.q[i] = dd_div(L.q[i], nrm);
+  __syncthreads();
+  if (tid == 0) {
+    int lead = 0;
+    while (lead < n - 1 && dd_zero_p(L.q[lead])) ++lead;
+    L.iv[1] = dd_neg_p(L.q[lead]) ? 1 : 0;
+  }
+  __syncthreads();
+  if (L.iv[1])
+    for (int i = tid; i < n; i += bs) L.q[i] = dd_neg(L.q[i]);
+  __syncthreads();
+}
+
+// mode 0: solver iteration (reads picked x/z, writes dx, updates the state
+// and the table: update_x).  mode 1: test hook (state holds the inputs; no
+// table access).
+__global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const double* Hs,
+                                             const double* px, const double* pz, double* dx,
+                                             double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
+  __shared__ EigLds L;
+  __shared__ double Alds[kEigLdsN * kEigLdsN];
+  if (st->error) return;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int i = tid; i < k; i += bs) {
+    L.x[i] = px[i];
+    L.z[i] = pz[i];
+  }
+  __syncthreads();
+  auto Hsym = [&](int r, int c) { return r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r]; };
+  for (int r = tid; r < k; r += bs) {
+    double acc = 0.0;
+    for (int c = 0; c < k; ++c) acc = acc + Hsym(r, c) * L.x[c];
+    L.Hx[r] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
+    for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
+    for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
+    for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
+    dd xx{st->xx_hi, st->xx_lo};
+    double yy = dd_add_d(xx, -xTx).hi;
+    L.sc[4] = yy;
+    L.sc[5] = __builtin_sqrt(yy);
+    dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
+    dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+    L.sc[6] = dd_div_d(a00, yy).hi;
+  }
+  __syncthreads();
+  const double yy = L.sc[4], y1 = L.sc[5], a00 = L.sc[6];
+  const bool normal = __builtin_fabs(yy) > 1e-14;
+  const int n = normal ? k + 1 : k;
+  double* A = (n <= kEigLdsN) ? Alds : Aglob;
+  auto Aorig = [&](int r, int c) -> double {
+    if (!normal) return Hsym(r, c);
+    if (r == 0 && c == 0) return a00;
+    if (r == 0) return (L.z[c - 1] - L.Hx[c - 1]) / y1;
+    if (c == 0) return (L.z[r - 1] - L.Hx[r - 1]) / y1;
+    return Hsym(r - 1, c - 1);
+  };
+  for (int idx = tid; idx < n * n; idx += bs) A[idx] = Aorig(idx / n, idx % n);
+  __syncthreads();
+  lowest_eigpair(L, A, n);
+  const double lam = L.sc[0];
+  if (normal) {
+    for (int i = tid; i < n; i += bs) {
+      L.ev[i] = L.q[i].hi;
+      L.dl[i] = L.q[i].lo;
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += bs) {
+      double acc = 0.0;
+      for (int c = 0; c < n; ++c) acc = acc + Aorig(r, c) * L.ev[c];
+      L.Ae[r] = acc;
+    }
+    __syncthreads();
+    const double res2 = red64(n, [&](int i) {
+      double t = L.Ae[i] + (-lam) * L.ev[i];
+      return t * t;
+    });
+    const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
+    const dd t2 = red64_dd(n, [&](int i) { return two_prod(L.dl[i], L.Ae[i]); });
+    const dd rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    const dd lam_q = dd_neg(rq);
+    if (!(lam_q.hi > 0.0)) {
+      if (tid == 0) st->error = ERR_NUMERIC;
+      return;
+    }
+    const dd gam = dd_div_d(L.q[0], y1);
+    for (int i = tid; i < k; i += bs) dx[i] = dd_add_d(dd_div(L.q[i + 1], gam), -L.x[i]).hi;
+    if (tid == 0) {
+      const dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
+      double mev = 0.0;
+      for (int i = 1; i < n; ++i) mev = fmax(mev, __builtin_fabs(L.ev[i]));
+      st->max_residual = fmax(st->max_residual, __builtin_sqrt(res2));
+      st->curr_max_abs_evec = mev;
+      const double sc_d = L.ev[0] / y1 * __builtin_sqrt(-lam);
+      const dd diff = dd_add_d(dd_abs(sc_q), -__builtin_fabs(sc_d));
+      const double ad = __builtin_fabs(diff.hi);
+      st->sum_iterations += 1;
+      st->sum_inv_iter += 1;
+      st->curr_inv_iter = 1;
+      st->sum_diff_inv += ad;
+      st->max_diff_inv = fmax(st->max_diff_inv, ad);
+      st->curr_diff_inv = ad;
+      const dd xx = dd_div(dd_div(lam_q, sc_q), sc_q);
+      st->sc_hi = sc_q.hi;
+      st->sc_lo = sc_q.lo;
+      st->xx_hi = xx.hi;
+      st->xx_lo = xx.lo;
+    }
+  } else {
+    if (!(lam < 0.0)) {
+      if (tid == 0) st->error = ERR_NUMERIC;
+      return;
+    }
+    const double r = __builtin_sqrt(-lam);
+    for (int i = tid; i < k; i += bs) dx[i] = L.q[i].hi * r - L.x[i];
+    if (tid == 0) {
+      st->xx_hi = -lam;
+      st->xx_lo = 0.0;
+      st->sc_hi = 1.0;
+      st->sc_lo = 0.0;
+    }
+  }
+  if (mode != 0) return;
+  __syncthreads();
+  // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
+  if (tid == 0) {
+    i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
+    long long nx = 0;
+    for (int t = 0; t < k; ++t) {
+      Slot* s = &tab[pslot[t]];
+      double xo = s->x;
+      if (!(__builtin_fabs(xo) != 0.0)) ++nx;
+      s->x = xo + dx[t];
+      xz += to_fix(L.z[t] * dx[t]);
+    }
+    st->xz_hi = (int64_t)(xz >> 64);
+    st->xz_lo = (uint64_t)xz;
+    st->x_size += nx;
+    st->k_cur = k;
+  }
+}
+
+void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
+                const double* pz, double* dx, double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
+  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kWG), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
+}
+
+}  // namespace mc

@@ -1,0 +1,917 @@
+// CDFCI device kernels for gfx950 (one 64-bit word per determinant).
+//
+// One iteration of the reference loop (src/solver.h:701-759) is six launches:
+//   k_pick       gcd_grad pick: refetch candidates, sort, take k distinct
+//                (src/solver.h:118-224)
+//   k_submatrix  H_sub by Slater-Condon rules (src/solver.h:637-670,
+//                src/hamiltonian.h:252-328)
+//   k_eig        (k+1)-dim subspace update + update_x (src/solver.h:333-609,
+//                src/wavefunction.h:425-451)
+//   k_scatter    column generation (src/hamiltonian.h:336-418) fused with the
+//                z += H dx scatter-add and insert gate (src/wavefunction.h:499-528)
+//   k_pass2      post-scatter values: new_z, x.z, per-chunk top-M
+//                (src/wavefunction.h:540-551)
+//   k_merge      per-column top-M, z reinsert, scalar bookkeeping
+//                (src/wavefunction.h:554-604)
+// Columns are never materialised: every workgroup regenerates its chunk of
+// column elements from the determinant and the L2-resident integral tables.
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace mc {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int parity1(uint64_t d, int i, int a) {
+  uint64_t m = ((1ULL << i) - 1) ^ ((1ULL << a) - 1);
+  return __popcll(d & m) & 1;
+}
+
+__device__ __forceinline__ uint64_t aload(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void zadd(Slot* s, i128 v) {
+  int64_t a, b;
+  fix_split(v, a, b);
+  if (a) __hip_atomic_fetch_add(&s->za, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (b) __hip_atomic_fetch_add(&s->zb, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double dbits(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t bitsd(double d) { return (uint64_t)__double_as_longlong(d); }
+
+// Gradient magnitude |x*s + z| with s = xx*scale^2 in double-double
+// (src/solver.h:124,211; src/wavefunction.h:475,545).
+__device__ __forceinline__ double grad_key(dd s, double x, double z) {
+  dd r = dd_add_d(dd_mul_d(s, x), z);
+  return __builtin_fabs(r.hi);
+}
+
+__device__ __forceinline__ dd load_xss(const DevState* st) {
+  dd xx{st->xx_hi, st->xx_lo}, sc{st->sc_hi, st->sc_lo};
+  return dd_mul(dd_mul(xx, sc), sc);
+}
+
+// Bucketed linear probe. Returns the slot index holding det, or -1.
+__device__ __forceinline__ long long find_slot(const Slot* tab, uint64_t mask, uint64_t det) {
+  uint64_t b = mix64(det) & mask & ~3ULL;
+  for (uint64_t n = 0; n <= mask; n += 4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      uint64_t k = tab[b + j].key;
+      if (k == det) return (long long)(b + j);
+      if (k == 0) return -1;
+    }
+    b = (b + 4) & mask;
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------ column context
+struct ColCtx {
+  uint64_t det;
+  int ne;
+  uint8_t occ[64];  // interleaved occupied spin-orbitals, ascending
+  uint8_t oa[32], ob[32], va[32], vb[32];
+  uint16_t poa[496], pob[496], pva[496], pvb[496];  // pairs x | y << 8, x < y
+};
+
+__device__ void fill_pairs(uint16_t* out, int n, int tid, int nthreads) {
+  for (int x = tid; x < n; x += nthreads) {
+    int base = x * n - x * (x + 1) / 2;
+    for (int y = x + 1; y < n; ++y) out[base + (y - x - 1)] = (uint16_t)(x | (y << 8));
+  }
+}
+
+__device__ void setup_ctx(ColCtx& C, const DevHam& H, uint64_t det) {
+  if (threadIdx.x == 0) {
+    C.det = det;
+    int ne = 0, na = 0, nb = 0, nva = 0, nvb = 0;
+    for (int s = 0; s < H.ns; ++s) {
+      bool o = (det >> s) & 1;
+      int p = s >> 1;
+      if (o) C.occ[ne++] = (uint8_t)s;
+      if (s & 1) {
+        if (o) C.ob[nb++] = (uint8_t)p;
+        else C.vb[nvb++] = (uint8_t)p;
+      } else {
+        if (o) C.oa[na++] = (uint8_t)p;
+        else C.va[nva++] = (uint8_t)p;
+      }
+    }
+    C.ne = ne;
+  }
+  __syncthreads();
+  fill_pairs(C.poa, H.na, threadIdx.x, blockDim.x);
+  fill_pairs(C.pob, H.nb, threadIdx.x, blockDim.x);
+  fill_pairs(C.pva, H.nva, threadIdx.x, blockDim.x);
+  fill_pairs(C.pvb, H.nvb, threadIdx.x, blockDim.x);
+  __syncthreads();
+}
+
+// Single excitation value h_pr + sum_{k in occ} <ik||ak> (src/hamiltonian.h:343-347),
+// summed over the occupied spin-orbitals in ascending order.
+template <bool MIXED>
+__device__ __forceinline__ double single_value(const DevHam& H, const uint8_t* occ, int ne, int p,
+                                               int r, int spin) {
+  const int M = H.M;
+  const size_t base = ((size_t)p * M + r) * M;
+  if (MIXED) {
+    float v = H.h132[p * M + r];
+    for (int x = 0; x < ne; ++x) {
+      int k = occ[x];
+      float t = ((k & 1) == spin) ? H.s_same32[base + (k >> 1)] : H.s_opp32[base + (k >> 1)];
+      v = v + t;
+    }
+    return (double)v;
+  } else {
+    double v = H.h1[p * M + r];
+    for (int x = 0; x < ne; ++x) {
+      int k = occ[x];
+      double t = ((k & 1) == spin) ? H.s_same[base + (k >> 1)] : H.s_opp[base + (k >> 1)];
+      v = v + t;
+    }
+    return v;
+  }
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double eri_val(const DevHam& H, int p, int r, int q, int s) {
+  const int M = H.M;
+  size_t i = (((size_t)p * M + r) * M + q) * M + s;
+  return MIXED ? (double)H.eri32[i] : H.eri[i];
+}
+
+template <bool MIXED>
+__device__ __forceinline__ double same_spin_val(const DevHam& H, int p, int r, int q, int s) {
+  const int M = H.M;
+  size_t i1 = (((size_t)p * M + r) * M + q) * M + s;
+  size_t i2 = (((size_t)p * M + s) * M + q) * M + r;
+  if (MIXED) return (double)(H.eri32[i1] - H.eri32[i2]);
+  return H.eri[i1] - H.eri[i2];
+}
+
+// Element e >= 1 of the column of C.det.  Returns false when the element is
+// not part of the column (threshold filter of src/hamiltonian.h:138-161,206).
+template <bool MIXED>
+__device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e, uint64_t& out,
+                                         double& V) {
+  const uint64_t D = C.det;
+  int i, j, a, b;
+  double v;
+  if (e < H.off_daa) {
+    int f = e - 1;
+    int spin, o, u;
+    if (f < H.n_sa) {
+      spin = 0;
+      o = f / H.nva;
+      u = f - o * H.nva;
+    } else {
+      spin = 1;
+      f -= H.n_sa;
+      o = f / H.nvb;
+      u = f - o * H.nvb;
+    }
+    int p = spin ? C.ob[o] : C.oa[o];
+    int r = spin ? C.vb[u] : C.va[u];
+    if (!H.single_ok[p * H.M + r]) return false;
+    v = single_value<MIXED>(H, C.occ, C.ne, p, r, spin);
+    i = 2 * p + spin;
+    a = 2 * r + spin;
+    uint64_t d1 = D ^ (1ULL << i);
+    if (parity1(d1, i, a)) v = -v;
+    out = d1 | (1ULL << a);
+    V = v;
+    return true;
+  }
+  if (e < H.off_dab) {
+    int spin, f, cv;
+    if (e < H.off_dbb) {
+      spin = 0;
+      f = e - H.off_daa;
+      cv = H.cpa_v;
+    } else {
+      spin = 1;
+      f = e - H.off_dbb;
+      cv = H.cpb_v;
+    }
+    int op = f / cv;
+    int vp = f - op * cv;
+    uint16_t po = spin ? C.pob[op] : C.poa[op];
+    uint16_t pv = spin ? C.pvb[vp] : C.pva[vp];
+    const uint8_t* ol = spin ? C.ob : C.oa;
+    const uint8_t* vl = spin ? C.vb : C.va;
+    int p = ol[po & 0xff], q = ol[po >> 8], r = vl[pv & 0xff], s = vl[pv >> 8];
+    v = same_spin_val<MIXED>(H, p, r, q, s);
+    if (!(__builtin_fabs(v) > H.thr)) return false;
+    i = 2 * p + spin;
+    j = 2 * q + spin;
+    a = 2 * r + spin;
+    b = 2 * s + spin;
+  } else {
+    int f = e - H.off_dab;
+    int vbi = f % H.nvb;
+    int t = f / H.nvb;
+    int vai = t % H.nva;
+    t = t / H.nva;
+    int obi = t % H.nb;
+    int oai = t / H.nb;
+    int p = C.oa[oai], q = C.ob[obi], r = C.va[vai], s = C.vb[vbi];
+    v = eri_val<MIXED>(H, p, r, q, s);
+    if (!(__builtin_fabs(v) > H.thr)) return false;
+    if (p <= q) {
+      i = 2 * p;
+      j = 2 * q + 1;
+      a = 2 * r;
+      b = 2 * s + 1;
+    } else {
+      i = 2 * q + 1;
+      j = 2 * p;
+      a = 2 * s + 1;
+      b = 2 * r;
+    }
+  }
+  uint64_t d1 = D ^ (1ULL << i);
+  int s1 = parity1(d1, i, a);
+  d1 |= 1ULL << a;
+  d1 ^= 1ULL << j;
+  int s2 = parity1(d1, j, b);
+  d1 |= 1ULL << b;
+  if (s1 ^ s2) v = -v;
+  out = d1;
+  V = v;
+  return true;
+}
+
+// ------------------------------------------------------------------ diagonal / off-diagonal
+// get_diagonal (src/hamiltonian.h:257-272): sequential over occupied pairs.
+__device__ double diagonal_of(const DevHam& H, uint64_t D) {
+  double v = 0.0;
+  uint64_t r = D;
+  while (r) {
+    int i = __ffsll((long long)r) - 1;
+    r &= r - 1;
+    v += H.d1[i];
+    uint64_t r2 = r;
+    while (r2) {
+      int j = __ffsll((long long)r2) - 1;
+      r2 &= r2 - 1;
+      v += H.d2[i * H.ns + j];
+    }
+  }
+  v += H.core;
+  return v;
+}
+
+// get_off_diagonal (src/hamiltonian.h:274-328), fp64.
+__device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
+  uint64_t x = D1 ^ D2;
+  int pc = __popcll(x);
+  if (pc == 2) {
+    int i = __ffsll((long long)(x & D1)) - 1;
+    int a = __ffsll((long long)(x & D2)) - 1;
+    int spin = i & 1;
+    int p = i >> 1, r = a >> 1;
+    if (!H.single_ok[p * H.M + r]) return 0.0;
+    const size_t base = ((size_t)p * H.M + r) * H.M;
+    double v = H.h1[p * H.M + r];
+    uint64_t o = D1;
+    while (o) {
+      int k = __ffsll((long long)o) - 1;
+      o &= o - 1;
+      double t = ((k & 1) == spin) ? H.s_same[base + (k >> 1)] : H.s_opp[base + (k >> 1)];
+      v = v + t;
+    }
+    uint64_t d1 = D1 ^ (1ULL << i);
+    if (parity1(d1, i, a)) v = -v;
+    return v;
+  }
+  if (pc == 4) {
+    uint64_t from = x & D1, to = x & D2;
+    int i = __ffsll((long long)from) - 1;
+    from &= from - 1;
+    int j = __ffsll((long long)from) - 1;
+    int a0 = __ffsll((long long)to) - 1;
+    to &= to - 1;
+    int a1 = __ffsll((long long)to) - 1;
+    int a, b;
+    double v;
+    if ((i & 1) == (j & 1)) {
+      a = a0;
+      b = a1;
+      v = H.eri[(((size_t)(i >> 1) * H.M + (a >> 1)) * H.M + (j >> 1)) * H.M + (b >> 1)] -
+          H.eri[(((size_t)(i >> 1) * H.M + (b >> 1)) * H.M + (j >> 1)) * H.M + (a >> 1)];
+    } else {
+      if ((a0 & 1) == (i & 1)) {
+        a = a0;
+        b = a1;
+      } else {
+        a = a1;
+        b = a0;
+      }
+      v = H.eri[(((size_t)(i >> 1) * H.M + (a >> 1)) * H.M + (j >> 1)) * H.M + (b >> 1)];
+    }
+    if ((i & 1) != (a & 1) || (j & 1) != (b & 1)) return 0.0;
+    if (!(__builtin_fabs(v) > H.thr)) return 0.0;
+    uint64_t d1 = D1 ^ (1ULL << i);
+    int s1 = parity1(d1, i, a);
+    d1 |= 1ULL << a;
+    d1 ^= 1ULL << j;
+    int s2 = parity1(d1, j, b);
+    if (s1 ^ s2) v = -v;
+    return v;
+  }
+  return 0.0;
+}
+
+// ------------------------------------------------------------------ k_submatrix
+// One workgroup per row i; H[i*ldh + j] for j >= i.
+__global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const uint64_t* dets,
+                                                   int k, double* Hs, int ldh) {
+  if (st && st->error) return;
+  int i = blockIdx.x;
+  uint64_t Di = dets[i];
+  for (int j = i + threadIdx.x; j < k; j += blockDim.x) {
+    double v = (j == i) ? diagonal_of(H, Di) : off_diagonal_of(H, Di, dets[j]);
+    Hs[i * ldh + j] = v;
+  }
+}
+
+// ------------------------------------------------------------------ wave / block reductions
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ i128 wave_sum_i128(i128 v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    int64_t hi = (int64_t)(v >> 64);
+    uint64_t lo = (uint64_t)v;
+    int64_t ohi = __shfl_xor(hi, off, 64);
+    uint64_t olo = (uint64_t)__shfl_xor((long long)lo, off, 64);
+    v += (((i128)ohi) << 64) | (i128)olo;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------ k_scatter
+template <bool MIXED>
+__global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const uint64_t* pdet,
+                                                 const double* dx, const double* Hs, int ldh,
+                                                 Slot* tab, uint64_t mask, uint64_t* vis,
+                                                 int vis_words) {
+  __shared__ ColCtx C;
+  if (st->error) return;
+  const int t = blockIdx.y;
+  const int chunk = blockIdx.x;
+  setup_ctx(C, H, pdet[t]);
+  const double dxt = dx[t];
+  const dd sc{st->sc_hi, st->sc_lo};
+  const double zthr = st->z_thr;
+  const double diag = Hs[t * ldh + t];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long long n_new = 0, n_upd = 0, n_valid = 0;
+  bool fail = false;
+  for (int r = 0; r < kEPT; ++r) {
+    const int e = chunk * kChunk + r * kWG + threadIdx.x;
+    bool visible = false;
+    if (e < H.n_enum) {
+      uint64_t D = 0;
+      double V = 0.0;
+      bool valid;
+      if (e == 0) {
+        D = C.det;
+        V = diag;
+        valid = true;
+      } else {
+        valid = gen_elem<MIXED>(H, C, e, D, V);
+      }
+      if (valid) {
+        ++n_valid;
+        const double dz = V * dxt;
+        const bool gate = dd_abs_gt(dd_mul_d(sc, dz), zthr);
+        Slot* hit = nullptr;
+        bool mine = false;
+        uint64_t b = mix64(D) & mask & ~3ULL;
+        bool done = false;
+        for (uint64_t n = 0; n <= mask && !done; n += 4) {
+          for (int jj = 0; jj < 4 && !done; ++jj) {
+            Slot* s = &tab[b + jj];
+            uint64_t k = aload(&s->key);
+            if (k == D) {
+              hit = s;
+              done = true;
+            } else if (k == 0) {
+              if (!gate) {
+                done = true;
+              } else {
+                __hip_atomic_store((uint64_t*)&s->x, kMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t expect = 0;
+                bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D, __ATOMIC_ACQ_REL,
+                                                               __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (ok) {
+                  hit = s;
+                  mine = true;
+                  done = true;
+                } else if (expect == D) {
+                  hit = s;
+                  done = true;
+                }
+              }
+            }
+          }
+          b = (b + 4) & mask;
+        }
+        if (!done) fail = true;
+        if (hit) {
+          if (gate) {
+            visible = true;
+          } else {
+            uint64_t xb = __hip_atomic_load((const uint64_t*)&hit->x, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+            visible = (xb != kMark);
+          }
+          if (mine) ++n_new;
+          if (visible) {
+            zadd(hit, to_fix(dz));
+            ++n_upd;
+          }
+        }
+      }
+    }
+    unsigned long long ball = __ballot(visible);
+    const int word = chunk * (kChunk / 64) + r * (kWG / 64) + wave;
+    if (lane == 0 && word < vis_words) vis[(size_t)t * vis_words + word] = ball;
+  }
+  n_new = wave_sum_ll(n_new);
+  n_upd = wave_sum_ll(n_upd);
+  n_valid = wave_sum_ll(n_valid);
+  if (lane == 0) {
+    if (n_new) atomicAdd((unsigned long long*)&st->it_new, (unsigned long long)n_new);
+    if (n_upd) atomicAdd((unsigned long long*)&st->it_upd, (unsigned long long)n_upd);
+    if (n_valid) atomicAdd((unsigned long long*)&st->it_cols, (unsigned long long)n_valid);
+  }
+  if (fail) st->error = ERR_PROBE;
+}
+
+// ------------------------------------------------------------------ bitonic sort in LDS
+// Sorts n (power of two) entries descending by (a, b); payload follows.
+__device__ void bitonic_desc(uint64_t* ka, uint64_t* kb, uint32_t* pl, int n) {
+  for (int size = 2; size <= n; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int j = i ^ stride;
+        if (j > i) {
+          bool desc = (i & size) == 0;
+          bool less = (ka[i] < ka[j]) || (ka[i] == ka[j] && kb[i] < kb[j]);
+          bool greater = (ka[i] > ka[j]) || (ka[i] == ka[j] && kb[i] > kb[j]);
+          if (desc ? less : greater) {
+            uint64_t t0 = ka[i];
+            ka[i] = ka[j];
+            ka[j] = t0;
+            uint64_t t1 = kb[i];
+            kb[i] = kb[j];
+            kb[j] = t1;
+            uint32_t t2 = pl[i];
+            pl[i] = pl[j];
+            pl[j] = t2;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ k_pass2
+// Values after the scatter: per-element |x*xss+z| (0 when the element does
+// not see its determinant), exact partial sums of H*x and dz*x, and the
+// chunk's top-M list sorted by (value, determinant) descending.
+template <bool MIXED>
+__global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uint64_t* pdet,
+                                               const double* dx, const double* Hs, int ldh, Slot* tab,
+                                               uint64_t mask, const uint64_t* vis, int vis_words,
+                                               int M, int nchunks, StageEnt* stage, int* stage_cnt,
+                                               int64_t* part) {
+  __shared__ ColCtx C;
+  __shared__ uint64_t ka[kChunk];
+  __shared__ uint64_t kb[kChunk];
+  __shared__ uint32_t pl[kChunk];
+  __shared__ i128 red[kWG / 64][2];
+  if (st->error) return;
+  const int t = blockIdx.y;
+  const int chunk = blockIdx.x;
+  setup_ctx(C, H, pdet[t]);
+  const double dxt = dx[t];
+  const dd s = load_xss(st);
+  const double diag = Hs[t * ldh + t];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  i128 nz = 0, xt = 0;
+  for (int r = 0; r < kEPT; ++r) {
+    const int li = r * kWG + threadIdx.x;
+    const int e = chunk * kChunk + li;
+    uint64_t keyv = 0, keyd = 0;
+    uint32_t slot = 0xffffffffu;
+    if (e < H.n_enum) {
+      uint64_t D = 0;
+      double V = 0.0;
+      bool valid;
+      if (e == 0) {
+        D = C.det;
+        V = diag;
+        valid = true;
+      } else {
+        valid = gen_elem<MIXED>(H, C, e, D, V);
+      }
+      if (valid) {
+        const double dz = V * dxt;
+        long long si = find_slot(tab, mask, D);
+        double x = 0.0, val = 0.0;
+        if (si >= 0) {
+          Slot* sl = &tab[si];
+          uint64_t xb = bitsd(sl->x);
+          x = dbits(xb);
+          if (xb == kMark) {
+            x = 0.0;
+            sl->x = 0.0;
+          }
+          const uint64_t w = vis[(size_t)t * vis_words + (e >> 6)];
+          if ((w >> (e & 63)) & 1) {
+            double z = fix_to_d(fix_join(sl->za, sl->zb));
+            val = grad_key(s, x, z);
+          }
+          slot = (uint32_t)si;
+        }
+        nz += to_fix(V * x);
+        xt += to_fix(dz * x);
+        keyv = bitsd(val);
+        keyd = D;
+      }
+    }
+    ka[li] = keyv;
+    kb[li] = keyd;
+    pl[li] = slot;
+  }
+  // Exact chunk sums.
+  nz = wave_sum_i128(nz);
+  xt = wave_sum_i128(xt);
+  if (lane == 0) {
+    red[wave][0] = nz;
+    red[wave][1] = xt;
+  }
+  bitonic_desc(ka, kb, pl, kChunk);
+  if (threadIdx.x == 0) {
+    i128 a = 0, b = 0;
+    for (int w = 0; w < kWG / 64; ++w) {
+      a += red[w][0];
+      b += red[w][1];
+    }
+    int64_t* o = part + ((size_t)t * nchunks + chunk) * 4;
+    o[0] = (int64_t)(a >> 64);
+    o[1] = (int64_t)(uint64_t)a;
+    o[2] = (int64_t)(b >> 64);
+    o[3] = (int64_t)(uint64_t)b;
+  }
+  // Valid entries have keyd != 0 and sort before invalid ones (key (0,0)).
+  const int mw = M < kChunk ? M : kChunk;
+  StageEnt* out = stage + ((size_t)t * nchunks + chunk) * mw;
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < mw; i += blockDim.x) {
+    if (kb[i] != 0) {
+      out[i].val = ka[i];
+      out[i].det = kb[i];
+      out[i].slot = pl[i];
+      out[i].pad = 0;
+      atomicAdd(&cnt, 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) stage_cnt[t * nchunks + chunk] = cnt;
+}
+
+// ------------------------------------------------------------------ k_merge
+// One workgroup per column: merge chunk lists into the column's top-M, keep
+// the determinants present in the table (the refetch of
+// src/wavefunction.h:567-575), reinsert z_t := new_z_t, and let the last
+// workgroup update x.z and the counters.
+__global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pdet, const uint32_t* pslot,
+                                               Slot* tab, int M, int nchunks, const StageEnt* stage,
+                                               const int* stage_cnt, const int64_t* part,
+                                               int64_t* col_xz, CandEnt* cand, int* cand_cnt,
+                                               const double* dx, int count_iteration) {
+  __shared__ uint64_t ka[kSortCap];
+  __shared__ uint64_t kb[kSortCap];
+  __shared__ uint32_t pl[kSortCap];
+  __shared__ int offs[1024];
+  __shared__ int total;
+  __shared__ bool last;
+  if (st->error) return;
+  const int t = blockIdx.x;
+  const int k = gridDim.x;
+  const int mw = M < kChunk ? M : kChunk;
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int c = 0; c < nchunks; ++c) {
+      offs[c] = acc;
+      acc += stage_cnt[t * nchunks + c];
+    }
+    total = acc;
+  }
+  __syncthreads();
+  int n = total;
+  if (n > kSortCap) {
+    if (threadIdx.x == 0) st->error = ERR_CAP;
+    return;
+  }
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  for (int c = 0; c < nchunks; ++c) {
+    int cc = stage_cnt[t * nchunks + c];
+    const StageEnt* src = stage + ((size_t)t * nchunks + c) * mw;
+    for (int i = threadIdx.x; i < cc; i += blockDim.x) {
+      ka[offs[c] + i] = src[i].val;
+      kb[offs[c] + i] = src[i].det;
+      pl[offs[c] + i] = src[i].slot;
+    }
+  }
+  for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
+    ka[i] = 0;
+    kb[i] = 0;
+    pl[i] = 0xffffffffu;
+  }
+  bitonic_desc(ka, kb, pl, np2);
+  // First M entries; keep those present in the table.
+  if (threadIdx.x == 0) {
+    int m = n < M ? n : M;
+    int c = 0;
+    for (int i = 0; i < m; ++i) {
+      if (pl[i] != 0xffffffffu) {
+        cand[(size_t)t * M + c].det = kb[i];
+        cand[(size_t)t * M + c].slot = pl[i];
+        cand[(size_t)t * M + c].pad = 0;
+        ++c;
+      }
+    }
+    cand_cnt[t] = c;
+    // Exact per-column sums; reinsert z of the picked determinant.
+    i128 nz = 0, xt = 0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const int64_t* o = part + ((size_t)t * nchunks + ch) * 4;
+      nz += (((i128)o[0]) << 64) | (i128)(uint64_t)o[1];
+      xt += (((i128)o[2]) << 64) | (i128)(uint64_t)o[3];
+    }
+    Slot* sp = &tab[pslot[t]];
+    sp->za = (int64_t)(nz >> 32);
+    sp->zb = (int64_t)(uint64_t)(nz & 0xffffffffULL);
+    col_xz[2 * t] = (int64_t)(xt >> 64);
+    col_xz[2 * t + 1] = (int64_t)(uint64_t)xt;
+    __threadfence();
+    unsigned int tk = atomicAdd(&st->ticket, 1u);
+    last = (tk == (unsigned int)(k - 1));
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
+  for (int c = 0; c < k; ++c) xz += (((i128)col_xz[2 * c]) << 64) | (i128)(uint64_t)col_xz[2 * c + 1];
+  st->xz_hi = (int64_t)(xz >> 64);
+  st->xz_lo = (uint64_t)xz;
+  st->z_size += st->it_new;
+  st->updates = st->it_upd;
+  st->cols_size = st->it_cols;
+  st->it_new = st->it_upd = st->it_cols = 0;
+  st->ticket = 0;
+  st->k_prev = k;
+  double dm = 0.0;
+  for (int c = 0; c < k; ++c) dm = fmax(dm, __builtin_fabs(dx[c]));
+  st->dx_max = dm;
+  if (count_iteration) st->iterations += 1;
+  if ((double)st->z_size > 0.79 * st->max_size) st->error = ERR_TABLE_FULL;
+}
+
+// ------------------------------------------------------------------ k_pick
+// gcd_grad pick (src/solver.h:118-224): refetch every candidate's (x, z),
+// sort by (|x*s+z| desc, determinant desc), keep the first k distinct.
+__global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const CandEnt* cand,
+                                               const int* cand_cnt, const Slot* tab, uint64_t* pdet,
+                                               uint32_t* pslot, double* px, double* pz) {
+  __shared__ uint64_t ka[kSortCap];
+  __shared__ uint64_t kb[kSortCap];
+  __shared__ uint32_t pl[kSortCap];
+  __shared__ int offs[kMaxK + 1];
+  __shared__ int flag[kSortCap];
+  if (st->error) return;
+  const int kp = st->k_prev;
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int c = 0; c < kp; ++c) {
+      offs[c] = acc;
+      acc += cand_cnt[c];
+    }
+    offs[kp] = acc;
+  }
+  __syncthreads();
+  const int n = offs[kp];
+  if (n > kSortCap) {
+    if (threadIdx.x == 0) st->error = ERR_CAP;
+    return;
+  }
+  const dd s = load_xss(st);
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  for (int c = 0; c < kp; ++c) {
+    for (int i = threadIdx.x; i < cand_cnt[c]; i += blockDim.x) {
+      const CandEnt ce = cand[(size_t)c * M + i];
+      const Slot& sl = tab[ce.slot];
+      double z = fix_to_d(fix_join(sl.za, sl.zb));
+      ka[offs[c] + i] = bitsd(grad_key(s, sl.x, z));
+      kb[offs[c] + i] = ce.det;
+      pl[offs[c] + i] = ce.slot;
+    }
+  }
+  for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
+    ka[i] = 0;
+    kb[i] = 0;
+    pl[i] = 0xffffffffu;
+  }
+  bitonic_desc(ka, kb, pl, np2);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) flag[i] = (i == 0 || kb[i] != kb[i - 1]) ? 1 : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int i = 0; i < n && c < k; ++i) {
+      if (!flag[i]) continue;
+      pdet[c] = kb[i];
+      pslot[c] = pl[i];
+      const Slot& sl = tab[pl[i]];
+      px[c] = sl.x;
+      pz[c] = fix_to_d(fix_join(sl.za, sl.zb));
+      ++c;
+    }
+    if (c < k) st->error = ERR_CANDIDATES;
+  }
+}
+
+// ------------------------------------------------------------------ init
+// update_x of the reference determinant with dx = 1 (src/solver.h:864-872).
+__global__ void k_init_ref(DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
+                           uint32_t* pslot, double* px, double* pz, double* dx) {
+  if (threadIdx.x != 0) return;
+  uint64_t b = mix64(det) & mask & ~3ULL;
+  long long si = -1;
+  for (uint64_t n = 0; n <= mask && si < 0; n += 4) {
+    for (int j = 0; j < 4; ++j) {
+      if (tab[b + j].key == det || tab[b + j].key == 0) {
+        si = (long long)(b + j);
+        break;
+      }
+    }
+    b = (b + 4) & mask;
+  }
+  if (si < 0) {
+    st->error = ERR_PROBE;
+    return;
+  }
+  Slot* s = &tab[si];
+  bool fresh = s->key == 0;
+  s->key = det;
+  if (fresh) {
+    s->x = 1.0;
+    s->za = 0;
+    s->zb = 0;
+  } else {
+    s->x = s->x + 1.0;
+  }
+  st->x_size = 1;
+  st->z_size = 0;
+  st->updates = 0;
+  st->xx_hi = 1.0;
+  st->xx_lo = 0.0;
+  st->sc_hi = 1.0;
+  st->sc_lo = 0.0;
+  st->xz_hi = 0;
+  st->xz_lo = 0;
+  st->k_cur = 1;
+  st->k_prev = 1;
+  st->iterations = 0;
+  pdet[0] = det;
+  pslot[0] = (uint32_t)si;
+  px[0] = 0.0;
+  pz[0] = 0.0;
+  dx[0] = 1.0;
+}
+
+// ------------------------------------------------------------------ test hooks
+template <bool MIXED>
+__global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, uint64_t det, double diag, uint64_t* odet,
+                                                     double* oval, unsigned long long* count,
+                                                     long long cap) {
+  __shared__ ColCtx C;
+  setup_ctx(C, H, det);
+  for (int r = 0; r < kEPT; ++r) {
+    int e = blockIdx.x * kChunk + r * kWG + threadIdx.x;
+    if (e >= H.n_enum) continue;
+    uint64_t D;
+    double V;
+    bool valid;
+    if (e == 0) {
+      D = det;
+      V = diag;
+      valid = true;
+    } else {
+      valid = gen_elem<MIXED>(H, C, e, D, V);
+    }
+    if (!valid) continue;
+    unsigned long long pos = atomicAdd(count, 1ULL);
+    if ((long long)pos < cap) {
+      odet[pos] = D;
+      oval[pos] = V;
+    }
+  }
+}
+
+__global__ void k_table_compact(const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox, int64_t* oz,
+                                unsigned long long* count, long long cap) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  Slot s = tab[i];
+  if (s.key == 0) return;
+  unsigned long long pos = atomicAdd(count, 1ULL);
+  if ((long long)pos < cap) {
+    odet[pos] = s.key;
+    ox[pos] = s.x;
+    oz[2 * pos] = s.za;
+    oz[2 * pos + 1] = s.zb;
+  }
+}
+
+// ------------------------------------------------------------------ launch wrappers
+void launch_submatrix(hipStream_t s, const DevHam& H, const DevState* st, const uint64_t* dets, int k,
+                      double* Hs, int ldh) {
+  hipLaunchKernelGGL(k_submatrix, dim3(k), dim3(kWG), 0, s, H, st, dets, k, Hs, ldh);
+}
+
+void launch_scatter(hipStream_t s, const DevHam& H, DevState* st, const uint64_t* pdet, const double* dx,
+                    const double* Hs, int ldh, Slot* tab, uint64_t mask, uint64_t* vis, int vis_words,
+                    int k, int nchunks) {
+  if (H.mixed)
+    hipLaunchKernelGGL(k_scatter<true>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
+                       mask, vis, vis_words);
+  else
+    hipLaunchKernelGGL(k_scatter<false>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
+                       mask, vis, vis_words);
+}
+
+void launch_pass2(hipStream_t s, const DevHam& H, DevState* st, const uint64_t* pdet, const double* dx,
+                  const double* Hs, int ldh, Slot* tab, uint64_t mask, const uint64_t* vis, int vis_words,
+                  int M, int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
+  if (H.mixed)
+    hipLaunchKernelGGL(k_pass2<true>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab, mask,
+                       vis, vis_words, M, nchunks, stage, stage_cnt, part);
+  else
+    hipLaunchKernelGGL(k_pass2<false>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
+                       mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
+}
+
+void launch_merge(hipStream_t s, DevState* st, const uint64_t* pdet, const uint32_t* pslot, Slot* tab,
+                  int M, int k, int nchunks, const StageEnt* stage, const int* stage_cnt,
+                  const int64_t* part, int64_t* col_xz, CandEnt* cand, int* cand_cnt, const double* dx,
+                  int count_iteration) {
+  hipLaunchKernelGGL(k_merge, dim3(k), dim3(kWG), 0, s, st, pdet, pslot, tab, M, nchunks, stage, stage_cnt,
+                     part, col_xz, cand, cand_cnt, dx, count_iteration);
+}
+
+void launch_pick(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
+                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz) {
+  hipLaunchKernelGGL(k_pick, dim3(1), dim3(1024), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot, px, pz);
+}
+
+void launch_init_ref(hipStream_t s, DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
+                     uint32_t* pslot, double* px, double* pz, double* dx) {
+  hipLaunchKernelGGL(k_init_ref, dim3(1), dim3(64), 0, s, st, det, tab, mask, pdet, pslot, px, pz, dx);
+}
+
+void launch_column_dump(hipStream_t s, const DevHam& H, uint64_t det, double diag, uint64_t* odet,
+                        double* oval, unsigned long long* count, long long cap) {
+  int nch = (H.n_enum + kChunk - 1) / kChunk;
+  if (H.mixed)
+    hipLaunchKernelGGL(k_column_dump<true>, dim3(nch), dim3(kWG), 0, s, H, det, diag, odet, oval, count, cap);
+  else
+    hipLaunchKernelGGL(k_column_dump<false>, dim3(nch), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
+                       cap);
+}
+
+void launch_table_compact(hipStream_t s, const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox,
+                          int64_t* oz, unsigned long long* count, long long cap) {
+  uint64_t blocks = (nslots + 255) / 256;
+  hipLaunchKernelGGL(k_table_compact, dim3((unsigned)blocks), dim3(256), 0, s, tab, nslots, odet, ox, oz,
+                     count, cap);
+}
+
+}  // namespace mc
