@@ -88,6 +88,7 @@ typedef struct {
   double max_diff_inv_iter;
   int64_t table_slots;  /* device table capacity (slots) */
   double max_size;      /* the reference's max_wavefunction_size */
+  int64_t candidates;   /* length of the candidate list for the next pick (sub_xz.size()) */
 } mcdfci_report;
 
 const char* mcdfci_last_error(const mcdfci_ctx* ctx);
@@ -99,7 +100,8 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
 int mcdfci_create_from_fcidump(const char* path, const mcdfci_opts* opts, mcdfci_ctx** out);
 int mcdfci_destroy(mcdfci_ctx* ctx);
 
-/* info[0..5] = {nwords, spin-orbitals, nelec, ms2, column enumeration size, k} */
+/* info[0..6] = {nwords, spin-orbitals, nelec, ms2, column enumeration size,
+ * k, 1 if the FCIDUMP carried orbital energies} */
 int mcdfci_info(const mcdfci_ctx* ctx, int64_t* info);
 
 /* HamiltonianMolecule::get_hartree_fock (src/hamiltonian.h:469-492). */
@@ -108,6 +110,10 @@ int mcdfci_hartree_fock(mcdfci_ctx* ctx, uint64_t* det);
 /* Outer CDFCI::solve initialisation (src/solver.h:853-873): x = e_ref,
  * z = H[:,ref], candidates = top-M of that column.  ref_det NULL -> HF. */
 int mcdfci_init(mcdfci_ctx* ctx, const uint64_t* ref_det);
+
+/* Compression threshold used by the next mcdfci_init; the reference's
+ * z_threshold_search restart doubles it (src/solver.h:882-888). */
+int mcdfci_set_z_threshold(mcdfci_ctx* ctx, double z_threshold);
 
 /* n_iter iterations of the inner loop (src/solver.h:701-759) on the device. */
 int mcdfci_step(mcdfci_ctx* ctx, int64_t n_iter);

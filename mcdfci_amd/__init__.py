@@ -62,7 +62,7 @@ class Report(C.Structure):
                 ("sum_inv_iter", C.c_int64), ("curr_inv_iter", C.c_int64),
                 ("sum_diff_inv_iter", C.c_double), ("curr_diff_inv_iter", C.c_double),
                 ("max_diff_inv_iter", C.c_double), ("table_slots", C.c_int64),
-                ("max_size", C.c_double)]
+                ("max_size", C.c_double), ("candidates", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -83,6 +83,7 @@ SIGNATURES = {
     "mcdfci_info": (C.c_int, [C.c_void_p, _i64p]),
     "mcdfci_hartree_fock": (C.c_int, [C.c_void_p, _u64p]),
     "mcdfci_init": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mcdfci_set_z_threshold": (C.c_int, [C.c_void_p, C.c_double]),
     "mcdfci_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "mcdfci_report_get": (C.c_int, [C.c_void_p, C.POINTER(Report)]),
     "mcdfci_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -133,9 +134,10 @@ class Cdfci:
         if rc:
             raise McdfciError(rc, self.lib.mcdfci_last_error(None).decode())
         self.h = h
-        info = np.zeros(6, np.int64)
+        info = np.zeros(8, np.int64)
         self._check(self.lib.mcdfci_info(self.h, info))
-        self.nwords, self.nso, self.nelec, self.ms2, self.n_enum, self.k = (int(v) for v in info)
+        self.nwords, self.nso, self.nelec, self.ms2, self.n_enum, self.k, self.has_orbital_energy = (
+            int(v) for v in info[:7])
         self.estimate = int(estimate)
 
     def _check(self, rc):

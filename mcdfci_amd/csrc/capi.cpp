@@ -437,6 +437,7 @@ int mcdfci_info(const mcdfci_ctx* c, int64_t* info) {
   info[3] = c->ham.ms2;
   info[4] = c->dh.n_enum;
   info[5] = c->k;
+  info[6] = c->ham.orb_e.empty() ? 0 : 1;
   return MCDFCI_OK;
 }
 
@@ -486,6 +487,12 @@ int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
   return check_state_error(c, s);
 }
 
+int mcdfci_set_z_threshold(mcdfci_ctx* c, double z) {
+  if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
+  c->opts.z_threshold = z;
+  return MCDFCI_OK;
+}
+
 int mcdfci_step(mcdfci_ctx* c, int64_t n) {
   if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
@@ -524,6 +531,13 @@ int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {
   r->max_diff_inv_iter = s.max_diff_inv;
   r->table_slots = (int64_t)c->nslots;
   r->max_size = c->max_size;
+  {
+    std::vector<int> cnt(c->k);
+    HIPCHECK(c, hipMemcpy(cnt.data(), c->cand_cnt, sizeof(int) * c->k, hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int t = 0; t < s.k_prev && t < c->k; ++t) tot += cnt[t];
+    r->candidates = tot;
+  }
   return check_state_error(c, s);
 }
 

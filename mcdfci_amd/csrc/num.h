@@ -122,13 +122,17 @@ MC_HD i128 to_fix(double d) {
   return (((i128)(int64_t)yh) << 32) + (i128)(int64_t)yl;
 }
 
-MC_HD double fix_to_d(i128 v) {
+MC_HD double fix_to_d_pos(i128 v) {
   int64_t hi = (int64_t)(v >> 64);
   uint64_t lo = (uint64_t)v;
   double dlo = (double)(uint32_t)(lo >> 32) * 4294967296.0 + (double)(uint32_t)lo;
   double d = (double)hi * 18446744073709551616.0 + dlo;
   return __builtin_ldexp(d, -kFixBits);
 }
+
+// Sign-magnitude conversion: the magnitude is rounded, then negated, so
+// small negative values do not cancel against a two's-complement high word.
+MC_HD double fix_to_d(i128 v) { return v < 0 ? -fix_to_d_pos(-v) : fix_to_d_pos(v); }
 
 MC_HD dd fix_to_dd(i128 v) {
   double h = fix_to_d(v);

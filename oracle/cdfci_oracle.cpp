@@ -904,6 +904,81 @@ long oc_candidates(void* p, uint64_t* dets, double* x, double* z, long cap) {
   return (long)cs.size();
 }
 
+// Wavefunction-level hooks (update_x / update_z on caller-chosen picks).
+void oc_wf_reset(void* p, int k, int M, double z_thr, double max_size, int nthreads) {
+  Ctx* c = (Ctx*)p;
+  Solver& s = c->s;
+  s.k = k;
+  s.M = M;
+  s.z_thr = z_thr;
+  s.max_size = max_size;
+  s.nthreads = nthreads > 0 ? nthreads : 1;
+  s.shards.assign(s.nthreads, Shard());
+  s.xx = dd_make(1.0);
+  s.sc = dd_make(1.0);
+  s.xz = 0;
+  s.x_size = s.z_size = s.updates = 0;
+  s.iter_stamp = 0;
+  s.error = 0;
+  s.cands.clear();
+}
+
+// s: {xx.hi, xx.lo, sc.hi, sc.lo}; xz set from a double.
+void oc_wf_set_scalars(void* p, const double* v, double xz) {
+  Ctx* c = (Ctx*)p;
+  c->s.xx = dd{v[0], v[1]};
+  c->s.sc = dd{v[2], v[3]};
+  c->s.xz = to_fixed(xz);
+}
+
+// out: {x_size, z_size, updates, xz, error}
+void oc_wf_state(void* p, double* out) {
+  Ctx* c = (Ctx*)p;
+  out[0] = (double)c->s.x_size;
+  out[1] = (double)c->s.z_size;
+  out[2] = (double)c->s.updates;
+  out[3] = fixed_to_double(c->s.xz);
+  out[4] = (double)c->s.error;
+}
+
+void oc_wf_update(void* p, int k, const uint64_t* dets, const double* dx) {
+  Ctx* c = (Ctx*)p;
+  int nw = c->h.nw;
+  std::vector<Det> pd(k);
+  for (int t = 0; t < k; ++t) det_in(dets + t * nw, nw, pd[t]);
+  std::vector<double> d(dx, dx + k);
+  c->s.update_x(pd, d);
+  c->s.cols.assign(k, {});
+  for (int t = 0; t < k; ++t) c->h.column(pd[t], c->s.cols[t]);
+  c->s.update_z(pd, d);
+}
+
+// Canonical pick on a caller-provided candidate list; returns the number picked.
+int oc_pick(void* p, long n, const uint64_t* dets, const double* x, const double* z, const double* s4, int k,
+            uint64_t* out) {
+  Ctx* c = (Ctx*)p;
+  int nw = c->h.nw;
+  Solver& s = c->s;
+  s.cands.clear();
+  for (long i = 0; i < n; ++i) {
+    Cand cd;
+    det_in(dets + i * nw, nw, cd.d);
+    cd.x = x[i];
+    cd.z = z[i];
+    s.cands.push_back(cd);
+  }
+  s.xx = dd{s4[0], s4[1]};
+  s.sc = dd{s4[2], s4[3]};
+  int kk = s.k;
+  s.k = k;
+  std::vector<Det> pd;
+  std::vector<double> px, pz;
+  s.pick(pd, px, pz);
+  s.k = kk;
+  for (size_t i = 0; i < pd.size(); ++i) det_out(pd[i], nw, out + i * nw);
+  return (int)pd.size();
+}
+
 // Numeric helpers exposed for pinning tests.
 void oc_to_fixed(double d, int64_t* out) {
   i128 v = to_fixed(d);

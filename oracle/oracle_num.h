@@ -126,6 +126,7 @@ static inline i128 to_fixed(double d) {
 }
 
 static inline double fixed_to_double(i128 v) {
+  if (v < 0) return -fixed_to_double(-v);
   int64_t hi = (int64_t)(v >> 64);
   uint64_t lo = (uint64_t)v;
   double dlo = (double)(uint32_t)(lo >> 32) * 4294967296.0 + (double)(uint32_t)lo;

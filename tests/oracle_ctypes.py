@@ -54,6 +54,11 @@ def oracle_lib():
         _sig(lib, "oc_to_fixed", None, [C.c_double, _i64p])
         _sig(lib, "oc_fixed_to_double", C.c_double, [C.c_int64, C.c_int64])
         _sig(lib, "oc_dd_ops", None, [C.c_double] * 4 + [_f64p])
+        _sig(lib, "oc_wf_reset", None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int])
+        _sig(lib, "oc_wf_set_scalars", None, [C.c_void_p, _f64p, C.c_double])
+        _sig(lib, "oc_wf_state", None, [C.c_void_p, _f64p])
+        _sig(lib, "oc_wf_update", None, [C.c_void_p, C.c_int, _u64p, _f64p])
+        _sig(lib, "oc_pick", C.c_int, [C.c_void_p, C.c_long, _u64p, _f64p, _f64p, _f64p, C.c_int, _u64p])
         _oracle = lib
     return _oracle
 

@@ -1,0 +1,216 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+The device path and the oracle implement the same canonical arithmetic, so
+every comparison here is exact (==), not a tolerance: column determinants,
+signs and values; H_sub; the subspace update; and whole trajectories --
+energy, |x|_0, |z|_0, scale, and the full wavefunction table (keys, x, z)
+after hundreds of iterations.  Mixed precision is checked against the FP64
+path with the tolerance stated in each test.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import mcdfci_amd
+from mcdfci_amd import Cdfci, McdfciError, PREC_MIXED
+from oracle_ctypes import Oracle
+
+
+def _path(name, workloads, h2o_fcidump):
+    return h2o_fcidump if name == "h2o" else workloads(name)["fcidump"]
+
+
+def _dict(dets, vals):
+    return dict(zip(dets[:, 0].tolist(), vals.tolist()))
+
+
+@pytest.mark.parametrize("name", ["c1", "c3_synth", "c3_hubbard", "h2o", "c2", "c4"])
+def test_columns_bit_exact(name, workloads, h2o_fcidump):
+    path = _path(name, workloads, h2o_fcidump)
+    print("case", name, flush=True)
+    o = Oracle(path)
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.01, threshold=0.0)
+    hf = g.hartree_fock()
+    assert np.array_equal(hf, o.hf())
+    od, ov = o.column(hf)
+    rng = np.random.default_rng(2)
+    seeds = [hf] + [od[i] for i in rng.choice(len(od), 6, replace=False)]
+    for d in seeds:
+        gd, gv = g.column(d)
+        od2, ov2 = o.column(d)
+        A, B = _dict(gd, gv), _dict(od2, ov2)
+        assert len(gd) == len(od2)
+        assert A.keys() == B.keys()
+        assert all(A[key] == B[key] for key in A)
+        assert gd[0, 0] == d[0]
+
+
+@pytest.mark.parametrize("name", ["c1", "c3_synth", "h2o"])
+def test_submatrix_bit_exact(name, workloads, h2o_fcidump):
+    path = _path(name, workloads, h2o_fcidump)
+    o = Oracle(path)
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.01, threshold=0.0)
+    od, _ = o.column(o.hf())
+    rng = np.random.default_rng(4)
+    dets = od[rng.choice(len(od), min(40, len(od)), replace=False), 0]
+    H = g.submatrix(dets)
+    for i in range(len(dets)):
+        for j in range(i, len(dets)):
+            ref = o.diagonal(dets[i]) if i == j else o.off_diagonal(dets[i], dets[j])
+            assert H[i, j] == ref
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 33, 64, 89, 90, 128, 200, 256])
+def test_eig_update_bit_exact(k, workloads):
+    g = Cdfci(workloads("c1")["fcidump"], num_of_coordinate=8, estimate=8, max_memory=0.01)
+    o = Oracle(workloads("c1")["fcidump"])
+    rng = np.random.default_rng(k)
+    Hs = rng.normal(size=(k, k)) * 0.05
+    Hs = Hs + Hs.T
+    Hs[np.arange(k), np.arange(k)] = -20.0 + rng.normal(size=k)
+    Hs = np.ascontiguousarray(Hs.reshape(-1))
+    for first in (False, True):
+        x = rng.normal(size=k) * 0.3
+        z = -20.0 * x + rng.normal(size=k) * 0.1
+        xx = float(x @ x) if first else float(x @ x) + 3.0
+        xz = float(np.round(-20.0 * xx, 6))
+        dxg, xxg, scg = g.eig_update(Hs, x, z, (xx, 0.0), (0.9, 0.0), (xz, 0.0))
+        s = np.array([xx, 0.0, 0.9, 0.0, xz, 0.0])
+        dxo = np.zeros(k)
+        assert o.lib.oc_eig_update(k, Hs, x, z, s, dxo, np.zeros(3)) == 0
+        assert np.array_equal(dxg, dxo)
+        assert xxg == (s[0], s[1]) and scg == (s[2], s[3])
+
+
+def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
+    g = Cdfci(path, num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=max_memory, threshold=0.0)
+    g.init()
+    o = Oracle(path)
+    o.solver_init(k, M, z_thr, g.report()["max_size"], nthreads=4)
+    done = 0
+    for n in list(checkpoints) + [iters]:
+        step = n - done
+        if step <= 0:
+            continue
+        g.step(step)
+        o.step(step)
+        done = n
+        rg, ro = g.report(), o.report()
+        assert rg["energy"] == ro["energy"], (done, rg["energy"], ro["energy"])
+        for key in ("x_size", "z_size", "cols_size", "updates", "scale", "xx", "dx_max"):
+            assert rg[key] == ro[key], (key, done, rg[key], ro[key])
+    gd, gx, gz = g.table()
+    od, ox, oz, _ = o.table()
+    assert np.array_equal(gd[:, 0], od[:, 0])
+    assert np.array_equal(gx, ox)
+    assert np.array_equal(gz, oz)
+    cd, cx, cz = g.candidates()
+    kd, kx, kz = o.candidates()
+    assert sorted(cd[:, 0].tolist()) == sorted(kd[:, 0].tolist())
+    return g, o
+
+
+@pytest.mark.parametrize("name,k,M,z_thr,iters", [
+    ("c1", 8, 8, 0.0, 600),
+    ("c1", 1, 4, 0.0, 300),
+    ("c1", 3, 20, 0.0, 300),
+    ("c3_synth", 8, 8, 0.0, 200),
+    ("c3_synth", 16, 16, 1e-4, 150),
+    ("c3_hubbard", 8, 8, 0.0, 200),
+    ("h2o", 8, 8, 0.0, 300),
+    ("c2", 8, 8, 1e-7, 60),
+])
+def test_trajectory_bit_exact(name, k, M, z_thr, iters, workloads, h2o_fcidump):
+    print("case", name, k, M, z_thr, iters, flush=True)
+    path = _path(name, workloads, h2o_fcidump)
+    mem = 8.0 if name == "c2" else 0.05
+    _run_pair(path, k, M, z_thr, iters, max_memory=mem)
+
+
+def test_c1_reaches_exact_ground_state(workloads):
+    import ed
+    path = workloads("c1")["fcidump"]
+    norb, nelec, ms2, h, eri, core = ed.read_fcidump(path)
+    _, H = ed.hamiltonian(h, eri, core, nelec // 2, nelec // 2)
+    e0 = np.linalg.eigvalsh(H)[0]
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.005, threshold=0.0)
+    g.init()
+    g.step(20000)
+    assert abs(g.report()["energy"] - e0) < 1e-10
+
+
+def test_table_full_matches_oracle(workloads):
+    path = workloads("c3_synth")["fcidump"]
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.0005, threshold=0.0)
+    g.init()
+    o = Oracle(path)
+    o.solver_init(8, 8, 0.0, g.report()["max_size"], nthreads=2)
+    with pytest.raises(McdfciError) as ei:
+        for _ in range(50):
+            g.step(1)
+            g.report()
+    assert ei.value.code == 3
+    rc = 0
+    for _ in range(50):
+        rc = o.step(1)
+        if rc:
+            break
+    assert rc == 1
+
+
+def test_positive_spectrum_is_an_error(tmp_path, workloads):
+    from mcdfci_amd import synth
+    spec = synth.SyntheticSpec("pos", norb=6, nelec=6, rank=8, seed=5, core=500.0)
+    info = synth.generate(spec, str(tmp_path))
+    g = Cdfci(info["fcidump"], num_of_coordinate=4, estimate=4, max_memory=0.01, threshold=0.0)
+    g.init()
+    g.step(2)
+    with pytest.raises(McdfciError) as ei:
+        g.report()
+    assert ei.value.code == 5
+
+
+def test_mixed_precision_close_to_fp64(workloads):
+    """FP32 integrals/H elements, FP64 z, energy and eigensolve: the energy
+    after 300 iterations agrees with the FP64 path to 1e-6 Ha."""
+    path = workloads("c3_synth")["fcidump"]
+    res = {}
+    for prec in (0, PREC_MIXED):
+        g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.05, threshold=0.0, precision=prec)
+        g.init()
+        g.step(300)
+        res[prec] = g.report()["energy"]
+        d, v = g.column(g.hartree_fock())
+        res[(prec, "col")] = dict(zip(d[:, 0].tolist(), v.tolist()))
+    assert abs(res[0] - res[PREC_MIXED]) < 1e-6
+    a, b = res[(0, "col")], res[(PREC_MIXED, "col")]
+    assert a.keys() == b.keys()
+    assert max(abs(a[key] - b[key]) for key in a) < 1e-6
+
+
+def test_driver_end_to_end(workloads, tmp_path):
+    info = workloads("c1")
+    import json
+    doc = json.load(open(info["input"]))
+    doc["solver"]["cdfci"].update(num_iterations=2000, report_interval=500)
+    inp = tmp_path / "input.json"
+    inp.write_text(json.dumps(doc))
+    r = subprocess.run([mcdfci_amd.DRIVER_PATH, str(inp)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [l.split() for l in r.stdout.splitlines() if l[:1].isdigit() and len(l.split()) == 9]
+    assert [int(x[0]) for x in rows] == [500, 1000, 1500, 2000]
+    o = Oracle(info["fcidump"])
+    o.solver_init(8, 8, 0.0, 103546.0, nthreads=2)
+    o.step(2000)
+    assert rows[-1][1] == "%.10f" % o.report()["energy"]
+    assert "[LOG] max residual" in r.stdout
+    # unsupported pick: the reference's exit(-1)
+    doc["solver"]["cdfci"]["coordinate_pick"] = "random"
+    inp.write_text(json.dumps(doc))
+    r = subprocess.run([mcdfci_amd.DRIVER_PATH, str(inp)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 255 and "is not implemented" in r.stdout

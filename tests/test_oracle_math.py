@@ -1,0 +1,235 @@
+"""Pins the oracle with mathematics that does not depend on its code:
+second-quantised exact diagonalisation, closed forms, LAPACK (numpy) eigen
+decompositions, the algorithm's own invariants, and exact rational
+arithmetic for the numeric building blocks."""
+
+import itertools
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import ed
+from oracle_ctypes import Oracle, oracle_lib
+from mcdfci_amd import synth
+
+
+def _ed_matrix(path):
+    norb, nelec, ms2, h, eri, core = ed.read_fcidump(path)
+    na, nb = (nelec + ms2) // 2, (nelec - ms2) // 2
+    return ed.hamiltonian(h, eri, core, na, nb)
+
+
+def test_columns_match_second_quantisation(workloads):
+    """Every column element (value and sign) equals <D'|H|D> built from
+    creation/annihilation operators in interleaved Jordan-Wigner order."""
+    path = workloads("c1")["fcidump"]
+    dets, H = _ed_matrix(path)
+    idx = {int(d): i for i, d in enumerate(dets)}
+    o = Oracle(path)
+    for j, d in enumerate(dets):
+        cd, cv = o.column(np.array([d]))
+        col = np.zeros(len(dets))
+        for dd, v in zip(cd[:, 0], cv):
+            col[idx[int(dd)]] = v
+        assert np.allclose(col, H[:, j], atol=1e-13, rtol=0)
+        # column support = nonzero pattern of H (no element missing)
+        assert np.all((np.abs(H[:, j]) > 1e-12) <= (col != 0))
+
+
+def test_c1_converges_to_exact_ground_state(workloads):
+    path = workloads("c1")["fcidump"]
+    dets, H = _ed_matrix(path)
+    e0 = np.linalg.eigvalsh(H)[0]
+    o = Oracle(path)
+    o.solver_init(8, 8, 0.0, 103546.0, nthreads=2)
+    o.step(20000)
+    r = o.report()
+    assert abs(r["energy"] - e0) < 1e-10
+
+
+def test_invariants_z_equals_Hx(workloads):
+    """With z_threshold = 0 the table keeps z = H x, xx = sum x^2 and
+    x.z = sum x z (the reference's debug invariant, src/wavefunction.h:640-667)."""
+    path = workloads("c1")["fcidump"]
+    dets, H = _ed_matrix(path)
+    idx = {int(d): i for i, d in enumerate(dets)}
+    o = Oracle(path)
+    o.solver_init(8, 8, 0.0, 103546.0, nthreads=2)
+    o.step(150)
+    td, tx, tz, _ = o.table()
+    x = np.zeros(len(dets))
+    z = np.zeros(len(dets))
+    for d, a, b in zip(td[:, 0], tx, tz):
+        x[idx[int(d)]] = a
+        z[idx[int(d)]] = b
+    assert np.allclose(z, H @ x, atol=1e-11)
+    r = o.report()
+    assert abs(r["xx"] - x @ x) < 1e-10 * (x @ x)
+    assert abs(r["xz"] - x @ z) < 1e-10 * abs(x @ z)
+    assert abs(r["energy"] - (x @ H @ x) / (x @ x)) < 1e-11
+
+
+def test_two_site_hubbard_closed_form(tmp_path):
+    """H2 Hubbard dimer: E0 = (U - sqrt(U^2 + 16 t^2)) / 2 (+ core)."""
+    t, U, core = 1.0, 4.0, -10.0
+    spec = synth.SyntheticSpec("hub2", norb=2, nelec=2, kind="hubbard", hubbard_t=t, hubbard_u=U, core=core)
+    h, eri = synth.hubbard_integrals(spec)
+    # periodic ring of two sites double-counts the bond; use a single bond
+    h = np.array([[0.0, -t], [-t, 0.0]])
+    path = tmp_path / "FCIDUMP"
+    synth.write_fcidump(str(path), h, eri, 2, 0, core)
+    o = Oracle(str(path))
+    o.solver_init(2, 4, 0.0, 1000.0, nthreads=1)
+    o.step(200)
+    exact = core + (U - np.sqrt(U * U + 16 * t * t)) / 2
+    assert abs(o.report()["energy"] - exact) < 1e-12
+
+
+def test_small_hubbard_ring_matches_ed(tmp_path):
+    spec = synth.SyntheticSpec("hub6", norb=6, nelec=6, kind="hubbard")
+    h, eri = synth.hubbard_integrals(spec)
+    path = tmp_path / "FCIDUMP"
+    synth.write_fcidump(str(path), h, eri, 6, 0, -20.0)
+    dets, H = _ed_matrix(str(path))
+    e0 = np.linalg.eigvalsh(H)[0]
+    o = Oracle(str(path))
+    o.solver_init(4, 8, 0.0, 1e5, nthreads=1)
+    o.step(8000)
+    assert abs(o.report()["energy"] - e0) < 1e-9
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 17, 33, 65, 129, 257])
+def test_lowest_eigpair_vs_lapack(n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    A = A + A.T
+    A[np.arange(n), np.arange(n)] -= 5.0 + np.arange(n) * 0.1
+    lib = oracle_lib()
+    import ctypes as C
+    lam = C.c_double()
+    qh, ql = np.zeros(n), np.zeros(n)
+    lib.oc_lowest_eig(n, np.ascontiguousarray(A.reshape(-1)), C.byref(lam), qh, ql)
+    w, V = np.linalg.eigh(A)
+    assert abs(lam.value - w[0]) <= 1e-12 * max(1.0, abs(w[0]))
+    v = V[:, 0] * np.sign(V[np.argmax(np.abs(V[:, 0]) > 1e-300), 0])
+    lead = np.nonzero(qh)[0][0]
+    v = v * np.sign(v[lead])
+    assert np.allclose(qh, v, atol=1e-10)
+    assert abs(np.dot(qh, qh) - 1.0) < 1e-14
+
+
+def _reference_update_numpy(Hs, x, z, xx, xz):
+    """The subspace update of src/solver.h:333-609 written with numpy.eigh."""
+    k = len(x)
+    Hsym = np.triu(Hs) + np.triu(Hs, 1).T
+    Hx = Hsym @ x
+    yy = xx - x @ x
+    if abs(yy) > 1e-14:
+        y1 = np.sqrt(yy)
+        A = np.zeros((k + 1, k + 1))
+        A[0, 0] = (xz - 2 * (x @ z) + x @ Hx) / yy
+        A[0, 1:] = A[1:, 0] = (z - Hx) / y1
+        A[1:, 1:] = Hsym
+        w, V = np.linalg.eigh(A)
+        v = V[:, 0]
+        lam = -w[0]
+        gam = v[0] / y1
+        dx = v[1:] / gam - x
+        sc = gam * np.sqrt(lam)
+        return dx, lam / sc / sc, abs(sc)
+    w, V = np.linalg.eigh(Hsym)
+    v = V[:, 0]
+    return v * np.sqrt(-w[0]) - x, -w[0], 1.0
+
+
+@pytest.mark.parametrize("k", [1, 2, 8, 33, 64])
+def test_eig_update_vs_numpy_and_identities(k):
+    rng = np.random.default_rng(100 + k)
+    Hs = rng.normal(size=(k, k)) * 0.1
+    Hs = Hs + Hs.T
+    Hs[np.arange(k), np.arange(k)] = -10.0 + rng.normal(size=k)
+    x = rng.normal(size=k) * 0.2
+    z = -10.0 * x + rng.normal(size=k) * 0.05
+    xx = float(x @ x) + 1.5
+    xz = -10.0 * xx
+    s = np.array([xx, 0.0, 1.0, 0.0, xz, 0.0])
+    dx = np.zeros(k)
+    lib = oracle_lib()
+    rc = lib.oc_eig_update(k, np.ascontiguousarray(Hs.reshape(-1)), x, z, s, dx, np.zeros(3))
+    assert rc == 0
+    dx_ref, xx_ref, sc_ref = _reference_update_numpy(Hs, x, z, xx, xz)
+    assert np.allclose(dx, dx_ref, atol=1e-11)
+    assert abs(s[0] - xx_ref) < 1e-11 * xx_ref
+    assert abs(abs(s[2]) - sc_ref) < 1e-11
+    # identity: |x_new|^2 = yy + sum (x + dx)^2 = xx_new
+    yy = xx - x @ x
+    assert abs(yy + np.sum((x + dx) ** 2) - s[0]) < 1e-10 * s[0]
+
+
+def test_eig_update_first_iteration_branch():
+    k = 8
+    rng = np.random.default_rng(9)
+    Hs = rng.normal(size=(k, k)) * 0.1
+    Hs = Hs + Hs.T
+    Hs[np.arange(k), np.arange(k)] = -5.0
+    x = np.zeros(k)
+    x[0] = 1.0
+    z = Hs[:, 0].copy()
+    s = np.array([1.0, 0.0, 1.0, 0.0, Hs[0, 0], 0.0])
+    dx = np.zeros(k)
+    assert oracle_lib().oc_eig_update(k, np.ascontiguousarray(Hs.reshape(-1)), x, z, s, dx, np.zeros(3)) == 0
+    w, V = np.linalg.eigh(Hs)
+    v = V[:, 0] * np.sign(V[0, 0])
+    assert np.allclose(dx, v * np.sqrt(-w[0]) - x, atol=1e-12)
+    assert abs(s[0] + w[0]) < 1e-12 and s[2] == 1.0
+
+
+def test_eig_update_rejects_positive_spectrum():
+    k = 3
+    Hs = np.eye(k) * 2.0
+    x = np.array([1.0, 0, 0])
+    s = np.array([1.0, 0.0, 1.0, 0.0, 2.0, 0.0])
+    assert oracle_lib().oc_eig_update(k, np.ascontiguousarray(Hs.reshape(-1)), x, Hs[:, 0].copy(), s,
+                                      np.zeros(k), np.zeros(3)) == 1
+
+
+def _fx(v):
+    return Fraction(v)
+
+
+def test_double_double_against_exact_rationals():
+    lib = oracle_lib()
+    rng = np.random.default_rng(0)
+    out = np.zeros(8)
+    for _ in range(400):
+        a = float(rng.normal() * 10.0 ** rng.integers(-3, 4))
+        al = a * 1e-17 * rng.normal()
+        b = float(rng.normal() * 10.0 ** rng.integers(-3, 4))
+        bl = b * 1e-17 * rng.normal()
+        lib.oc_dd_ops(a, al, b, bl, out)
+        A, B = _fx(a) + _fx(al), _fx(b) + _fx(bl)
+        for got, exact in ((out[0:2], A + B), (out[2:4], A * B), (out[4:6], A / B)):
+            err = abs(_fx(got[0]) + _fx(got[1]) - exact)
+            assert err <= abs(exact) * Fraction(1, 2 ** 100) + Fraction(1, 2 ** 1000)
+        if a > 0:
+            got = _fx(out[6]) + _fx(out[7])
+            assert abs(got * got - A) <= A * Fraction(1, 2 ** 98)
+
+
+def test_fixed_point_round_trip_exact():
+    lib = oracle_lib()
+    rng = np.random.default_rng(1)
+    o = np.zeros(2, np.int64)
+    for _ in range(2000):
+        d = float(rng.normal() * 10.0 ** rng.integers(-25, 6))
+        lib.oc_to_fixed(d, o)
+        v = (int(o[0]) << 64) | (int(o[1]) & ((1 << 64) - 1))
+        if v >= 1 << 127:
+            v -= 1 << 128
+        exact = Fraction(d) * 2 ** 72
+        # round half even to the nearest integer
+        assert abs(Fraction(v) - exact) <= Fraction(1, 2)
+        back = lib.oc_fixed_to_double(int(o[0]), int(o[1]))
+        assert abs(_fx(back) - Fraction(v, 2 ** 72)) <= abs(Fraction(v, 2 ** 72)) * Fraction(1, 2 ** 51)
