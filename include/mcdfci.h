@@ -126,6 +126,9 @@ int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
  * with HIP events when timing is enabled (mcdfci_set_timing). */
 int mcdfci_set_timing(mcdfci_ctx* ctx, int enable);
 int mcdfci_timers(mcdfci_ctx* ctx, double* t6);
+/* Per-kernel device times in seconds over the timed iterations:
+ * {pick, submatrix, eig (+update_x), scatter, pass2, merge}. */
+int mcdfci_kernel_times(mcdfci_ctx* ctx, double* t6, int64_t* iterations);
 
 /* Test hooks. */
 /* HamiltonianMolecule::get_column (src/hamiltonian.h:412-418): writes the

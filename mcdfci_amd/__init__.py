@@ -88,6 +88,7 @@ SIGNATURES = {
     "mcdfci_report_get": (C.c_int, [C.c_void_p, C.POINTER(Report)]),
     "mcdfci_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mcdfci_timers": (C.c_int, [C.c_void_p, _f64p]),
+    "mcdfci_kernel_times": (C.c_int, [C.c_void_p, _f64p, C.POINTER(C.c_int64)]),
     "mcdfci_column": (C.c_int, [C.c_void_p, _u64p, _u64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_submatrix": (C.c_int, [C.c_void_p, C.c_int, _u64p, _f64p]),
     "mcdfci_eig_update": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
@@ -189,6 +190,14 @@ class Cdfci:
         t = np.zeros(6)
         self._check(self.lib.mcdfci_timers(self.h, t))
         return t
+
+    def kernel_times(self):
+        """Seconds per kernel {pick, submatrix, eig, scatter, pass2, merge} and the
+        number of timed iterations (timing mode only)."""
+        t = np.zeros(6)
+        n = C.c_int64()
+        self._check(self.lib.mcdfci_kernel_times(self.h, t, C.byref(n)))
+        return dict(zip(["pick", "submatrix", "eig", "scatter", "pass2", "merge"], t.tolist())), n.value
 
     def column(self, det):
         d = np.zeros(2, np.uint64)

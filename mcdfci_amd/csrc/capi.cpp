@@ -75,6 +75,8 @@ struct mcdfci_ctx {
   bool ready = false;
   bool timing = false;
   double timers[6] = {0, 0, 0, 0, 0, 0};
+  double ktimes[6] = {0, 0, 0, 0, 0, 0};  // pick, submatrix, eig, scatter, pass2, merge
+  long long ktimed = 0;
   hipEvent_t ev[7] = {};
   std::vector<void*> allocs;
 };
@@ -297,12 +299,17 @@ void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
   hipStream_t s = c->stream;
   launch_scatter(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, k,
                  c->nchunks);
+  if (c->timing && count_iteration) (void)hipEventRecord(c->ev[4], s);
   launch_pass2(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, c->M,
                k, c->nchunks, c->stage, c->stage_cnt, c->part);
+  if (c->timing && count_iteration) (void)hipEventRecord(c->ev[5], s);
   launch_merge(s, c->st, c->pdet, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
                c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration);
 }
 
+// One iteration of src/solver.h:701-759.  With timing enabled every kernel
+// is bracketed by HIP events on the solver stream and the iteration is
+// synchronised to read them.
 void launch_iteration(mcdfci_ctx* c) {
   hipStream_t s = c->stream;
   const int k = c->k;
@@ -315,17 +322,18 @@ void launch_iteration(mcdfci_ctx* c) {
   if (c->timing) (void)hipEventRecord(c->ev[3], s);
   launch_update_z(c, k, 1);
   if (c->timing) {
-    (void)hipEventRecord(c->ev[4], s);
-    (void)hipEventSynchronize(c->ev[4]);
-    float ms;
-    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-    c->timers[0] += ms * 1e-3;
-    (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
-    c->timers[1] += ms * 1e-3;
-    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-    c->timers[3] += ms * 1e-3;
-    (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]);
-    c->timers[5] += ms * 1e-3;
+    (void)hipEventRecord(c->ev[6], s);
+    (void)hipEventSynchronize(c->ev[6]);
+    float ms[6];
+    for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+    for (int i = 0; i < 6; ++i) c->ktimes[i] += ms[i] * 1e-3;
+    c->ktimed += 1;
+    // step_timer laps: pick, submatrix, columns (fused into update_z), eig
+    // (with update_x), update_x, update_z
+    c->timers[0] += ms[0] * 1e-3;
+    c->timers[1] += ms[1] * 1e-3;
+    c->timers[3] += ms[2] * 1e-3;
+    c->timers[5] += (ms[3] + ms[4] + ms[5]) * 1e-3;
   }
 }
 
@@ -483,6 +491,8 @@ int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   for (double& t : c->timers) t = 0.0;
+  for (double& t : c->ktimes) t = 0.0;
+  c->ktimed = 0;
   c->ready = true;
   return check_state_error(c, s);
 }
@@ -550,6 +560,13 @@ int mcdfci_set_timing(mcdfci_ctx* c, int enable) {
 int mcdfci_timers(mcdfci_ctx* c, double* t6) {
   if (!c || !t6) return fail(c, MCDFCI_E_INPUT, "null argument");
   for (int i = 0; i < 6; ++i) t6[i] = c->timers[i];
+  return MCDFCI_OK;
+}
+
+int mcdfci_kernel_times(mcdfci_ctx* c, double* t6, int64_t* iterations) {
+  if (!c || !t6) return fail(c, MCDFCI_E_INPUT, "null argument");
+  for (int i = 0; i < 6; ++i) t6[i] = c->ktimes[i];
+  if (iterations) *iterations = c->ktimed;
   return MCDFCI_OK;
 }
 
