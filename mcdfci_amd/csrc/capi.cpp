@@ -514,6 +514,13 @@ int mcdfci_step(mcdfci_ctx* c, int64_t n) {
 
 int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {
   if (!c || !r) return fail(c, MCDFCI_E_INPUT, "null argument");
+  if (!c->ready) {
+    // Before mcdfci_init only the sizing fields are meaningful.
+    std::memset(r, 0, sizeof(*r));
+    r->table_slots = (int64_t)c->nslots;
+    r->max_size = c->max_size;
+    return MCDFCI_OK;
+  }
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   DevState s;

@@ -3,8 +3,8 @@
 // HBM layout (one context):
 //   table      : 2^p slots of {key, x, zA, zB} (32 B, 4 slots per 128 B line),
 //                open addressing with bucketed linear probing; key 0 = empty.
-//                z = (zA * 2^32 + zB) * 2^-72 exactly; x = -0.0 marks a slot
-//                inserted during the current iteration (cleared in pass 2).
+//                z = (zA * 2^32 + zB) * 2^-72 exactly; key bit 63 marks a
+//                slot inserted during the current iteration (cleared in pass 2).
 //   integrals  : dense spatial (pq|rs) [M^4] and h [M^2] in fp64 and fp32,
 //                single-excitation sums S_same/S_opp [M^3], diagonal terms
 //                per spin-orbital; all read through L2/MALL.
@@ -24,8 +24,11 @@ constexpr int kEPT = 4;                // column elements per thread
 constexpr int kChunk = kWG * kEPT;     // column elements per workgroup
 constexpr int kSortCap = 4096;         // entries sorted by one workgroup
 constexpr int kMaxK = 256;             // coordinates per iteration
-constexpr int kMaxOrb = 32;            // spatial orbitals (one 64-bit word)
-constexpr uint64_t kMark = 0x8000000000000000ULL;  // bits of -0.0
+constexpr int kMaxOrb = 31;            // spatial orbitals (one 64-bit word, bit 63 spare)
+// Bit 63 of a table key marks a determinant inserted during the running
+// iteration (set by the inserting CAS, cleared by the pass that follows).
+constexpr uint64_t kMark = 0x8000000000000000ULL;
+constexpr uint64_t kKeyMask = ~kMark;
 
 struct Slot {
   uint64_t key;

@@ -430,19 +430,42 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   if (mode != 0) return;
   __syncthreads();
   // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
+  // Picked determinants are distinct, so every thread owns its slots; the
+  // x.z increments are exact fixed-point integers (order-free sum).
+  __shared__ long long nx_w[kWG / 64];
+  __shared__ i128 xz_w[kWG / 64];
+  long long nx = 0;
+  i128 inc = 0;
+  for (int t = tid; t < k; t += bs) {
+    Slot* s = &tab[pslot[t]];
+    const double xo = s->x;
+    if (!(__builtin_fabs(xo) != 0.0)) ++nx;
+    s->x = xo + dx[t];
+    inc += to_fix(L.z[t] * dx[t]);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    nx += __shfl_xor(nx, off, 64);
+    int64_t hi = (int64_t)(inc >> 64);
+    uint64_t lo = (uint64_t)inc;
+    int64_t ohi = __shfl_xor(hi, off, 64);
+    uint64_t olo = (uint64_t)__shfl_xor((long long)lo, off, 64);
+    inc += (((i128)ohi) << 64) | (i128)olo;
+  }
+  if ((tid & 63) == 0) {
+    nx_w[tid >> 6] = nx;
+    xz_w[tid >> 6] = inc;
+  }
+  __syncthreads();
   if (tid == 0) {
     i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
-    long long nx = 0;
-    for (int t = 0; t < k; ++t) {
-      Slot* s = &tab[pslot[t]];
-      double xo = s->x;
-      if (!(__builtin_fabs(xo) != 0.0)) ++nx;
-      s->x = xo + dx[t];
-      xz += to_fix(L.z[t] * dx[t]);
+    long long n = 0;
+    for (int w = 0; w < (bs >> 6); ++w) {
+      xz += xz_w[w];
+      n += nx_w[w];
     }
     st->xz_hi = (int64_t)(xz >> 64);
     st->xz_lo = (uint64_t)xz;
-    st->x_size += nx;
+    st->x_size += n;
     st->k_cur = k;
   }
 }

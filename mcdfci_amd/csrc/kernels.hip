@@ -29,8 +29,11 @@ __device__ __forceinline__ int parity1(uint64_t d, int i, int a) {
   return __popcll(d & m) & 1;
 }
 
-__device__ __forceinline__ uint64_t aload(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Relaxed device-coherent load: no cache invalidation, no ordering.  Every
+// cross-workgroup fact the scatter needs (present / inserted this iteration)
+// lives in the key word itself, so no acquire/release fences are required.
+__device__ __forceinline__ uint64_t rload(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void zadd(Slot* s, i128 v) {
@@ -61,7 +64,7 @@ __device__ __forceinline__ long long find_slot(const Slot* tab, uint64_t mask, u
   for (uint64_t n = 0; n <= mask; n += 4) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      uint64_t k = tab[b + j].key;
+      uint64_t k = tab[b + j].key & kKeyMask;
       if (k == det) return (long long)(b + j);
       if (k == 0) return -1;
     }
@@ -394,29 +397,29 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
         const double dz = V * dxt;
         const bool gate = dd_abs_gt(dd_mul_d(sc, dz), zthr);
         Slot* hit = nullptr;
-        bool mine = false;
+        bool mine = false, marked = false;
         uint64_t b = mix64(D) & mask & ~3ULL;
         bool done = false;
         for (uint64_t n = 0; n <= mask && !done; n += 4) {
           for (int jj = 0; jj < 4 && !done; ++jj) {
             Slot* s = &tab[b + jj];
-            uint64_t k = aload(&s->key);
-            if (k == D) {
+            uint64_t k = rload(&s->key);
+            if ((k & kKeyMask) == D) {
               hit = s;
+              marked = (k & kMark) != 0;
               done = true;
             } else if (k == 0) {
               if (!gate) {
                 done = true;
               } else {
-                __hip_atomic_store((uint64_t*)&s->x, kMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 uint64_t expect = 0;
-                bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D, __ATOMIC_ACQ_REL,
-                                                               __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D | kMark, __ATOMIC_RELAXED,
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (ok) {
                   hit = s;
                   mine = true;
                   done = true;
-                } else if (expect == D) {
+                } else if ((expect & kKeyMask) == D) {
                   hit = s;
                   done = true;
                 }
@@ -427,13 +430,9 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
         }
         if (!done) fail = true;
         if (hit) {
-          if (gate) {
-            visible = true;
-          } else {
-            uint64_t xb = __hip_atomic_load((const uint64_t*)&hit->x, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-            visible = (xb != kMark);
-          }
+          // Gate-passing elements always add; the others only into
+          // determinants that were present before this iteration.
+          visible = gate || !marked;
           if (mine) ++n_new;
           if (visible) {
             zadd(hit, to_fix(dz));
@@ -533,12 +532,8 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
         double x = 0.0, val = 0.0;
         if (si >= 0) {
           Slot* sl = &tab[si];
-          uint64_t xb = bitsd(sl->x);
-          x = dbits(xb);
-          if (xb == kMark) {
-            x = 0.0;
-            sl->x = 0.0;
-          }
+          if (sl->key & kMark) sl->key = D;  // clear the inserted-this-iteration mark
+          x = sl->x;
           const uint64_t w = vis[(size_t)t * vis_words + (e >> 6)];
           if ((w >> (e & 63)) & 1) {
             double z = fix_to_d(fix_join(sl->za, sl->zb));
@@ -646,25 +641,57 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
     pl[i] = 0xffffffffu;
   }
   bitonic_desc(ka, kb, pl, np2);
-  // First M entries; keep those present in the table.
-  if (threadIdx.x == 0) {
-    int m = n < M ? n : M;
+  // First M entries; keep those present in the table (order preserved).
+  __shared__ int keep[kWG];
+  const int m = n < M ? n : M;
+  {
+    const int per = (m + kWG - 1) / kWG;
+    const int lo = threadIdx.x * per, hi = min(m, lo + per);
     int c = 0;
-    for (int i = 0; i < m; ++i) {
-      if (pl[i] != 0xffffffffu) {
-        cand[(size_t)t * M + c].det = kb[i];
-        cand[(size_t)t * M + c].slot = pl[i];
-        cand[(size_t)t * M + c].pad = 0;
-        ++c;
+    for (int i = lo; i < hi; ++i) c += pl[i] != 0xffffffffu;
+    keep[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int i = 0; i < kWG; ++i) {
+        int v = keep[i];
+        keep[i] = acc;
+        acc += v;
       }
+      cand_cnt[t] = acc;
     }
-    cand_cnt[t] = c;
-    // Exact per-column sums; reinsert z of the picked determinant.
+    __syncthreads();
+    int pos = keep[threadIdx.x];
+    for (int i = lo; i < hi; ++i) {
+      if (pl[i] == 0xffffffffu) continue;
+      cand[(size_t)t * M + pos].det = kb[i];
+      cand[(size_t)t * M + pos].slot = pl[i];
+      cand[(size_t)t * M + pos].pad = 0;
+      ++pos;
+    }
+  }
+  // Exact per-column sums (order-free) and the reinsert of z_t := new_z_t.
+  __shared__ i128 sums[kWG / 64][2];
+  {
     i128 nz = 0, xt = 0;
-    for (int ch = 0; ch < nchunks; ++ch) {
+    for (int ch = threadIdx.x; ch < nchunks; ch += kWG) {
       const int64_t* o = part + ((size_t)t * nchunks + ch) * 4;
       nz += (((i128)o[0]) << 64) | (i128)(uint64_t)o[1];
       xt += (((i128)o[2]) << 64) | (i128)(uint64_t)o[3];
+    }
+    nz = wave_sum_i128(nz);
+    xt = wave_sum_i128(xt);
+    if ((threadIdx.x & 63) == 0) {
+      sums[threadIdx.x >> 6][0] = nz;
+      sums[threadIdx.x >> 6][1] = xt;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i128 nz = 0, xt = 0;
+    for (int w = 0; w < kWG / 64; ++w) {
+      nz += sums[w][0];
+      xt += sums[w][1];
     }
     Slot* sp = &tab[pslot[t]];
     sp->za = (int64_t)(nz >> 32);
@@ -743,18 +770,34 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   bitonic_desc(ka, kb, pl, np2);
   for (int i = threadIdx.x; i < n; i += blockDim.x) flag[i] = (i == 0 || kb[i] != kb[i - 1]) ? 1 : 0;
   __syncthreads();
+  // Exclusive prefix count of distinct entries: thread t scans its block of
+  // n/blockDim entries, then thread 0 scans the block totals.
+  __shared__ int bsum[1024];
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int lo = threadIdx.x * per, hi = min(n, lo + per);
+  int cnt = 0;
+  for (int i = lo; i < hi; ++i) cnt += flag[i];
+  bsum[threadIdx.x] = cnt;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int c = 0;
-    for (int i = 0; i < n && c < k; ++i) {
-      if (!flag[i]) continue;
-      pdet[c] = kb[i];
-      pslot[c] = pl[i];
-      const Slot& sl = tab[pl[i]];
-      px[c] = sl.x;
-      pz[c] = fix_to_d(fix_join(sl.za, sl.zb));
-      ++c;
+    int acc = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      int v = bsum[t];
+      bsum[t] = acc;
+      acc += v;
     }
-    if (c < k) st->error = ERR_CANDIDATES;
+    if (acc < k) st->error = ERR_CANDIDATES;
+  }
+  __syncthreads();
+  int pos = bsum[threadIdx.x];
+  for (int i = lo; i < hi && pos < k; ++i) {
+    if (!flag[i]) continue;
+    pdet[pos] = kb[i];
+    pslot[pos] = pl[i];
+    const Slot& sl = tab[pl[i]];
+    px[pos] = sl.x;
+    pz[pos] = fix_to_d(fix_join(sl.za, sl.zb));
+    ++pos;
   }
 }
 
@@ -844,7 +887,7 @@ __global__ void k_table_compact(const Slot* tab, uint64_t nslots, uint64_t* odet
   if (s.key == 0) return;
   unsigned long long pos = atomicAdd(count, 1ULL);
   if ((long long)pos < cap) {
-    odet[pos] = s.key;
+    odet[pos] = s.key & kKeyMask;
     ox[pos] = s.x;
     oz[2 * pos] = s.za;
     oz[2 * pos + 1] = s.zb;

@@ -39,7 +39,7 @@ def test_columns_bit_exact(name, workloads, h2o_fcidump):
     assert np.array_equal(hf, o.hf())
     od, ov = o.column(hf)
     rng = np.random.default_rng(2)
-    seeds = [hf] + [od[i] for i in rng.choice(len(od), 6, replace=False)]
+    seeds = [hf] + [od[i] for i in rng.choice(len(od), min(6, len(od)), replace=False)]
     for d in seeds:
         gd, gv = g.column(d)
         od2, ov2 = o.column(d)
@@ -121,7 +121,7 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
     ("c1", 3, 20, 0.0, 300),
     ("c3_synth", 8, 8, 0.0, 200),
     ("c3_synth", 16, 16, 1e-4, 150),
-    ("c3_hubbard", 8, 8, 0.0, 200),
+    ("c3_hubbard", 4, 8, 0.0, 200),
     ("h2o", 8, 8, 0.0, 300),
     ("c2", 8, 8, 1e-7, 60),
 ])
@@ -161,6 +161,21 @@ def test_table_full_matches_oracle(workloads):
         if rc:
             break
     assert rc == 1
+
+
+def test_too_few_candidates_matches_oracle(workloads):
+    """The Hubbard HF column has 5 elements; k = 8 distinct picks are impossible
+    (the reference would read past its priority queue, src/solver.h:218-223)."""
+    path = workloads("c3_hubbard")["fcidump"]
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.05, threshold=0.0)
+    g.init()
+    g.step(1)
+    with pytest.raises(McdfciError) as ei:
+        g.report()
+    assert ei.value.code == 6
+    o = Oracle(path)
+    o.solver_init(8, 8, 0.0, 1e6, nthreads=1)
+    assert o.step(1) == 3
 
 
 def test_positive_spectrum_is_an_error(tmp_path, workloads):
