@@ -246,6 +246,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   c->nchunks = (H.n_enum + kChunk - 1) / kChunk;
   c->vis_words = c->nchunks * (kChunk / 64);
   const int mw = std::min(M, kChunk);
+  if (c->nchunks > 1024) return fail(c, MCDFCI_E_UNSUPPORTED, "column too long for the merge kernel");
   if ((long long)c->nchunks * mw > kSortCap)
     return fail(c, MCDFCI_E_UNSUPPORTED,
                 "estimate * column chunks exceeds the per-column merge capacity (" + std::to_string(kSortCap) +
@@ -624,6 +625,27 @@ int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* va
     dets[i] = hd[order[i]];
     vals[i] = hv[order[i]];
   }
+  return MCDFCI_OK;
+}
+
+// Diagnostic (not part of the public header): phase timestamps of the last
+// k_eig launch in wall-clock ticks.
+int mcdfci_diag_eig_profile(mcdfci_ctx* c, int64_t* out16) {
+  if (!c || !out16) return MCDFCI_E_INPUT;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 16; ++i) out16[i] = s.prof[i];
+  return MCDFCI_OK;
+}
+
+// Diagnostic (not part of the public header): the last iteration's picked
+// determinants and dx.
+int mcdfci_diag_last_step(mcdfci_ctx* c, uint64_t* dets, double* dx) {
+  if (!c) return MCDFCI_E_INPUT;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(dets, c->pdet, 8 * (size_t)c->k, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(dx, c->dx, 8 * (size_t)c->k, hipMemcpyDeviceToHost));
   return MCDFCI_OK;
 }
 

@@ -3,7 +3,7 @@
 // HBM layout (one context):
 //   table      : 2^p slots of {key, x, zA, zB} (32 B, 4 slots per 128 B line),
 //                open addressing with bucketed linear probing; key 0 = empty.
-//                z = (zA * 2^32 + zB) * 2^-72 exactly; key bit 63 marks a
+//                z = (zA * 2^32 + zB) * 2^-64 exactly; key bit 63 marks a
 //                slot inserted during the current iteration (cleared in pass 2).
 //   integrals  : dense spatial (pq|rs) [M^4] and h [M^2] in fp64 and fp32,
 //                single-excitation sums S_same/S_opp [M^3], diagonal terms
@@ -81,6 +81,7 @@ struct DevState {
   double dx_max;
   double max_residual, curr_max_abs_evec, sum_diff_inv, max_diff_inv, curr_diff_inv;
   long long sum_iterations, sum_inv_iter, curr_inv_iter;
+  long long prof[16];  // phase timestamps of the last k_eig (wall clock ticks)
 };
 
 enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9 };

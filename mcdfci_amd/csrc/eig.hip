@@ -49,17 +49,19 @@ struct EigLds {
   double x[kMaxK], z[kMaxK], Hx[kMaxK];
   double v[kNMax], p[kNMax], w[kNMax], col[kNMax];
   double d[kNMax], e[kNMax], tau[kNMax], y[kNMax];
-  double t0[kNMax], t1[kNMax], t2[kNMax], t3[kNMax];
+  double t0[kNMax], t1[kNMax], t2[kNMax], t3[kNMax], t4[kNMax];
+  int sw[kNMax];
   dd q[kNMax];
-  dd u0[kNMax], u1[kNMax], u2[kNMax], u3[kNMax];
   double ev[kNMax], dl[kNMax], Ae[kNMax];
   double sc[16];
   int iv[8];
-  int wmin[4];
+  int tmin[kWG];
 };
 
-__device__ void tri_solve_d(int n, const double* d, const double* e, double shift, double* b, double* dg,
-                            double* du, double* dl, double* du2) {
+// LU factorisation with partial pivoting of T - shift*I (dgttrf
+// conventions), by one thread; factors land in dg/du/du2/mul/swp.
+__device__ __forceinline__ void tri_factor(int n, const double* d, const double* e, double shift, double* dg,
+                                           double* du, double* du2, double* mul, int* swp, double* dl) {
   const double tiny = 1e-290;
   for (int i = 0; i < n; ++i) dg[i] = d[i] - shift;
   for (int i = 0; i + 1 < n; ++i) {
@@ -70,28 +72,40 @@ __device__ void tri_solve_d(int n, const double* d, const double* e, double shif
   for (int i = 0; i + 1 < n; ++i) {
     if (__builtin_fabs(dg[i]) >= __builtin_fabs(dl[i])) {
       if (dg[i] == 0.0) dg[i] = tiny;
-      double fact = dl[i] / dg[i];
-      dg[i + 1] = dg[i + 1] - fact * du[i];
-      b[i + 1] = b[i + 1] - fact * b[i];
-      if (i < n - 2) du2[i] = 0.0;
+      const double m = dl[i] / dg[i];
+      mul[i] = m;
+      swp[i] = 0;
+      dg[i + 1] = dg[i + 1] - m * du[i];
     } else {
-      double fact = dg[i] / dl[i];
-      double ndgi = dl[i];
-      double ndui = dg[i + 1];
-      double ndgi1 = du[i] - fact * dg[i + 1];
+      const double m = dg[i] / dl[i];
+      mul[i] = m;
+      swp[i] = 1;
+      const double ndgi = dl[i];
+      const double ndui = dg[i + 1];
+      const double ndgi1 = du[i] - m * dg[i + 1];
       if (i < n - 2) {
         du2[i] = du[i + 1];
-        du[i + 1] = -fact * du[i + 1];
+        du[i + 1] = -m * du[i + 1];
       }
       dg[i] = ndgi;
       du[i] = ndui;
       dg[i + 1] = ndgi1;
-      double bi = b[i], bi1 = b[i + 1];
-      b[i] = bi1;
-      b[i + 1] = bi - fact * bi1;
     }
   }
   if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
+}
+
+__device__ __forceinline__ void tri_solve_d(int n, const double* dg, const double* du, const double* du2,
+                                            const double* mul, const int* swp, double* b) {
+  for (int i = 0; i + 1 < n; ++i) {
+    if (!swp[i]) {
+      b[i + 1] = b[i + 1] - mul[i] * b[i];
+    } else {
+      const double bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = bi - mul[i] * bi1;
+    }
+  }
   b[n - 1] = b[n - 1] / dg[n - 1];
   if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
   for (int i = n - 3; i >= 0; --i) {
@@ -101,46 +115,25 @@ __device__ void tri_solve_d(int n, const double* d, const double* e, double shif
   }
 }
 
-__device__ void tri_solve_dd(int n, const double* d, const double* e, double shift, dd* b, dd* dg, dd* du,
-                             dd* dl, dd* du2) {
-  const dd tiny{1e-290, 0.0};
-  for (int i = 0; i < n; ++i) dg[i] = two_sum(d[i], -shift);
+// Double factors, double-double right-hand side (the reference's double LU
+// with binary128 vectors, src/solver.h:459-478).
+__device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const double* du, const double* du2,
+                                             const double* mul, const int* swp, dd* b) {
   for (int i = 0; i + 1 < n; ++i) {
-    du[i] = mkdd(e[i]);
-    dl[i] = mkdd(e[i]);
-  }
-  for (int i = 0; i + 2 < n; ++i) du2[i] = dd{0.0, 0.0};
-  for (int i = 0; i + 1 < n; ++i) {
-    if (dd_ge(dd_abs(dg[i]), dd_abs(dl[i]))) {
-      if (dd_zero_p(dg[i])) dg[i] = tiny;
-      dd fact = dd_div(dl[i], dg[i]);
-      dg[i + 1] = dd_sub(dg[i + 1], dd_mul(fact, du[i]));
-      b[i + 1] = dd_sub(b[i + 1], dd_mul(fact, b[i]));
-      if (i < n - 2) du2[i] = dd{0.0, 0.0};
+    if (!swp[i]) {
+      b[i + 1] = dd_sub(b[i + 1], dd_mul_d(b[i], mul[i]));
     } else {
-      dd fact = dd_div(dg[i], dl[i]);
-      dd ndgi = dl[i];
-      dd ndui = dg[i + 1];
-      dd ndgi1 = dd_sub(du[i], dd_mul(fact, dg[i + 1]));
-      if (i < n - 2) {
-        du2[i] = du[i + 1];
-        du[i + 1] = dd_neg(dd_mul(fact, du[i + 1]));
-      }
-      dg[i] = ndgi;
-      du[i] = ndui;
-      dg[i + 1] = ndgi1;
-      dd bi = b[i], bi1 = b[i + 1];
+      const dd bi = b[i], bi1 = b[i + 1];
       b[i] = bi1;
-      b[i + 1] = dd_sub(bi, dd_mul(fact, bi1));
+      b[i + 1] = dd_sub(bi, dd_mul_d(bi1, mul[i]));
     }
   }
-  if (dd_zero_p(dg[n - 1])) dg[n - 1] = tiny;
-  b[n - 1] = dd_div(b[n - 1], dg[n - 1]);
-  if (n >= 2) b[n - 2] = dd_div(dd_sub(b[n - 2], dd_mul(du[n - 2], b[n - 1])), dg[n - 2]);
+  b[n - 1] = dd_div_dbl(b[n - 1], dg[n - 1]);
+  if (n >= 2) b[n - 2] = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b[n - 1], du[n - 2])), dg[n - 2]);
   for (int i = n - 3; i >= 0; --i) {
-    dd t = dd_sub(b[i], dd_mul(du[i], b[i + 1]));
-    t = dd_sub(t, dd_mul(du2[i], b[i + 2]));
-    b[i] = dd_div(t, dg[i]);
+    dd t = dd_sub(b[i], dd_mul_d(b[i + 1], du[i]));
+    t = dd_sub(t, dd_mul_d(b[i + 2], du2[i]));
+    b[i] = dd_div_dbl(t, dg[i]);
   }
 }
 
@@ -158,7 +151,12 @@ __device__ __forceinline__ int sturm_count(int n, const double* d, const double*
 
 // Lowest eigenpair of the symmetric n x n matrix in A (destroyed).  Leaves
 // lambda in L.sc[0] and the normalised dd eigenvector in L.q.
-__device__ void lowest_eigpair(EigLds& L, double* A, int n) {
+#define EIG_STAMP(i) \
+  do {                 \
+    if (threadIdx.x == 0 && prof) prof[i] = (long long)wall_clock64(); \
+  } while (0)
+
+__device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long long* prof) {
   const int tid = threadIdx.x;
   const int bs = blockDim.x;
   if (n == 1) {
@@ -220,6 +218,7 @@ __device__ void lowest_eigpair(EigLds& L, double* A, int n) {
     }
     __syncthreads();
   }
+  EIG_STAMP(3);
   if (tid == 0) {
     L.d[n - 2] = A[(n - 2) * n + (n - 2)];
     L.d[n - 1] = A[(n - 1) * n + (n - 1)];
@@ -245,21 +244,22 @@ __device__ void lowest_eigpair(EigLds& L, double* A, int n) {
     L.iv[0] = 0;
   }
   __syncthreads();
-  // Multisection: thread t evaluates the count at lo + (t+1) * (hi-lo)/257.
+  // Multisection: 256 points lo + (t+1) * (hi-lo)/257 per round, spread over
+  // the threads; the new interval is bracketed by the first point whose
+  // Sturm count is >= 1.
   for (int round = 0; round < 16; ++round) {
     const double lo = L.sc[1], hi = L.sc[2];
     const double w = (hi - lo) / 257.0;
-    bool hit = false;
-    if (tid < 256) {
-      const double xt = lo + (double)(tid + 1) * w;
-      hit = sturm_count(n, L.d, L.t0, L.sc[3], xt) >= 1;
+    int first = 256;
+    for (int t = tid; t < 256; t += bs) {
+      const double xt = lo + (double)(t + 1) * w;
+      if (first == 256 && sturm_count(n, L.d, L.t0, L.sc[3], xt) >= 1) first = t;
     }
-    unsigned long long bal = __ballot(hit);
-    if ((tid & 63) == 0) L.wmin[tid >> 6] = bal ? (tid + __ffsll((long long)bal) - 1) : 256;
+    L.tmin[tid] = first;
     __syncthreads();
     if (tid == 0) {
       int j = 256;
-      for (int q = 0; q < (int)(bs >> 6); ++q) j = min(j, L.wmin[q]);
+      for (int q = 0; q < bs; ++q) j = min(j, L.tmin[q]);
       double nlo = (j == 0) ? lo : lo + (double)j * w;
       double nhi = (j == 256) ? hi : lo + (double)(j + 1) * w;
       if (nlo == lo && nhi == hi) {
@@ -272,21 +272,25 @@ __device__ void lowest_eigpair(EigLds& L, double* A, int n) {
     __syncthreads();
     if (L.iv[0]) break;
   }
+  EIG_STAMP(4);
   if (tid == 0) {
     const double lo = L.sc[1], hi = L.sc[2];
     const double lam = lo + 0.5 * (hi - lo);
     L.sc[0] = lam;
     for (int i = 0; i < n; ++i) L.y[i] = 1.0;
+    tri_factor(n, L.d, L.e, lam, L.t1, L.t2, L.t3, L.t4, L.sw, L.col);
     for (int it = 0; it < 3; ++it) {
-      tri_solve_d(n, L.d, L.e, lam, L.y, L.t0, L.t1, L.t2, L.t3);
+      tri_solve_d(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.y);
       double mx = 0.0;
       for (int i = 0; i < n; ++i) mx = fmax(mx, __builtin_fabs(L.y[i]));
       for (int i = 0; i < n; ++i) L.y[i] = L.y[i] / mx;
     }
     for (int i = 0; i < n; ++i) L.q[i] = mkdd(L.y[i]);
-    tri_solve_dd(n, L.d, L.e, lam - 1e-14, L.q, L.u0, L.u1, L.u2, L.u3);
+    tri_factor(n, L.d, L.e, lam - 1e-14, L.t1, L.t2, L.t3, L.t4, L.sw, L.col);
+    tri_solve_dd(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.q);
   }
   __syncthreads();
+  EIG_STAMP(5);
   // Back-transform q <- H_0 ... H_{n-3} q in double-double.
   for (int s = n - 3; s >= 0; --s) {
     const double ts = L.tau[s];
@@ -299,6 +303,7 @@ __device__ void lowest_eigpair(EigLds& L, double* A, int n) {
     for (int i = tid; i < m; i += bs) L.q[s + 1 + i] = dd_sub(L.q[s + 1 + i], dd_mul_d(t, vs[i]));
     __syncthreads();
   }
+  EIG_STAMP(6);
   const dd nrm2 = red64_dd(n, [&](int i) { return dd_mul(L.q[i], L.q[i]); });
   const dd nrm = dd_sqrt(nrm2);
   __syncthreads();
@@ -324,6 +329,8 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   __shared__ EigLds L;
   __shared__ double Alds[kEigLdsN * kEigLdsN];
   if (st->error) return;
+  long long* prof = st->prof;
+  EIG_STAMP(0);
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int i = tid; i < k; i += bs) {
     L.x[i] = px[i];
@@ -354,7 +361,6 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   const double yy = L.sc[4], y1 = L.sc[5], a00 = L.sc[6];
   const bool normal = __builtin_fabs(yy) > 1e-14;
   const int n = normal ? k + 1 : k;
-  double* A = (n <= kEigLdsN) ? Alds : Aglob;
   auto Aorig = [&](int r, int c) -> double {
     if (!normal) return Hsym(r, c);
     if (r == 0 && c == 0) return a00;
@@ -362,9 +368,20 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     if (c == 0) return (L.z[r - 1] - L.Hx[r - 1]) / y1;
     return Hsym(r - 1, c - 1);
   };
-  for (int idx = tid; idx < n * n; idx += bs) A[idx] = Aorig(idx / n, idx % n);
-  __syncthreads();
-  lowest_eigpair(L, A, n);
+  EIG_STAMP(1);
+  // Separate inlined calls keep the LDS case on ds_* instructions.
+  if (n <= kEigLdsN) {
+    for (int idx = tid; idx < n * n; idx += bs) Alds[idx] = Aorig(idx / n, idx % n);
+    __syncthreads();
+    EIG_STAMP(2);
+    lowest_eigpair(L, Alds, n, prof);
+  } else {
+    for (int idx = tid; idx < n * n; idx += bs) Aglob[idx] = Aorig(idx / n, idx % n);
+    __syncthreads();
+    EIG_STAMP(2);
+    lowest_eigpair(L, Aglob, n, prof);
+  }
+  EIG_STAMP(7);
   const double lam = L.sc[0];
   if (normal) {
     for (int i = tid; i < n; i += bs) {
@@ -427,6 +444,7 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       st->sc_lo = 0.0;
     }
   }
+  EIG_STAMP(8);
   if (mode != 0) return;
   __syncthreads();
   // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
@@ -468,11 +486,15 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     st->x_size += n;
     st->k_cur = k;
   }
+  EIG_STAMP(9);
 }
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
-  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kWG), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
+  // One wavefront covers every Householder column up to k = 64 and needs no
+  // cross-wave barriers; larger problems use four.
+  const int threads = k <= 64 ? 64 : kWG;
+  hipLaunchKernelGGL(k_eig, dim3(1), dim3(threads), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
 }
 
 }  // namespace mc

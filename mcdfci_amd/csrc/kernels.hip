@@ -603,20 +603,19 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   __shared__ uint64_t ka[kSortCap];
   __shared__ uint64_t kb[kSortCap];
   __shared__ uint32_t pl[kSortCap];
-  __shared__ int offs[1024];
+  __shared__ int offs[1025];
   __shared__ int total;
   __shared__ bool last;
   if (st->error) return;
   const int t = blockIdx.x;
   const int k = gridDim.x;
   const int mw = M < kChunk ? M : kChunk;
+  for (int c = threadIdx.x; c < nchunks; c += blockDim.x) offs[c + 1] = stage_cnt[t * nchunks + c];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int c = 0; c < nchunks; ++c) {
-      offs[c] = acc;
-      acc += stage_cnt[t * nchunks + c];
-    }
-    total = acc;
+    offs[0] = 0;
+    for (int c = 1; c <= nchunks; ++c) offs[c] += offs[c - 1];
+    total = offs[nchunks];
   }
   __syncthreads();
   int n = total;
@@ -626,14 +625,17 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   }
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
-  for (int c = 0; c < nchunks; ++c) {
-    int cc = stage_cnt[t * nchunks + c];
-    const StageEnt* src = stage + ((size_t)t * nchunks + c) * mw;
-    for (int i = threadIdx.x; i < cc; i += blockDim.x) {
-      ka[offs[c] + i] = src[i].val;
-      kb[offs[c] + i] = src[i].det;
-      pl[offs[c] + i] = src[i].slot;
+  for (int g = threadIdx.x; g < n; g += blockDim.x) {
+    int lo = 0, hi = nchunks;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (offs[mid] <= g) lo = mid;
+      else hi = mid;
     }
+    const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
+    ka[g] = e.val;
+    kb[g] = e.det;
+    pl[g] = e.slot;
   }
   for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
     ka[i] = 0;
@@ -731,17 +733,15 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   __shared__ uint64_t ka[kSortCap];
   __shared__ uint64_t kb[kSortCap];
   __shared__ uint32_t pl[kSortCap];
-  __shared__ int offs[kMaxK + 1];
+  __shared__ int offs[kMaxK + 2];
   __shared__ int flag[kSortCap];
   if (st->error) return;
   const int kp = st->k_prev;
+  for (int c = threadIdx.x; c < kp; c += blockDim.x) offs[c + 1] = cand_cnt[c];
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int c = 0; c < kp; ++c) {
-      offs[c] = acc;
-      acc += cand_cnt[c];
-    }
-    offs[kp] = acc;
+    offs[0] = 0;
+    for (int c = 1; c <= kp; ++c) offs[c] += offs[c - 1];
   }
   __syncthreads();
   const int n = offs[kp];
@@ -752,15 +752,21 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   const dd s = load_xss(st);
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
-  for (int c = 0; c < kp; ++c) {
-    for (int i = threadIdx.x; i < cand_cnt[c]; i += blockDim.x) {
-      const CandEnt ce = cand[(size_t)c * M + i];
-      const Slot& sl = tab[ce.slot];
-      double z = fix_to_d(fix_join(sl.za, sl.zb));
-      ka[offs[c] + i] = bitsd(grad_key(s, sl.x, z));
-      kb[offs[c] + i] = ce.det;
-      pl[offs[c] + i] = ce.slot;
+  // Flattened refetch: candidate g belongs to the column c with
+  // offs[c] <= g < offs[c+1] (binary search), so all loads are in flight at once.
+  for (int g = threadIdx.x; g < n; g += blockDim.x) {
+    int lo = 0, hi = kp;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (offs[mid] <= g) lo = mid;
+      else hi = mid;
     }
+    const CandEnt ce = cand[(size_t)lo * M + (g - offs[lo])];
+    const Slot& sl = tab[ce.slot];
+    double z = fix_to_d(fix_join(sl.za, sl.zb));
+    ka[g] = bitsd(grad_key(s, sl.x, z));
+    kb[g] = ce.det;
+    pl[g] = ce.slot;
   }
   for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
     ka[i] = 0;

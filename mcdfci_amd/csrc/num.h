@@ -9,8 +9,9 @@
 //    add/mul/div/sqrt of Hida, Li & Bailey 2000) for the scalars xx, scale
 //    and the subspace refinement -- the reference keeps these in binary128
 //    (src/config.h:44-51, src/wavefunction.h:163-198, src/solver.h:455-538);
-//  * an exact fixed-point format with 2^-72 units for z, new_z and x.z, so
-//    that scatter-adds from many columns commute exactly;
+//  * an exact fixed-point format with 2^-64 units for z, new_z and x.z, so
+//    that scatter-adds from many columns commute exactly (z within +-2^31,
+//    at most 2^32 adds into one slot between two reinserts);
 //  * the 64-lane reduction order used by every non-exact reduction.
 #pragma once
 
@@ -87,6 +88,16 @@ MC_HD dd dd_div(dd a, dd b) {
 
 MC_HD dd dd_div_d(dd a, double b) { return dd_div(a, mkdd(b)); }
 
+// double-double divided by a double (QD's div(dd, double)).
+MC_HD dd dd_div_dbl(dd a, double b) {
+  double q1 = a.hi / b;
+  dd p = two_prod(q1, b);
+  dd s = two_sum(a.hi, -p.hi);
+  double sl = (s.lo - p.lo) + a.lo;
+  double q2 = (s.hi + sl) / b;
+  return fast_two_sum(q1, q2);
+}
+
 MC_HD dd dd_sqrt(dd a) {
   if (!(a.hi > 0.0)) return dd{__builtin_sqrt(a.hi), 0.0};
   double r = 1.0 / __builtin_sqrt(a.hi);
@@ -111,15 +122,21 @@ MC_HD bool dd_abs_gt(dd a, double t) {
 }
 
 // ---------------------------------------------------------------- fixed point
-constexpr int kFixBits = 72;
+constexpr int kFixBits = 64;
 
-// round-half-even(d * 2^72) for |d| < 2^22
+// round-half-even(d * 2^64), exact for |d| < 2^62
 MC_HD i128 to_fix(double d) {
   double y = __builtin_ldexp(d, kFixBits);
   if (__builtin_fabs(y) < 4503599627370496.0) return (i128)(int64_t)__builtin_rint(y);
-  double yh = __builtin_floor(__builtin_ldexp(y, -32));
-  double yl = y - __builtin_ldexp(yh, 32);
-  return (((i128)(int64_t)yh) << 32) + (i128)(int64_t)yl;
+  // |y| >= 2^52: y is an integer; split |y| exactly into 64 + 32 + 32 bits.
+  const bool neg = y < 0.0;
+  const double ay = neg ? -y : y;
+  const double yh = __builtin_floor(__builtin_ldexp(ay, -64));
+  const double ym = ay - __builtin_ldexp(yh, 64);  // in [0, 2^64), exact
+  const double ymh = __builtin_floor(__builtin_ldexp(ym, -32));
+  const double yml = ym - __builtin_ldexp(ymh, 32);  // in [0, 2^32), exact
+  const i128 v = (((i128)(int64_t)yh) << 64) + (((i128)(int64_t)ymh) << 32) + (i128)(int64_t)yml;
+  return neg ? -v : v;
 }
 
 MC_HD double fix_to_d_pos(i128 v) {
@@ -139,7 +156,7 @@ MC_HD dd fix_to_dd(i128 v) {
   return fast_two_sum(h, fix_to_d(v - to_fix(h)));
 }
 
-// A table z is stored as two int64 accumulators: z = (A * 2^32 + B) * 2^-72.
+// A table z is stored as two int64 accumulators: z = (A * 2^32 + B) * 2^-64.
 // Adding v splits it so that both parts can be added with 64-bit atomics.
 MC_HD void fix_split(i128 v, int64_t& a, int64_t& b) {
   int64_t lo = (int64_t)(int32_t)(uint32_t)(uint64_t)v;

@@ -14,7 +14,7 @@
 //   * update_x / update_z_and_get_sub             src/wavefunction.h:425-605
 //   * the iteration loop and report               src/solver.h:672-893
 // with the reference's thread races (k > 1) fixed to one canonical order:
-//   - z is an exact fixed-point accumulator (2^-72 units), so the result of
+//   - z is an exact fixed-point accumulator (2^-64 units), so the result of
 //     the scatter-add does not depend on the order of the adds;
 //   - an element whose dz fails the compression gate updates a determinant
 //     only if that determinant was in the table before this iteration;
@@ -663,6 +663,13 @@ struct Solver {
     for (int t = 0; t < k; ++t) h->column(pd[t], cols[t]);
     double t3 = omp_get_wtime();
     std::vector<double> dx(k);
+    last_pd = pd;
+    last_px = px;
+    last_pz = pz;
+    last_Hs = Hs;
+    last_xx = xx;
+    last_sc = sc;
+    last_xz = xz;
     int rc = eig_update(k, Hs.data(), px.data(), pz.data(), xx, sc, fixed_to_dd(xz), dx.data(), st);
     if (rc != EIG_OK) {
       error = 2;
@@ -689,6 +696,12 @@ struct Solver {
   }
 
   double energy() const { return dd_div(fixed_to_dd(xz), xx).hi; }
+
+  // inputs of the last eig update (diagnostics)
+  std::vector<Det> last_pd;
+  std::vector<double> last_px, last_pz, last_Hs;
+  dd last_xx{0, 0}, last_sc{0, 0};
+  i128 last_xz = 0;
 };
 
 struct Ctx {
@@ -792,7 +805,8 @@ void oc_lowest_eig(int n, const double* A, double* lambda, double* q_hi, double*
 // out the same layout (xz unchanged).  Returns 0 ok, 1 non-negative eigenvalue.
 int oc_eig_update(int k, const double* Hs, const double* x, const double* z, double* s, double* dx,
                   double* stats) {
-  dd xx{s[0], s[1]}, sc{s[2], s[3]}, xz{s[4], s[5]};
+  dd xx{s[0], s[1]}, sc{s[2], s[3]};
+  dd xz = fixed_to_dd(to_fixed(s[4]) + to_fixed(s[5]));
   EigStats st;
   int rc = eig_update(k, Hs, x, z, xx, sc, xz, dx, st);
   s[0] = xx.hi;
@@ -902,6 +916,31 @@ long oc_candidates(void* p, uint64_t* dets, double* x, double* z, long cap) {
     z[i] = cs[i].z;
   }
   return (long)cs.size();
+}
+
+// Inputs and output of the last iteration's eig update: dets[k], x[k],
+// z[k], Hs[k*k], s6 = {xx.hi, xx.lo, sc.hi, sc.lo, xz_fixed_hi, xz_fixed_lo}, dx[k].
+int oc_last_step(void* p, uint64_t* dets, double* x, double* z, double* Hs, double* s6, double* dx) {
+  Ctx* c = (Ctx*)p;
+  Solver& s = c->s;
+  int k = (int)s.last_pd.size();
+  int nw = c->h.nw;
+  for (int i = 0; i < k; ++i) {
+    det_out(s.last_pd[i], nw, dets + i * nw);
+    x[i] = s.last_px[i];
+    z[i] = s.last_pz[i];
+    dx[i] = s.dx_last[i];
+  }
+  for (int i = 0; i < k * k; ++i) Hs[i] = s.last_Hs[i];
+  s6[0] = s.last_xx.hi;
+  s6[1] = s.last_xx.lo;
+  s6[2] = s.last_sc.hi;
+  s6[3] = s.last_sc.lo;
+  int64_t hi = (int64_t)(s.last_xz >> 64);
+  uint64_t lo = (uint64_t)s.last_xz;
+  std::memcpy(&s6[4], &hi, 8);
+  std::memcpy(&s6[5], &lo, 8);
+  return k;
 }
 
 // Wavefunction-level hooks (update_x / update_z on caller-chosen picks).

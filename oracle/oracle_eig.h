@@ -9,12 +9,13 @@
 //   1. Householder tridiagonalisation (LAPACK dsytd2 conventions),
 //   2. Sturm-count multisection (256 points per round) for the lowest
 //      eigenvalue of the tridiagonal matrix,
-//   3. three steps of inverse iteration on the tridiagonal matrix in double
-//      (dgtsv-style elimination with partial pivoting),
-//   4. the reference's extended-precision refinement step, here one
-//      inverse-iteration solve with shift lambda - 1e-14 (solver.h:439) in
-//      double-double, back-transformed with the Householder vectors in
-//      double-double and normalised.
+//   3. three steps of inverse iteration on the tridiagonal matrix in double,
+//      sharing one LU factorisation with partial pivoting (dgttrf style),
+//   4. the reference's extended-precision refinement step: one more solve
+//      with shift lambda - 1e-14 (solver.h:439) using double LU factors and
+//      a double-double vector (as the reference uses double LU factors with
+//      binary128 vectors, solver.h:437-503), back-transformed with the
+//      Householder vectors in double-double and normalised.
 // Reductions use the canonical 64-lane order of oracle_num.h.
 #pragma once
 
@@ -34,89 +35,91 @@ static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
 static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
 static inline bool dd_ge(dd a, dd b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
 
-// Tridiagonal solve (T - shift) y = b, T = tridiag(e, d, e), dgtsv-style.
-static void tri_solve_d(int n, const double* d, const double* e, double shift, double* b) {
+// LU factorisation with partial pivoting of T - shift*I, T = tridiag(e, d, e)
+// (LAPACK dgttrf conventions: row i and i+1 swap when |sub| > |diag|).
+struct TriLU {
+  std::vector<double> dg, du, du2, mul;
+  std::vector<char> swp;
+};
+
+static TriLU tri_factor(int n, const double* d, const double* e, double shift) {
   const double tiny = 1e-290;
-  std::vector<double> dg(n), du(n > 1 ? n - 1 : 1), dl(n > 1 ? n - 1 : 1), du2(n > 2 ? n - 2 : 1, 0.0);
-  for (int i = 0; i < n; ++i) dg[i] = d[i] - shift;
+  TriLU F;
+  F.dg.assign(n, 0.0);
+  F.du.assign(n > 1 ? n - 1 : 1, 0.0);
+  F.du2.assign(n > 2 ? n - 2 : 1, 0.0);
+  F.mul.assign(n > 1 ? n - 1 : 1, 0.0);
+  F.swp.assign(n > 1 ? n - 1 : 1, 0);
+  std::vector<double> dl(n > 1 ? n - 1 : 1);
+  for (int i = 0; i < n; ++i) F.dg[i] = d[i] - shift;
   for (int i = 0; i + 1 < n; ++i) {
-    du[i] = e[i];
+    F.du[i] = e[i];
     dl[i] = e[i];
   }
   for (int i = 0; i + 1 < n; ++i) {
-    if (std::fabs(dg[i]) >= std::fabs(dl[i])) {
-      if (dg[i] == 0.0) dg[i] = tiny;
-      double fact = dl[i] / dg[i];
-      dg[i + 1] = dg[i + 1] - fact * du[i];
-      b[i + 1] = b[i + 1] - fact * b[i];
-      if (i < n - 2) du2[i] = 0.0;
+    if (std::fabs(F.dg[i]) >= std::fabs(dl[i])) {
+      if (F.dg[i] == 0.0) F.dg[i] = tiny;
+      double m = dl[i] / F.dg[i];
+      F.mul[i] = m;
+      F.swp[i] = 0;
+      F.dg[i + 1] = F.dg[i + 1] - m * F.du[i];
     } else {
-      double fact = dg[i] / dl[i];
+      double m = F.dg[i] / dl[i];
+      F.mul[i] = m;
+      F.swp[i] = 1;
       double ndgi = dl[i];
-      double ndui = dg[i + 1];
-      double ndgi1 = du[i] - fact * dg[i + 1];
+      double ndui = F.dg[i + 1];
+      double ndgi1 = F.du[i] - m * F.dg[i + 1];
       if (i < n - 2) {
-        du2[i] = du[i + 1];
-        du[i + 1] = -fact * du[i + 1];
+        F.du2[i] = F.du[i + 1];
+        F.du[i + 1] = -m * F.du[i + 1];
       }
-      dg[i] = ndgi;
-      du[i] = ndui;
-      dg[i + 1] = ndgi1;
-      double bi = b[i], bi1 = b[i + 1];
-      b[i] = bi1;
-      b[i + 1] = bi - fact * bi1;
+      F.dg[i] = ndgi;
+      F.du[i] = ndui;
+      F.dg[i + 1] = ndgi1;
     }
   }
-  if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
-  b[n - 1] = b[n - 1] / dg[n - 1];
-  if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
+  if (F.dg[n - 1] == 0.0) F.dg[n - 1] = tiny;
+  return F;
+}
+
+static void tri_solve_d(int n, const TriLU& F, double* b) {
+  for (int i = 0; i + 1 < n; ++i) {
+    if (!F.swp[i]) {
+      b[i + 1] = b[i + 1] - F.mul[i] * b[i];
+    } else {
+      double bi = b[i], bi1 = b[i + 1];
+      b[i] = bi1;
+      b[i + 1] = bi - F.mul[i] * bi1;
+    }
+  }
+  b[n - 1] = b[n - 1] / F.dg[n - 1];
+  if (n >= 2) b[n - 2] = (b[n - 2] - F.du[n - 2] * b[n - 1]) / F.dg[n - 2];
   for (int i = n - 3; i >= 0; --i) {
-    double t = b[i] - du[i] * b[i + 1];
-    t = t - du2[i] * b[i + 2];
-    b[i] = t / dg[i];
+    double t = b[i] - F.du[i] * b[i + 1];
+    t = t - F.du2[i] * b[i + 2];
+    b[i] = t / F.dg[i];
   }
 }
 
-static void tri_solve_dd(int n, const double* d, const double* e, double shift, dd* b) {
-  const dd tiny{1e-290, 0.0};
-  std::vector<dd> dg(n), du(n > 1 ? n - 1 : 1), dl(n > 1 ? n - 1 : 1),
-      du2(n > 2 ? n - 2 : 1, dd{0.0, 0.0});
-  for (int i = 0; i < n; ++i) dg[i] = two_sum(d[i], -shift);
+// Substitution with double factors and a double-double right-hand side
+// (the reference's double LU with binary128 vectors, src/solver.h:459-478).
+static void tri_solve_dd(int n, const TriLU& F, dd* b) {
   for (int i = 0; i + 1 < n; ++i) {
-    du[i] = dd_make(e[i]);
-    dl[i] = dd_make(e[i]);
-  }
-  for (int i = 0; i + 1 < n; ++i) {
-    if (dd_ge(dd_abs(dg[i]), dd_abs(dl[i]))) {
-      if (dd_is_zero(dg[i])) dg[i] = tiny;
-      dd fact = dd_div(dl[i], dg[i]);
-      dg[i + 1] = dd_sub(dg[i + 1], dd_mul(fact, du[i]));
-      b[i + 1] = dd_sub(b[i + 1], dd_mul(fact, b[i]));
-      if (i < n - 2) du2[i] = dd{0.0, 0.0};
+    if (!F.swp[i]) {
+      b[i + 1] = dd_sub(b[i + 1], dd_mul_d(b[i], F.mul[i]));
     } else {
-      dd fact = dd_div(dg[i], dl[i]);
-      dd ndgi = dl[i];
-      dd ndui = dg[i + 1];
-      dd ndgi1 = dd_sub(du[i], dd_mul(fact, dg[i + 1]));
-      if (i < n - 2) {
-        du2[i] = du[i + 1];
-        du[i + 1] = dd_neg(dd_mul(fact, du[i + 1]));
-      }
-      dg[i] = ndgi;
-      du[i] = ndui;
-      dg[i + 1] = ndgi1;
       dd bi = b[i], bi1 = b[i + 1];
       b[i] = bi1;
-      b[i + 1] = dd_sub(bi, dd_mul(fact, bi1));
+      b[i + 1] = dd_sub(bi, dd_mul_d(bi1, F.mul[i]));
     }
   }
-  if (dd_is_zero(dg[n - 1])) dg[n - 1] = tiny;
-  b[n - 1] = dd_div(b[n - 1], dg[n - 1]);
-  if (n >= 2) b[n - 2] = dd_div(dd_sub(b[n - 2], dd_mul(du[n - 2], b[n - 1])), dg[n - 2]);
+  b[n - 1] = dd_div_dbl(b[n - 1], F.dg[n - 1]);
+  if (n >= 2) b[n - 2] = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b[n - 1], F.du[n - 2])), F.dg[n - 2]);
   for (int i = n - 3; i >= 0; --i) {
-    dd t = dd_sub(b[i], dd_mul(du[i], b[i + 1]));
-    t = dd_sub(t, dd_mul(du2[i], b[i + 2]));
-    b[i] = dd_div(t, dg[i]);
+    dd t = dd_sub(b[i], dd_mul_d(b[i + 1], F.du[i]));
+    t = dd_sub(t, dd_mul_d(b[i + 2], F.du2[i]));
+    b[i] = dd_div_dbl(t, F.dg[i]);
   }
 }
 
@@ -225,19 +228,25 @@ static EigPair lowest_eigpair(int n, std::vector<double> A) {
   double lam = tri_lowest(n, d.data(), e.data());
   out.lambda = lam;
 
-  // Inverse iteration on T in double.
+  // Inverse iteration on T in double: one factorisation, three solves.
   std::vector<double> y(n, 1.0);
-  for (int it = 0; it < 3; ++it) {
-    tri_solve_d(n, d.data(), e.data(), lam, y.data());
-    double mx = 0.0;
-    for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(y[i]));
-    for (int i = 0; i < n; ++i) y[i] = y[i] / mx;
+  {
+    TriLU F1 = tri_factor(n, d.data(), e.data(), lam);
+    for (int it = 0; it < 3; ++it) {
+      tri_solve_d(n, F1, y.data());
+      double mx = 0.0;
+      for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(y[i]));
+      for (int i = 0; i < n; ++i) y[i] = y[i] / mx;
+    }
   }
 
   // Extended-precision refinement step (shift lambda - 1e-14, solver.h:439).
   std::vector<dd> q(n);
   for (int i = 0; i < n; ++i) q[i] = dd_make(y[i]);
-  tri_solve_dd(n, d.data(), e.data(), lam - 1e-14, q.data());
+  {
+    TriLU F2 = tri_factor(n, d.data(), e.data(), lam - 1e-14);
+    tri_solve_dd(n, F2, q.data());
+  }
   for (int s = n - 3; s >= 0; --s) {
     if (tau[s] == 0.0) continue;
     int m = n - s - 1;

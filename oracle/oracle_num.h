@@ -88,6 +88,17 @@ static inline dd dd_div(dd a, dd b) {
 
 static inline dd dd_div_d(dd a, double b) { return dd_div(a, dd_make(b)); }
 
+// double-double divided by a double (QD's div(dd, double)).
+static inline dd dd_div_dbl(dd a, double b) {
+  double q1 = a.hi / b;
+  dd p = two_prod(q1, b);
+  dd s = two_sum(a.hi, -p.hi);
+  double sl = s.lo - p.lo;
+  sl = sl + a.lo;
+  double q2 = (s.hi + sl) / b;
+  return quick_two_sum(q1, q2);
+}
+
 static inline dd dd_sqrt(dd a) {
   if (!(a.hi > 0.0)) return dd{std::sqrt(a.hi), 0.0};  // 0 stays 0, negative gives NaN
   double x = 1.0 / std::sqrt(a.hi);
@@ -111,18 +122,24 @@ static inline bool dd_abs_gt(dd a, double t) {
 }
 
 // ---- exact fixed point: value = v * 2^-FIX_BITS ----------------------------
-static const int FIX_BITS = 72;
+static const int FIX_BITS = 64;
 
-// Round-half-even of d * 2^72 (|d| < 2^22).
+// Round-half-even of d * 2^64, exact for |d| < 2^62.
 static inline i128 to_fixed(double d) {
   double y = std::ldexp(d, FIX_BITS);
   if (std::fabs(y) < 4503599627370496.0) {
     double r = std::rint(y);
     return (i128)(int64_t)r;
   }
-  double yh = std::floor(std::ldexp(y, -32));
-  double yl = y - std::ldexp(yh, 32);
-  return (((i128)(int64_t)yh) << 32) + (i128)(int64_t)yl;
+  // |y| >= 2^52: y is an integer; split |y| exactly into 64 + 32 + 32 bits.
+  const bool neg = y < 0.0;
+  const double ay = neg ? -y : y;
+  const double yh = std::floor(std::ldexp(ay, -64));
+  const double ym = ay - std::ldexp(yh, 64);  // in [0, 2^64), exact
+  const double ymh = std::floor(std::ldexp(ym, -32));
+  const double yml = ym - std::ldexp(ymh, 32);  // in [0, 2^32), exact
+  const i128 v = (((i128)(int64_t)yh) << 64) + (((i128)(int64_t)ymh) << 32) + (i128)(int64_t)yml;
+  return neg ? -v : v;
 }
 
 static inline double fixed_to_double(i128 v) {

@@ -223,13 +223,13 @@ def test_fixed_point_round_trip_exact():
     rng = np.random.default_rng(1)
     o = np.zeros(2, np.int64)
     for _ in range(2000):
-        d = float(rng.normal() * 10.0 ** rng.integers(-25, 6))
+        d = float(rng.normal() * 10.0 ** rng.integers(-22, 17))
         lib.oc_to_fixed(d, o)
         v = (int(o[0]) << 64) | (int(o[1]) & ((1 << 64) - 1))
         if v >= 1 << 127:
             v -= 1 << 128
-        exact = Fraction(d) * 2 ** 72
+        exact = Fraction(d) * 2 ** 64
         # round half even to the nearest integer
         assert abs(Fraction(v) - exact) <= Fraction(1, 2)
         back = lib.oc_fixed_to_double(int(o[0]), int(o[1]))
-        assert abs(_fx(back) - Fraction(v, 2 ** 72)) <= abs(Fraction(v, 2 ** 72)) * Fraction(1, 2 ** 51)
+        assert abs(_fx(back) - Fraction(v, 2 ** 64)) <= abs(Fraction(v, 2 ** 64)) * Fraction(1, 2 ** 51)
