@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-ttt", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU (--gpus 1); the multi-GPU "
@@ -103,6 +105,17 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calibrate_copy_mb:
+        # A plain copy of known size: FETCH_SIZE/WRITE_SIZE of this kernel
+        # give the scale factor between counter units and bytes.
+        n = args.calibrate_copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 8}))
+        return
     from mcdfci_amd import Cdfci, PREC_FP64, PREC_MIXED, synth
 
     spec = synth.CONFIGS[args.workload]

@@ -59,59 +59,72 @@ struct EigLds {
 };
 
 // LU factorisation with partial pivoting of T - shift*I (dgttrf
-// conventions), by one thread; factors land in dg/du/du2/mul/swp.
+// conventions), by one thread; factors land in dg/du/du2/mul/swp.  The
+// elimination chain is carried in registers so that the LDS traffic stays
+// off the critical path.
 __device__ __forceinline__ void tri_factor(int n, const double* d, const double* e, double shift, double* dg,
-                                           double* du, double* du2, double* mul, int* swp, double* dl) {
+                                           double* du, double* du2, double* mul, int* swp) {
   const double tiny = 1e-290;
-  for (int i = 0; i < n; ++i) dg[i] = d[i] - shift;
+  double cur_dg = d[0] - shift;
+  double cur_du = n > 1 ? e[0] : 0.0;
   for (int i = 0; i + 1 < n; ++i) {
-    du[i] = e[i];
-    dl[i] = e[i];
-  }
-  for (int i = 0; i + 2 < n; ++i) du2[i] = 0.0;
-  for (int i = 0; i + 1 < n; ++i) {
-    if (__builtin_fabs(dg[i]) >= __builtin_fabs(dl[i])) {
-      if (dg[i] == 0.0) dg[i] = tiny;
-      const double m = dl[i] / dg[i];
+    const double dl = e[i];
+    double next_dg = d[i + 1] - shift;
+    double next_du = (i + 2 < n) ? e[i + 1] : 0.0;
+    if (__builtin_fabs(cur_dg) >= __builtin_fabs(dl)) {
+      if (cur_dg == 0.0) cur_dg = tiny;
+      const double m = dl / cur_dg;
       mul[i] = m;
       swp[i] = 0;
-      dg[i + 1] = dg[i + 1] - m * du[i];
+      dg[i] = cur_dg;
+      du[i] = cur_du;
+      if (i + 2 < n) du2[i] = 0.0;
+      next_dg = next_dg - m * cur_du;
     } else {
-      const double m = dg[i] / dl[i];
+      const double m = cur_dg / dl;
       mul[i] = m;
       swp[i] = 1;
-      const double ndgi = dl[i];
-      const double ndui = dg[i + 1];
-      const double ndgi1 = du[i] - m * dg[i + 1];
-      if (i < n - 2) {
-        du2[i] = du[i + 1];
-        du[i + 1] = -m * du[i + 1];
+      dg[i] = dl;
+      du[i] = next_dg;
+      const double ndg1 = cur_du - m * next_dg;
+      if (i + 2 < n) {
+        du2[i] = next_du;
+        next_du = -m * next_du;
       }
-      dg[i] = ndgi;
-      du[i] = ndui;
-      dg[i + 1] = ndgi1;
+      next_dg = ndg1;
     }
+    cur_dg = next_dg;
+    cur_du = next_du;
   }
-  if (dg[n - 1] == 0.0) dg[n - 1] = tiny;
+  if (cur_dg == 0.0) cur_dg = tiny;
+  dg[n - 1] = cur_dg;
 }
 
 __device__ __forceinline__ void tri_solve_d(int n, const double* dg, const double* du, const double* du2,
                                             const double* mul, const int* swp, double* b) {
+  double cur = b[0];
   for (int i = 0; i + 1 < n; ++i) {
+    const double nb = b[i + 1];
     if (!swp[i]) {
-      b[i + 1] = b[i + 1] - mul[i] * b[i];
+      b[i] = cur;
+      cur = nb - mul[i] * cur;
     } else {
-      const double bi = b[i], bi1 = b[i + 1];
-      b[i] = bi1;
-      b[i + 1] = bi - mul[i] * bi1;
+      b[i] = nb;
+      cur = cur - mul[i] * nb;
     }
   }
-  b[n - 1] = b[n - 1] / dg[n - 1];
-  if (n >= 2) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dg[n - 2];
+  double b1 = cur / dg[n - 1];
+  b[n - 1] = b1;
+  if (n < 2) return;
+  double b0 = (b[n - 2] - du[n - 2] * b1) / dg[n - 2];
+  b[n - 2] = b0;
   for (int i = n - 3; i >= 0; --i) {
-    double t = b[i] - du[i] * b[i + 1];
-    t = t - du2[i] * b[i + 2];
-    b[i] = t / dg[i];
+    double t = b[i] - du[i] * b0;
+    t = t - du2[i] * b1;
+    const double nb = t / dg[i];
+    b[i] = nb;
+    b1 = b0;
+    b0 = nb;
   }
 }
 
@@ -119,21 +132,52 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* dg, const doubl
 // with binary128 vectors, src/solver.h:459-478).
 __device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const double* du, const double* du2,
                                              const double* mul, const int* swp, dd* b) {
+  dd cur = b[0];
   for (int i = 0; i + 1 < n; ++i) {
+    const dd nb = b[i + 1];
     if (!swp[i]) {
-      b[i + 1] = dd_sub(b[i + 1], dd_mul_d(b[i], mul[i]));
+      b[i] = cur;
+      cur = dd_sub(nb, dd_mul_d(cur, mul[i]));
     } else {
-      const dd bi = b[i], bi1 = b[i + 1];
-      b[i] = bi1;
-      b[i + 1] = dd_sub(bi, dd_mul_d(bi1, mul[i]));
+      b[i] = nb;
+      cur = dd_sub(cur, dd_mul_d(nb, mul[i]));
     }
   }
-  b[n - 1] = dd_div_dbl(b[n - 1], dg[n - 1]);
-  if (n >= 2) b[n - 2] = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b[n - 1], du[n - 2])), dg[n - 2]);
+  dd b1 = dd_div_dbl(cur, dg[n - 1]);
+  b[n - 1] = b1;
+  if (n < 2) return;
+  dd b0 = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b1, du[n - 2])), dg[n - 2]);
+  b[n - 2] = b0;
   for (int i = n - 3; i >= 0; --i) {
-    dd t = dd_sub(b[i], dd_mul_d(b[i + 1], du[i]));
-    t = dd_sub(t, dd_mul_d(b[i + 2], du2[i]));
-    b[i] = dd_div_dbl(t, dg[i]);
+    dd t = dd_sub(b[i], dd_mul_d(b0, du[i]));
+    t = dd_sub(t, dd_mul_d(b1, du2[i]));
+    const dd nb = dd_div_dbl(t, dg[i]);
+    b[i] = nb;
+    b1 = b0;
+    b0 = nb;
+  }
+}
+
+// Sturm counts at up to four shifts at once (independent chains, same
+// arithmetic per shift as the single-shift recurrence).
+__device__ __forceinline__ void sturm_count4(int n, const double* d, const double* e2, double pivmin,
+                                             const double* x, int np, int* cnt) {
+  double q[4];
+  for (int j = 0; j < 4; ++j) {
+    q[j] = d[0] - x[j];
+    if (__builtin_fabs(q[j]) < pivmin) q[j] = -pivmin;
+    cnt[j] = q[j] < 0.0;
+  }
+  for (int i = 1; i < n; ++i) {
+    const double di = d[i], ei = e2[i - 1];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j < np) {
+        q[j] = (di - x[j]) - ei / q[j];
+        if (__builtin_fabs(q[j]) < pivmin) q[j] = -pivmin;
+        cnt[j] += q[j] < 0.0;
+      }
+    }
   }
 }
 
@@ -201,6 +245,7 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
     for (int r = tid; r < m; r += bs) {
       double acc = 0.0;
       const double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
+#pragma unroll 8
       for (int c = 0; c < m; ++c) acc = acc + cp[(size_t)c * n] * L.v[c];
       L.p[r] = tt * acc;
     }
@@ -211,6 +256,7 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
     for (int r = tid; r < m; r += bs) {
       const double vr = L.v[r], wr = L.w[r];
       double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
+#pragma unroll 8
       for (int c = 0; c < m; ++c) {
         double a = cp[(size_t)c * n];
         cp[(size_t)c * n] = a - (L.v[c] * wr + L.w[c] * vr);
@@ -251,9 +297,14 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
     const double lo = L.sc[1], hi = L.sc[2];
     const double w = (hi - lo) / 257.0;
     int first = 256;
-    for (int t = tid; t < 256; t += bs) {
-      const double xt = lo + (double)(t + 1) * w;
-      if (first == 256 && sturm_count(n, L.d, L.t0, L.sc[3], xt) >= 1) first = t;
+    {
+      const int np = (256 + bs - 1) / bs;  // points per thread (<= 4)
+      double xs[4];
+      int cnt[4];
+      for (int j = 0; j < 4; ++j) xs[j] = lo + (double)(tid + j * bs + 1) * w;
+      sturm_count4(n, L.d, L.t0, L.sc[3], xs, np, cnt);
+      for (int j = np - 1; j >= 0; --j)
+        if (tid + j * bs < 256 && cnt[j] >= 1) first = tid + j * bs;
     }
     L.tmin[tid] = first;
     __syncthreads();
@@ -278,7 +329,7 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
     const double lam = lo + 0.5 * (hi - lo);
     L.sc[0] = lam;
     for (int i = 0; i < n; ++i) L.y[i] = 1.0;
-    tri_factor(n, L.d, L.e, lam, L.t1, L.t2, L.t3, L.t4, L.sw, L.col);
+    tri_factor(n, L.d, L.e, lam, L.t1, L.t2, L.t3, L.t4, L.sw);
     for (int it = 0; it < 3; ++it) {
       tri_solve_d(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.y);
       double mx = 0.0;
@@ -286,7 +337,7 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
       for (int i = 0; i < n; ++i) L.y[i] = L.y[i] / mx;
     }
     for (int i = 0; i < n; ++i) L.q[i] = mkdd(L.y[i]);
-    tri_factor(n, L.d, L.e, lam - 1e-14, L.t1, L.t2, L.t3, L.t4, L.sw, L.col);
+    tri_factor(n, L.d, L.e, lam - 1e-14, L.t1, L.t2, L.t3, L.t4, L.sw);
     tri_solve_dd(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.q);
   }
   __syncthreads();

@@ -360,6 +360,34 @@ __device__ __forceinline__ i128 wave_sum_i128(i128 v) {
   return v;
 }
 
+// Exclusive prefix sum over the workgroup (blockDim <= 1024, multiple of
+// 64); wtot needs blockDim/64 + 1 ints of LDS.  Returns this thread's
+// prefix and the total in `total`.
+__device__ __forceinline__ int block_excl_scan(int v, int* wtot, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wtot[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < nw; ++i) {
+      int t = wtot[i];
+      wtot[i] = acc;
+      acc += t;
+    }
+    wtot[nw] = acc;
+  }
+  __syncthreads();
+  total = wtot[nw];
+  const int r = x - v + wtot[w];
+  __syncthreads();
+  return r;
+}
+
 // ------------------------------------------------------------------ k_scatter
 template <bool MIXED>
 __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const uint64_t* pdet,
@@ -610,12 +638,23 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   const int t = blockIdx.x;
   const int k = gridDim.x;
   const int mw = M < kChunk ? M : kChunk;
-  for (int c = threadIdx.x; c < nchunks; c += blockDim.x) offs[c + 1] = stage_cnt[t * nchunks + c];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    offs[0] = 0;
-    for (int c = 1; c <= nchunks; ++c) offs[c] += offs[c - 1];
-    total = offs[nchunks];
+  __shared__ int wtot[kWG / 64 + 1];
+  {
+    // nchunks <= 1024: up to four chunk counts per thread, scanned in order.
+    int v[4], sum = 0;
+    for (int j = 0; j < 4; ++j) {
+      const int c = threadIdx.x * 4 + j;
+      v[j] = c < nchunks ? stage_cnt[t * nchunks + c] : 0;
+      sum += v[j];
+    }
+    int tot;
+    int pre = block_excl_scan(sum, wtot, tot);
+    for (int j = 0; j < 4; ++j) {
+      const int c = threadIdx.x * 4 + j;
+      if (c <= nchunks) offs[c] = pre;
+      pre += v[j];
+    }
+    if (threadIdx.x == 0) total = tot;
   }
   __syncthreads();
   int n = total;
@@ -644,26 +683,15 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   }
   bitonic_desc(ka, kb, pl, np2);
   // First M entries; keep those present in the table (order preserved).
-  __shared__ int keep[kWG];
   const int m = n < M ? n : M;
   {
     const int per = (m + kWG - 1) / kWG;
     const int lo = threadIdx.x * per, hi = min(m, lo + per);
     int c = 0;
     for (int i = lo; i < hi; ++i) c += pl[i] != 0xffffffffu;
-    keep[threadIdx.x] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int acc = 0;
-      for (int i = 0; i < kWG; ++i) {
-        int v = keep[i];
-        keep[i] = acc;
-        acc += v;
-      }
-      cand_cnt[t] = acc;
-    }
-    __syncthreads();
-    int pos = keep[threadIdx.x];
+    int kept;
+    int pos = block_excl_scan(c, wtot, kept);
+    if (threadIdx.x == 0) cand_cnt[t] = kept;
     for (int i = lo; i < hi; ++i) {
       if (pl[i] == 0xffffffffu) continue;
       cand[(size_t)t * M + pos].det = kb[i];
@@ -736,12 +764,15 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   __shared__ int offs[kMaxK + 2];
   __shared__ int flag[kSortCap];
   if (st->error) return;
+  __shared__ int wtot[17];
   const int kp = st->k_prev;
-  for (int c = threadIdx.x; c < kp; c += blockDim.x) offs[c + 1] = cand_cnt[c];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    offs[0] = 0;
-    for (int c = 1; c <= kp; ++c) offs[c] += offs[c - 1];
+  {
+    // kp <= 256 < blockDim: one column count per thread.
+    const int v = threadIdx.x < kp ? cand_cnt[threadIdx.x] : 0;
+    int tot;
+    const int pre = block_excl_scan(v, wtot, tot);
+    if (threadIdx.x <= kp) offs[threadIdx.x] = pre;
+    if (threadIdx.x == 0) offs[kp] = tot;
   }
   __syncthreads();
   const int n = offs[kp];
@@ -776,26 +807,15 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   bitonic_desc(ka, kb, pl, np2);
   for (int i = threadIdx.x; i < n; i += blockDim.x) flag[i] = (i == 0 || kb[i] != kb[i - 1]) ? 1 : 0;
   __syncthreads();
-  // Exclusive prefix count of distinct entries: thread t scans its block of
-  // n/blockDim entries, then thread 0 scans the block totals.
-  __shared__ int bsum[1024];
+  // Exclusive prefix count of distinct entries over contiguous blocks.
   const int per = (n + blockDim.x - 1) / blockDim.x;
   const int lo = threadIdx.x * per, hi = min(n, lo + per);
   int cnt = 0;
   for (int i = lo; i < hi; ++i) cnt += flag[i];
-  bsum[threadIdx.x] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int t = 0; t < (int)blockDim.x; ++t) {
-      int v = bsum[t];
-      bsum[t] = acc;
-      acc += v;
-    }
-    if (acc < k) st->error = ERR_CANDIDATES;
-  }
-  __syncthreads();
-  int pos = bsum[threadIdx.x];
+  int distinct;
+  const int pos0 = block_excl_scan(cnt, wtot, distinct);
+  if (threadIdx.x == 0 && distinct < k) st->error = ERR_CANDIDATES;
+  int pos = pos0;
   for (int i = lo; i < hi && pos < k; ++i) {
     if (!flag[i]) continue;
     pdet[pos] = kb[i];
