@@ -46,25 +46,55 @@ def make_workload(name, outdir):
 
 
 def run_for(ctx, seconds, report_every, max_iters=10 ** 9):
-    """Run until `seconds` elapse; return [(t, iterations, energy)]."""
+    """Run until `seconds` elapse or the wavefunction table fills.
+
+    Returns ([(t, iterations, energy)], stop reason)."""
+    from mcdfci_amd import McdfciError
     traj = []
     t0 = time.perf_counter()
     it = 0
     while True:
         ctx.step(report_every)
-        r = ctx.report()
+        try:
+            r = ctx.report()
+        except McdfciError as e:
+            if e.code == 3:  # table full: the run ends at its last good report
+                return traj, "table full"
+            raise
         it += report_every
         t = time.perf_counter() - t0
         traj.append((t, it, r["energy"]))
         if t >= seconds or it >= max_iters:
-            return traj
+            return traj, "time"
 
 
 def first_within(traj, e_ref, tol):
+    """First report whose energy is within tol of e_ref or below it.
+
+    The CDFCI energy is a variational upper bound that decreases from report
+    to report, so a report below e_ref has passed through the window even
+    when consecutive reports are further apart than tol."""
     for t, it, e in traj:
-        if abs(e - e_ref) < tol:
+        if e - e_ref < tol:
             return t, it
     return None, None
+
+
+def measured_traffic(workload, k, precision):
+    """HBM bytes per k_scatter launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy-kernel calibration), or
+    None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", "k_scatter_c4_k64_mixed_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 6 and w[0] == "TRAFFIC" and w[1] == "k_scatter" and w[2] == workload \
+                        and int(w[3]) == k and w[4] == precision:
+                    return int(w[5])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(fcidump, k, M, z_thr, max_size, seconds, threads):
@@ -165,12 +195,12 @@ def main():
         plain = Cdfci(fcid, num_of_coordinate=args.plain_k, estimate=args.plain_k, z_threshold=z_thr,
                       max_memory=max_mem, threshold=0.0, precision=PREC_FP64)
         plain.init()
-        tp = run_for(plain, args.ttt_seconds, 200)
+        tp, stop_p = run_for(plain, args.ttt_seconds, 200)
         plain.close()
         mb2 = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem,
                     threshold=0.0, precision=PREC_MIXED)
         mb2.init()
-        tm = run_for(mb2, args.ttt_seconds, 20)
+        tm, stop_m = run_for(mb2, args.ttt_seconds, 20)
         mb2.close()
         e_ref = tp[-1][2]
         t_plain, i_plain = first_within(tp, e_ref, 1e-5)
@@ -192,6 +222,10 @@ def main():
             "mixed_minus_plain_at_equal_time": tm[-1][2] - e_plain_at,
             "plain_k": args.plain_k,
             "batched_k": k,
+            "plain_stop": stop_p,
+            "plain_last": tp[-1],
+            "mixed_stop": stop_m,
+            "mixed_last": tm[-1],
         }
         log("time-to-threshold: " + json.dumps(ttt))
 
@@ -226,7 +260,9 @@ def main():
         "h_elements_per_s": h_elem_rate,
         "energy_after_steps": mb_energy_steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, k, "mixed"),
+                     "algorithmic_bytes_per_launch": elems * BYTES_PER_ELEMENT,
                      "kernel": "k_scatter", "elements_per_launch": elems,
                      "bytes_per_element": BYTES_PER_ELEMENT, "avg_launch_us": 1e6 * scatter_s},
         "kernel_ms_per_iteration": {a: 1e3 * b / max(nt, 1) for a, b in kt.items()},
