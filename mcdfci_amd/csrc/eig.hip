@@ -1,18 +1,24 @@
 // Subspace update of the CDFCI iteration (src/solver.h:333-609) and
-// update_x (src/wavefunction.h:425-451) in one workgroup.
+// update_x (src/wavefunction.h:425-451) in one workgroup of 256 threads.
 //
 // The (k+1)-dimensional problem [alpha b^T; b H_sub] is solved for its
 // lowest eigenpair by
 //   1. Householder tridiagonalisation (dsytd2 conventions; the trailing
-//      block is updated column-per-thread, which keeps it bitwise symmetric),
-//   2. 256-point Sturm multisection per round for the lowest eigenvalue,
-//   3. three inverse-iteration solves on the tridiagonal matrix (double),
+//      block is updated column-per-lane, which keeps it bitwise symmetric,
+//      and the four waves take the columns c = w mod 4),
+//   2. Laguerre's iteration on det(T - lambda I) from the Gershgorin lower
+//      bound (monotone and cubically convergent for the smallest root),
+//      safeguarded by a bracket,
+//   3. three inverse-iteration solves on the tridiagonal matrix (double)
+//      with one LU factorisation and reciprocal pivots,
 //   4. the extended-precision refinement step of the reference
 //      (src/solver.h:437-503) as one double-double inverse-iteration solve
 //      with shift lambda - 1e-14, back-transformed in double-double.
-// Sequential sums run in ascending order; parallel sums use the 64-lane
-// order (strided partials, then xor butterfly), evaluated redundantly by
-// every wave so that all threads hold the same bits.
+// Row sums of matrix-vector products use four interleaved partial sums
+// (c = j mod 4, ascending, combined as (S0 + S1) + (S2 + S3)); the other
+// parallel sums use the 64-lane order (strided partials, then xor
+// butterfly), evaluated redundantly by every wave so that all threads hold
+// the same bits.  Sequential chains run on thread 0.
 
 #include <hip/hip_runtime.h>
 
@@ -23,6 +29,7 @@ namespace mc {
 
 constexpr int kEigLdsN = 90;  // A lives in LDS up to this dimension
 constexpr int kNMax = kMaxK + 1;
+constexpr int kEigThreads = kWG;  // four waves
 
 template <class F>
 __device__ __forceinline__ double red64(int n, F term) {
@@ -45,23 +52,48 @@ __device__ __forceinline__ dd red64_dd(int n, F term) {
   return acc;
 }
 
+// max |y_i| over i < n (exact, so any order gives the same bits).
+__device__ __forceinline__ double wave_absmax(int n, const double* y) {
+  const int lane = threadIdx.x & 63;
+  double m = 0.0;
+  for (int i = lane; i < n; i += 64) m = fmax(m, __builtin_fabs(y[i]));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  return m;
+}
+
 struct EigLds {
-  double x[kMaxK], z[kMaxK], Hx[kMaxK];
+  double x[kMaxK], z[kMaxK], Hx[kMaxK], b0[kMaxK];
   double v[kNMax], p[kNMax], w[kNMax], col[kNMax];
-  double d[kNMax], e[kNMax], tau[kNMax], y[kNMax];
-  double t0[kNMax], t1[kNMax], t2[kNMax], t3[kNMax], t4[kNMax];
+  double d[kNMax], e[kNMax], e2[kNMax], tau[kNMax], y[kNMax];
+  double part[4][kNMax];
+  double dg[kNMax], du[kNMax], du2[kNMax], mul[kNMax], rdg[kNMax];
   int sw[kNMax];
   dd q[kNMax];
   double ev[kNMax], dl[kNMax], Ae[kNMax];
   double sc[16];
   int iv[8];
-  int tmin[kWG];
 };
 
+// Four-way partial row sums: thread (wave w, lane) accumulates the columns
+// c = w mod 4 of rows lane, lane + 64, ...; part[w][r] receives S_w(r).
+template <class F>
+__device__ __forceinline__ void dot4_partials(EigLds& L, int rows, int cols, F term) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = lane; r < rows; r += 64) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (int c = w; c < cols; c += 4) acc = acc + term(r, c);
+    L.part[w][r] = acc;
+  }
+}
+
+__device__ __forceinline__ double dot4_combine(const EigLds& L, int r) {
+  return (L.part[0][r] + L.part[1][r]) + (L.part[2][r] + L.part[3][r]);
+}
+
 // LU factorisation with partial pivoting of T - shift*I (dgttrf
-// conventions), by one thread; factors land in dg/du/du2/mul/swp.  The
-// elimination chain is carried in registers so that the LDS traffic stays
-// off the critical path.
+// conventions), by one thread; factors land in dg/du/du2/mul/sw.  The
+// elimination chain is carried in registers.
 __device__ __forceinline__ void tri_factor(int n, const double* d, const double* e, double shift, double* dg,
                                            double* du, double* du2, double* mul, int* swp) {
   const double tiny = 1e-290;
@@ -100,7 +132,7 @@ __device__ __forceinline__ void tri_factor(int n, const double* d, const double*
   dg[n - 1] = cur_dg;
 }
 
-__device__ __forceinline__ void tri_solve_d(int n, const double* dg, const double* du, const double* du2,
+__device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const double* du, const double* du2,
                                             const double* mul, const int* swp, double* b) {
   double cur = b[0];
   for (int i = 0; i + 1 < n; ++i) {
@@ -113,15 +145,15 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* dg, const doubl
       cur = cur - mul[i] * nb;
     }
   }
-  double b1 = cur / dg[n - 1];
+  double b1 = cur * rdg[n - 1];
   b[n - 1] = b1;
   if (n < 2) return;
-  double b0 = (b[n - 2] - du[n - 2] * b1) / dg[n - 2];
+  double b0 = (b[n - 2] - du[n - 2] * b1) * rdg[n - 2];
   b[n - 2] = b0;
   for (int i = n - 3; i >= 0; --i) {
     double t = b[i] - du[i] * b0;
     t = t - du2[i] * b1;
-    const double nb = t / dg[i];
+    const double nb = t * rdg[i];
     b[i] = nb;
     b1 = b0;
     b0 = nb;
@@ -130,8 +162,8 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* dg, const doubl
 
 // Double factors, double-double right-hand side (the reference's double LU
 // with binary128 vectors, src/solver.h:459-478).
-__device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const double* du, const double* du2,
-                                             const double* mul, const int* swp, dd* b) {
+__device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const double* rdg, const double* du,
+                                             const double* du2, const double* mul, const int* swp, dd* b) {
   dd cur = b[0];
   for (int i = 0; i + 1 < n; ++i) {
     const dd nb = b[i + 1];
@@ -143,63 +175,107 @@ __device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const doub
       cur = dd_sub(cur, dd_mul_d(nb, mul[i]));
     }
   }
-  dd b1 = dd_div_dbl(cur, dg[n - 1]);
+  dd b1 = dd_div_rcp(cur, dg[n - 1], rdg[n - 1]);
   b[n - 1] = b1;
   if (n < 2) return;
-  dd b0 = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b1, du[n - 2])), dg[n - 2]);
+  dd b0 = dd_div_rcp(dd_sub(b[n - 2], dd_mul_d(b1, du[n - 2])), dg[n - 2], rdg[n - 2]);
   b[n - 2] = b0;
   for (int i = n - 3; i >= 0; --i) {
     dd t = dd_sub(b[i], dd_mul_d(b0, du[i]));
     t = dd_sub(t, dd_mul_d(b1, du2[i]));
-    const dd nb = dd_div_dbl(t, dg[i]);
+    const dd nb = dd_div_rcp(t, dg[i], rdg[i]);
     b[i] = nb;
     b1 = b0;
     b0 = nb;
   }
 }
 
-// Sturm counts at up to four shifts at once (independent chains, same
-// arithmetic per shift as the single-shift recurrence).
-__device__ __forceinline__ void sturm_count4(int n, const double* d, const double* e2, double pivmin,
-                                             const double* x, int np, int* cnt) {
-  double q[4];
-  for (int j = 0; j < 4; ++j) {
-    q[j] = d[0] - x[j];
-    if (__builtin_fabs(q[j]) < pivmin) q[j] = -pivmin;
-    cnt[j] = q[j] < 0.0;
-  }
+// Leading-minor recurrence of T - lam*I, p_i = (d_i - lam) p_{i-1} -
+// e_{i-1}^2 p_{i-2}, with first and second lam-derivatives, rescaled by
+// exact powers of two whenever p_i leaves [2^-300, 2^300] (no division on
+// the chain).  False as soon as some p_i is not positive; otherwise
+// G = f'/f and Hh = (f'/f)^2 - f''/f for f = det(T - lam*I).
+__device__ __forceinline__ bool minor_pass(int n, const double* d, const double* e2, double lam, double& G,
+                                           double& Hh) {
+  const double big = 0x1p300, small = 0x1p-300;
+  double pm = 1.0, dpm = 0.0, ddpm = 0.0;
+  double p = d[0] - lam, dp = -1.0, ddp = 0.0;
+  if (!(p > 0.0)) return false;
+  bool ok = true;
   for (int i = 1; i < n; ++i) {
-    const double di = d[i], ei = e2[i - 1];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (j < np) {
-        q[j] = (di - x[j]) - ei / q[j];
-        if (__builtin_fabs(q[j]) < pivmin) q[j] = -pivmin;
-        cnt[j] += q[j] < 0.0;
-      }
+    const double a = d[i] - lam, b = e2[i - 1];
+    double pn = a * p - b * pm;
+    double dpn = (a * dp - b * dpm) - p;
+    double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
+    if (!(pn > 0.0)) {
+      ok = false;
+      break;
     }
+    if (pn > big) {
+      pn *= small, dpn *= small, ddpn *= small, p *= small, dp *= small, ddp *= small;
+    } else if (pn < small) {
+      pn *= big, dpn *= big, ddpn *= big, p *= big, dp *= big, ddp *= big;
+    }
+    pm = p, dpm = dp, ddpm = ddp;
+    p = pn, dp = dpn, ddp = ddpn;
   }
+  if (!ok) return false;
+  const double g = dp / p;
+  G = g;
+  Hh = g * g - ddp / p;
+  return true;
 }
 
-__device__ __forceinline__ int sturm_count(int n, const double* d, const double* e2, double pivmin, double x) {
-  double q = d[0] - x;
-  if (__builtin_fabs(q) < pivmin) q = -pivmin;
-  int c = q < 0.0;
-  for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
-    if (__builtin_fabs(q) < pivmin) q = -pivmin;
-    c += q < 0.0;
+// Lowest eigenvalue of tridiag(e, d, e): Laguerre from the Gershgorin lower
+// bound; lam always passes minor_pass.  A failed candidate is within
+// rounding of the root (the exact iteration never passes it): the point
+// 1e-13 * (|cand| + 1) below it is tried once and, if it passes, is the
+// answer; otherwise the failed points bound a bisection.
+__device__ __forceinline__ double tri_lowest(int n, const double* d, const double* e, const double* e2) {
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    double r = 0.0;
+    if (i > 0) r = r + __builtin_fabs(e[i - 1]);
+    if (i < n - 1) r = r + __builtin_fabs(e[i]);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
   }
-  return c;
+  const double pad = (hi - lo) * 1e-6 + 1e-200;
+  double lam = lo - pad;
+  hi = hi + pad;
+  const double nn = (double)n;
+  double G, Hh;
+  if (!minor_pass(n, d, e2, lam, G, Hh)) return lam;
+  for (int it = 0; it < 100; ++it) {
+    double disc = (nn - 1.0) * (nn * Hh - G * G);
+    if (!(disc > 0.0)) disc = 0.0;
+    double cand = lam - nn / (G - __builtin_sqrt(disc));
+    if (!(cand < hi)) cand = lam + 0.5 * (hi - lam);
+    if (!(cand > lam) || !(cand < hi)) break;
+    double G2, H2;
+    if (minor_pass(n, d, e2, cand, G2, H2)) {
+      lam = cand, G = G2, Hh = H2;
+      continue;
+    }
+    hi = cand;
+    const double t = cand - 1e-13 * (__builtin_fabs(cand) + 1.0);
+    if (!(t > lam)) break;
+    if (minor_pass(n, d, e2, t, G2, H2)) {
+      lam = t;
+      break;
+    }
+    hi = t;
+  }
+  return lam;
 }
 
-// Lowest eigenpair of the symmetric n x n matrix in A (destroyed).  Leaves
-// lambda in L.sc[0] and the normalised dd eigenvector in L.q.
 #define EIG_STAMP(i) \
   do {                 \
     if (threadIdx.x == 0 && prof) prof[i] = (long long)wall_clock64(); \
   } while (0)
 
+// Lowest eigenpair of the symmetric n x n matrix in A (destroyed).  Leaves
+// lambda in L.sc[0] and the normalised dd eigenvector in L.q.
 __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long long* prof) {
   const int tid = threadIdx.x;
   const int bs = blockDim.x;
@@ -242,24 +318,24 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
       L.e[s] = beta;
     }
     __syncthreads();
-    for (int r = tid; r < m; r += bs) {
-      double acc = 0.0;
-      const double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
-#pragma unroll 8
-      for (int c = 0; c < m; ++c) acc = acc + cp[(size_t)c * n] * L.v[c];
-      L.p[r] = tt * acc;
-    }
+    const double* B = A + (size_t)(s + 1) * n + (s + 1);
+    dot4_partials(L, m, m, [&](int r, int c) { return B[(size_t)c * n + r] * L.v[c]; });
+    __syncthreads();
+    for (int r = tid; r < m; r += bs) L.p[r] = tt * dot4_combine(L, r);
     __syncthreads();
     const double K = (-0.5 * tt) * red64(m, [&](int i) { return L.p[i] * L.v[i]; });
     for (int r = tid; r < m; r += bs) L.w[r] = L.p[r] + K * L.v[r];
     __syncthreads();
-    for (int r = tid; r < m; r += bs) {
-      const double vr = L.v[r], wr = L.w[r];
-      double* cp = A + (size_t)(s + 1) * n + (s + 1 + r);
-#pragma unroll 8
-      for (int c = 0; c < m; ++c) {
-        double a = cp[(size_t)c * n];
-        cp[(size_t)c * n] = a - (L.v[c] * wr + L.w[c] * vr);
+    {
+      const int w = tid >> 6, lane = tid & 63;
+      double* Bw = A + (size_t)(s + 1) * n + (s + 1);
+      for (int r = lane; r < m; r += 64) {
+        const double vr = L.v[r], wr = L.w[r];
+#pragma unroll 4
+        for (int c = w; c < m; c += 4) {
+          double a = Bw[(size_t)c * n + r];
+          Bw[(size_t)c * n + r] = a - (L.v[c] * wr + L.w[c] * vr);
+        }
       }
     }
     __syncthreads();
@@ -269,77 +345,36 @@ __device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long
     L.d[n - 2] = A[(n - 2) * n + (n - 2)];
     L.d[n - 1] = A[(n - 1) * n + (n - 1)];
     L.e[n - 2] = A[(n - 1) * n + (n - 2)];
-    // Gershgorin interval and pivot guard for the Sturm counts.
-    double emax = 1.0;
-    for (int i = 0; i + 1 < n; ++i) {
-      L.t0[i] = L.e[i] * L.e[i];
-      emax = fmax(emax, L.t0[i]);
-    }
-    double lo = L.d[0], hi = L.d[0];
-    for (int i = 0; i < n; ++i) {
-      double r = 0.0;
-      if (i > 0) r = r + __builtin_fabs(L.e[i - 1]);
-      if (i < n - 1) r = r + __builtin_fabs(L.e[i]);
-      lo = fmin(lo, L.d[i] - r);
-      hi = fmax(hi, L.d[i] + r);
-    }
-    double pad = (hi - lo) * 1e-6 + 1e-200;
-    L.sc[1] = lo - pad;
-    L.sc[2] = hi + pad;
-    L.sc[3] = 2.2250738585072014e-308 * emax;
-    L.iv[0] = 0;
   }
   __syncthreads();
-  // Multisection: 256 points lo + (t+1) * (hi-lo)/257 per round, spread over
-  // the threads; the new interval is bracketed by the first point whose
-  // Sturm count is >= 1.
-  for (int round = 0; round < 16; ++round) {
-    const double lo = L.sc[1], hi = L.sc[2];
-    const double w = (hi - lo) / 257.0;
-    int first = 256;
-    {
-      const int np = (256 + bs - 1) / bs;  // points per thread (<= 4)
-      double xs[4];
-      int cnt[4];
-      for (int j = 0; j < 4; ++j) xs[j] = lo + (double)(tid + j * bs + 1) * w;
-      sturm_count4(n, L.d, L.t0, L.sc[3], xs, np, cnt);
-      for (int j = np - 1; j >= 0; --j)
-        if (tid + j * bs < 256 && cnt[j] >= 1) first = tid + j * bs;
-    }
-    L.tmin[tid] = first;
-    __syncthreads();
-    if (tid == 0) {
-      int j = 256;
-      for (int q = 0; q < bs; ++q) j = min(j, L.tmin[q]);
-      double nlo = (j == 0) ? lo : lo + (double)j * w;
-      double nhi = (j == 256) ? hi : lo + (double)(j + 1) * w;
-      if (nlo == lo && nhi == hi) {
-        L.iv[0] = 1;
-      } else {
-        L.sc[1] = nlo;
-        L.sc[2] = nhi;
-      }
-    }
-    __syncthreads();
-    if (L.iv[0]) break;
-  }
-  EIG_STAMP(4);
+  for (int i = tid; i + 1 < n; i += bs) L.e2[i] = L.e[i] * L.e[i];
+  __syncthreads();
   if (tid == 0) {
-    const double lo = L.sc[1], hi = L.sc[2];
-    const double lam = lo + 0.5 * (hi - lo);
+    const double lam = tri_lowest(n, L.d, L.e, L.e2);
     L.sc[0] = lam;
-    for (int i = 0; i < n; ++i) L.y[i] = 1.0;
-    tri_factor(n, L.d, L.e, lam, L.t1, L.t2, L.t3, L.t4, L.sw);
-    for (int it = 0; it < 3; ++it) {
-      tri_solve_d(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.y);
-      double mx = 0.0;
-      for (int i = 0; i < n; ++i) mx = fmax(mx, __builtin_fabs(L.y[i]));
-      for (int i = 0; i < n; ++i) L.y[i] = L.y[i] / mx;
-    }
-    for (int i = 0; i < n; ++i) L.q[i] = mkdd(L.y[i]);
-    tri_factor(n, L.d, L.e, lam - 1e-14, L.t1, L.t2, L.t3, L.t4, L.sw);
-    tri_solve_dd(n, L.t1, L.t2, L.t3, L.t4, L.sw, L.q);
+    tri_factor(n, L.d, L.e, lam, L.dg, L.du, L.du2, L.mul, L.sw);
   }
+  __syncthreads();
+  EIG_STAMP(4);
+  for (int i = tid; i < n; i += bs) {
+    L.rdg[i] = 1.0 / L.dg[i];
+    L.y[i] = 1.0;
+  }
+  __syncthreads();
+  for (int it = 0; it < 3; ++it) {
+    if (tid == 0) tri_solve_d(n, L.rdg, L.du, L.du2, L.mul, L.sw, L.y);
+    __syncthreads();
+    const double mx = wave_absmax(n, L.y);
+    __syncthreads();
+    for (int i = tid; i < n; i += bs) L.y[i] = L.y[i] / mx;
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += bs) L.q[i] = mkdd(L.y[i]);
+  if (tid == 0) tri_factor(n, L.d, L.e, L.sc[0] - 1e-14, L.dg, L.du, L.du2, L.mul, L.sw);
+  __syncthreads();
+  for (int i = tid; i < n; i += bs) L.rdg[i] = 1.0 / L.dg[i];
+  __syncthreads();
+  if (tid == 0) tri_solve_dd(n, L.dg, L.rdg, L.du, L.du2, L.mul, L.sw, L.q);
   __syncthreads();
   EIG_STAMP(5);
   // Back-transform q <- H_0 ... H_{n-3} q in double-double.
@@ -388,50 +423,66 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     L.z[i] = pz[i];
   }
   __syncthreads();
-  auto Hsym = [&](int r, int c) { return r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r]; };
-  for (int r = tid; r < k; r += bs) {
-    double acc = 0.0;
-    for (int c = 0; c < k; ++c) acc = acc + Hsym(r, c) * L.x[c];
-    L.Hx[r] = acc;
-  }
-  __syncthreads();
   if (tid == 0) {
-    double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
+    double xTx = 0.0;
     for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
-    for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
-    for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
     dd xx{st->xx_hi, st->xx_lo};
-    double yy = dd_add_d(xx, -xTx).hi;
+    const double yy = dd_add_d(xx, -xTx).hi;
     L.sc[4] = yy;
     L.sc[5] = __builtin_sqrt(yy);
-    dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
-    dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
-    L.sc[6] = dd_div_d(a00, yy).hi;
   }
   __syncthreads();
-  const double yy = L.sc[4], y1 = L.sc[5], a00 = L.sc[6];
+  const double yy = L.sc[4], y1 = L.sc[5];
   const bool normal = __builtin_fabs(yy) > 1e-14;
   const int n = normal ? k + 1 : k;
+  const int o = normal ? 1 : 0;
+  double* A = (n <= kEigLdsN) ? Alds : Aglob;
+  // H_sub into the trailing block (upper triangle of Hs mirrored).
+  for (int idx = tid; idx < k * k; idx += bs) {
+    const int r = idx / k, c = idx - r * k;
+    A[(size_t)(r + o) * n + (c + o)] = r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r];
+  }
+  __syncthreads();
+  const double* Hb = A + (size_t)o * n + o;
+  dot4_partials(L, k, k, [&](int r, int c) { return Hb[(size_t)c * n + r] * L.x[c]; });
+  __syncthreads();
+  for (int r = tid; r < k; r += bs) L.Hx[r] = dot4_combine(L, r);
+  __syncthreads();
+  if (normal) {
+    if (tid == 0) {
+      double xTz = 0.0, xTHx = 0.0;
+      for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
+      for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
+      dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
+      dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+      const double a = dd_div_d(a00, yy).hi;
+      L.sc[6] = a;
+      A[0] = a;
+    }
+    for (int i = tid; i < k; i += bs) {
+      const double b = (L.z[i] - L.Hx[i]) / y1;
+      L.b0[i] = b;
+      A[i + 1] = b;
+      A[(size_t)(i + 1) * n] = b;
+    }
+  }
+  __syncthreads();
+  const double a00 = L.sc[6];
+  // Entries of the original matrix (A itself is overwritten below).
   auto Aorig = [&](int r, int c) -> double {
-    if (!normal) return Hsym(r, c);
+    if (!normal) return r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r];
     if (r == 0 && c == 0) return a00;
-    if (r == 0) return (L.z[c - 1] - L.Hx[c - 1]) / y1;
-    if (c == 0) return (L.z[r - 1] - L.Hx[r - 1]) / y1;
-    return Hsym(r - 1, c - 1);
+    if (r == 0) return L.b0[c - 1];
+    if (c == 0) return L.b0[r - 1];
+    return (r - 1) <= (c - 1) ? Hs[(r - 1) * ldh + (c - 1)] : Hs[(c - 1) * ldh + (r - 1)];
   };
   EIG_STAMP(1);
+  EIG_STAMP(2);
   // Separate inlined calls keep the LDS case on ds_* instructions.
-  if (n <= kEigLdsN) {
-    for (int idx = tid; idx < n * n; idx += bs) Alds[idx] = Aorig(idx / n, idx % n);
-    __syncthreads();
-    EIG_STAMP(2);
+  if (n <= kEigLdsN)
     lowest_eigpair(L, Alds, n, prof);
-  } else {
-    for (int idx = tid; idx < n * n; idx += bs) Aglob[idx] = Aorig(idx / n, idx % n);
-    __syncthreads();
-    EIG_STAMP(2);
+  else
     lowest_eigpair(L, Aglob, n, prof);
-  }
   EIG_STAMP(7);
   const double lam = L.sc[0];
   if (normal) {
@@ -440,11 +491,9 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       L.dl[i] = L.q[i].lo;
     }
     __syncthreads();
-    for (int r = tid; r < n; r += bs) {
-      double acc = 0.0;
-      for (int c = 0; c < n; ++c) acc = acc + Aorig(r, c) * L.ev[c];
-      L.Ae[r] = acc;
-    }
+    dot4_partials(L, n, n, [&](int r, int c) { return Aorig(r, c) * L.ev[c]; });
+    __syncthreads();
+    for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L, r);
     __syncthreads();
     const double res2 = red64(n, [&](int i) {
       double t = L.Ae[i] + (-lam) * L.ev[i];
@@ -542,10 +591,8 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
-  // One wavefront covers every Householder column up to k = 64 and needs no
-  // cross-wave barriers; larger problems use four.
-  const int threads = k <= 64 ? 64 : kWG;
-  hipLaunchKernelGGL(k_eig, dim3(1), dim3(threads), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
+  // The four-way partial sums assume exactly four waves.
+  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
 }
 
 }  // namespace mc

@@ -20,6 +20,7 @@
 
 #include "device.h"
 #include "kernels.h"
+#include "sort.h"
 
 namespace mc {
 
@@ -484,36 +485,6 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
   if (fail) st->error = ERR_PROBE;
 }
 
-// ------------------------------------------------------------------ bitonic sort in LDS
-// Sorts n (power of two) entries descending by (a, b); payload follows.
-__device__ void bitonic_desc(uint64_t* ka, uint64_t* kb, uint32_t* pl, int n) {
-  for (int size = 2; size <= n; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        int j = i ^ stride;
-        if (j > i) {
-          bool desc = (i & size) == 0;
-          bool less = (ka[i] < ka[j]) || (ka[i] == ka[j] && kb[i] < kb[j]);
-          bool greater = (ka[i] > ka[j]) || (ka[i] == ka[j] && kb[i] > kb[j]);
-          if (desc ? less : greater) {
-            uint64_t t0 = ka[i];
-            ka[i] = ka[j];
-            ka[j] = t0;
-            uint64_t t1 = kb[i];
-            kb[i] = kb[j];
-            kb[j] = t1;
-            uint32_t t2 = pl[i];
-            pl[i] = pl[j];
-            pl[j] = t2;
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-}
-
 // ------------------------------------------------------------------ k_pass2
 // Values after the scatter: per-element |x*xss+z| (0 when the element does
 // not see its determinant), exact partial sums of H*x and dz*x, and the
@@ -538,6 +509,10 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
   const double diag = Hs[t * ldh + t];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   i128 nz = 0, xt = 0;
+  uint64_t A[kEPT], B[kEPT];
+  uint32_t P[kEPT];
+  int nvalid = 0;
+#pragma unroll
   for (int r = 0; r < kEPT; ++r) {
     const int li = r * kWG + threadIdx.x;
     const int e = chunk * kChunk + li;
@@ -573,11 +548,12 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
         xt += to_fix(dz * x);
         keyv = bitsd(val);
         keyd = D;
+        ++nvalid;
       }
     }
-    ka[li] = keyv;
-    kb[li] = keyd;
-    pl[li] = slot;
+    A[r] = keyv;
+    B[r] = keyd;
+    P[r] = slot;
   }
   // Exact chunk sums.
   nz = wave_sum_i128(nz);
@@ -586,7 +562,12 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
     red[wave][0] = nz;
     red[wave][1] = xt;
   }
-  bitonic_desc(ka, kb, pl, kChunk);
+  // Valid entries have a nonzero determinant and sort before the (0, 0)
+  // fillers.
+  block_sort_desc<kEPT>(A, B, P, SortScratch{ka, kb, pl});
+  __shared__ int wtot[kWG / 64 + 1];
+  int total_valid;
+  (void)block_excl_scan(nvalid, wtot, total_valid);
   if (threadIdx.x == 0) {
     i128 a = 0, b = 0;
     for (int w = 0; w < kWG / 64; ++w) {
@@ -599,23 +580,20 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
     o[2] = (int64_t)(b >> 64);
     o[3] = (int64_t)(uint64_t)b;
   }
-  // Valid entries have keyd != 0 and sort before invalid ones (key (0,0)).
   const int mw = M < kChunk ? M : kChunk;
   StageEnt* out = stage + ((size_t)t * nchunks + chunk) * mw;
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < mw; i += blockDim.x) {
-    if (kb[i] != 0) {
-      out[i].val = ka[i];
-      out[i].det = kb[i];
-      out[i].slot = pl[i];
+  const int keep = total_valid < mw ? total_valid : mw;
+#pragma unroll
+  for (int r = 0; r < kEPT; ++r) {
+    const int i = threadIdx.x * kEPT + r;
+    if (i < keep) {
+      out[i].val = A[r];
+      out[i].det = B[r];
+      out[i].slot = P[r];
       out[i].pad = 0;
-      atomicAdd(&cnt, 1);
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) stage_cnt[t * nchunks + chunk] = cnt;
+  if (threadIdx.x == 0) stage_cnt[t * nchunks + chunk] = keep;
 }
 
 // ------------------------------------------------------------------ k_merge
@@ -623,14 +601,15 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
 // the determinants present in the table (the refetch of
 // src/wavefunction.h:567-575), reinsert z_t := new_z_t, and let the last
 // workgroup update x.z and the counters.
-__global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pdet, const uint32_t* pslot,
+template <int E, int T>
+__global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet, const uint32_t* pslot,
                                                Slot* tab, int M, int nchunks, const StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
                                                const double* dx, int count_iteration) {
-  __shared__ uint64_t ka[kSortCap];
-  __shared__ uint64_t kb[kSortCap];
-  __shared__ uint32_t pl[kSortCap];
+  __shared__ uint64_t ka[T * E];
+  __shared__ uint64_t kb[T * E];
+  __shared__ uint32_t pl[T * E];
   __shared__ int offs[1025];
   __shared__ int total;
   __shared__ bool last;
@@ -638,9 +617,9 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   const int t = blockIdx.x;
   const int k = gridDim.x;
   const int mw = M < kChunk ? M : kChunk;
-  __shared__ int wtot[kWG / 64 + 1];
+  __shared__ int wtot[T / 64 + 1];
   {
-    // nchunks <= 1024: up to four chunk counts per thread, scanned in order.
+    // nchunks <= 1024 <= 4 * T: up to four chunk counts per thread, in order.
     int v[4], sum = 0;
     for (int j = 0; j < 4; ++j) {
       const int c = threadIdx.x * 4 + j;
@@ -657,54 +636,57 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
     if (threadIdx.x == 0) total = tot;
   }
   __syncthreads();
-  int n = total;
-  if (n > kSortCap) {
+  const int n = total;
+  if (n > T * E) {
     if (threadIdx.x == 0) st->error = ERR_CAP;
     return;
   }
-  int np2 = 1;
-  while (np2 < n) np2 <<= 1;
-  for (int g = threadIdx.x; g < n; g += blockDim.x) {
-    int lo = 0, hi = nchunks;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (offs[mid] <= g) lo = mid;
-      else hi = mid;
+  uint64_t A[E], B[E];
+  uint32_t P[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int g = threadIdx.x * E + r;
+    A[r] = 0;
+    B[r] = 0;
+    P[r] = 0xffffffffu;
+    if (g < n) {
+      int lo = 0, hi = nchunks;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (offs[mid] <= g) lo = mid;
+        else hi = mid;
+      }
+      const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
+      A[r] = e.val;
+      B[r] = e.det;
+      P[r] = e.slot;
     }
-    const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
-    ka[g] = e.val;
-    kb[g] = e.det;
-    pl[g] = e.slot;
   }
-  for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
-    ka[i] = 0;
-    kb[i] = 0;
-    pl[i] = 0xffffffffu;
-  }
-  bitonic_desc(ka, kb, pl, np2);
+  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
   // First M entries; keep those present in the table (order preserved).
   const int m = n < M ? n : M;
   {
-    const int per = (m + kWG - 1) / kWG;
-    const int lo = threadIdx.x * per, hi = min(m, lo + per);
     int c = 0;
-    for (int i = lo; i < hi; ++i) c += pl[i] != 0xffffffffu;
+#pragma unroll
+    for (int r = 0; r < E; ++r) c += (threadIdx.x * E + r < m && P[r] != 0xffffffffu) ? 1 : 0;
     int kept;
     int pos = block_excl_scan(c, wtot, kept);
     if (threadIdx.x == 0) cand_cnt[t] = kept;
-    for (int i = lo; i < hi; ++i) {
-      if (pl[i] == 0xffffffffu) continue;
-      cand[(size_t)t * M + pos].det = kb[i];
-      cand[(size_t)t * M + pos].slot = pl[i];
-      cand[(size_t)t * M + pos].pad = 0;
-      ++pos;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      if (threadIdx.x * E + r < m && P[r] != 0xffffffffu) {
+        cand[(size_t)t * M + pos].det = B[r];
+        cand[(size_t)t * M + pos].slot = P[r];
+        cand[(size_t)t * M + pos].pad = 0;
+        ++pos;
+      }
     }
   }
   // Exact per-column sums (order-free) and the reinsert of z_t := new_z_t.
-  __shared__ i128 sums[kWG / 64][2];
+  __shared__ i128 sums[T / 64][2];
   {
     i128 nz = 0, xt = 0;
-    for (int ch = threadIdx.x; ch < nchunks; ch += kWG) {
+    for (int ch = threadIdx.x; ch < nchunks; ch += T) {
       const int64_t* o = part + ((size_t)t * nchunks + ch) * 4;
       nz += (((i128)o[0]) << 64) | (i128)(uint64_t)o[1];
       xt += (((i128)o[2]) << 64) | (i128)(uint64_t)o[3];
@@ -719,7 +701,7 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
   __syncthreads();
   if (threadIdx.x == 0) {
     i128 nz = 0, xt = 0;
-    for (int w = 0; w < kWG / 64; ++w) {
+    for (int w = 0; w < T / 64; ++w) {
       nz += sums[w][0];
       xt += sums[w][1];
     }
@@ -755,72 +737,92 @@ __global__ __launch_bounds__(kWG) void k_merge(DevState* st, const uint64_t* pde
 // ------------------------------------------------------------------ k_pick
 // gcd_grad pick (src/solver.h:118-224): refetch every candidate's (x, z),
 // sort by (|x*s+z| desc, determinant desc), keep the first k distinct.
+template <int E>
 __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const CandEnt* cand,
                                                const int* cand_cnt, const Slot* tab, uint64_t* pdet,
                                                uint32_t* pslot, double* px, double* pz) {
-  __shared__ uint64_t ka[kSortCap];
-  __shared__ uint64_t kb[kSortCap];
-  __shared__ uint32_t pl[kSortCap];
+  __shared__ uint64_t ka[1024 * E];
+  __shared__ uint64_t kb[1024 * E];
+  __shared__ uint32_t pl[1024 * E];
   __shared__ int offs[kMaxK + 2];
-  __shared__ int flag[kSortCap];
   if (st->error) return;
   __shared__ int wtot[17];
   const int kp = st->k_prev;
   {
-    // kp <= 256 < blockDim: one column count per thread.
-    const int v = threadIdx.x < kp ? cand_cnt[threadIdx.x] : 0;
+    // kp <= 256: one column count per thread (the block has at least 64
+    // threads; a 64- or 128-thread block scans four counts per thread).
+    int v[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = threadIdx.x * 4 + j;
+      v[j] = c < kp ? cand_cnt[c] : 0;
+      sum += v[j];
+    }
     int tot;
-    const int pre = block_excl_scan(v, wtot, tot);
-    if (threadIdx.x <= kp) offs[threadIdx.x] = pre;
+    int pre = block_excl_scan(sum, wtot, tot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = threadIdx.x * 4 + j;
+      if (c <= kp) offs[c] = pre;
+      pre += v[j];
+    }
     if (threadIdx.x == 0) offs[kp] = tot;
   }
   __syncthreads();
   const int n = offs[kp];
-  if (n > kSortCap) {
+  const int cap = blockDim.x * E;
+  if (n > cap) {
     if (threadIdx.x == 0) st->error = ERR_CAP;
     return;
   }
   const dd s = load_xss(st);
-  int np2 = 1;
-  while (np2 < n) np2 <<= 1;
-  // Flattened refetch: candidate g belongs to the column c with
-  // offs[c] <= g < offs[c+1] (binary search), so all loads are in flight at once.
-  for (int g = threadIdx.x; g < n; g += blockDim.x) {
-    int lo = 0, hi = kp;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (offs[mid] <= g) lo = mid;
-      else hi = mid;
+  // Refetch: candidate g belongs to the column c with offs[c] <= g < offs[c+1].
+  uint64_t A[E], B[E];
+  uint32_t P[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int g = threadIdx.x * E + r;
+    A[r] = 0;
+    B[r] = 0;
+    P[r] = 0xffffffffu;
+    if (g < n) {
+      int lo = 0, hi = kp;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (offs[mid] <= g) lo = mid;
+        else hi = mid;
+      }
+      const CandEnt ce = cand[(size_t)lo * M + (g - offs[lo])];
+      const Slot& sl = tab[ce.slot];
+      double z = fix_to_d(fix_join(sl.za, sl.zb));
+      A[r] = bitsd(grad_key(s, sl.x, z));
+      B[r] = ce.det;
+      P[r] = ce.slot;
     }
-    const CandEnt ce = cand[(size_t)lo * M + (g - offs[lo])];
-    const Slot& sl = tab[ce.slot];
-    double z = fix_to_d(fix_join(sl.za, sl.zb));
-    ka[g] = bitsd(grad_key(s, sl.x, z));
-    kb[g] = ce.det;
-    pl[g] = ce.slot;
   }
-  for (int i = n + threadIdx.x; i < np2; i += blockDim.x) {
-    ka[i] = 0;
-    kb[i] = 0;
-    pl[i] = 0xffffffffu;
-  }
-  bitonic_desc(ka, kb, pl, np2);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) flag[i] = (i == 0 || kb[i] != kb[i - 1]) ? 1 : 0;
+  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
   __syncthreads();
-  // Exclusive prefix count of distinct entries over contiguous blocks.
-  const int per = (n + blockDim.x - 1) / blockDim.x;
-  const int lo = threadIdx.x * per, hi = min(n, lo + per);
+#pragma unroll
+  for (int r = 0; r < E; ++r) kb[threadIdx.x * E + r] = B[r];
+  __syncthreads();
+  // Adjacent duplicates (equal determinants sort together) count once.
   int cnt = 0;
-  for (int i = lo; i < hi; ++i) cnt += flag[i];
+  bool first[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    first[r] = i < n && (i == 0 || kb[i - 1] != B[r]);
+    cnt += first[r] ? 1 : 0;
+  }
   int distinct;
-  const int pos0 = block_excl_scan(cnt, wtot, distinct);
+  int pos = block_excl_scan(cnt, wtot, distinct);
   if (threadIdx.x == 0 && distinct < k) st->error = ERR_CANDIDATES;
-  int pos = pos0;
-  for (int i = lo; i < hi && pos < k; ++i) {
-    if (!flag[i]) continue;
-    pdet[pos] = kb[i];
-    pslot[pos] = pl[i];
-    const Slot& sl = tab[pl[i]];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    if (!first[r] || pos >= k) continue;
+    pdet[pos] = B[r];
+    pslot[pos] = P[r];
+    const Slot& sl = tab[P[r]];
     px[pos] = sl.x;
     pz[pos] = fix_to_d(fix_join(sl.za, sl.zb));
     ++pos;
@@ -952,13 +954,47 @@ void launch_merge(hipStream_t s, DevState* st, const uint64_t* pdet, const uint3
                   int M, int k, int nchunks, const StageEnt* stage, const int* stage_cnt,
                   const int64_t* part, int64_t* col_xz, CandEnt* cand, int* cand_cnt, const double* dx,
                   int count_iteration) {
-  hipLaunchKernelGGL(k_merge, dim3(k), dim3(kWG), 0, s, st, pdet, pslot, tab, M, nchunks, stage, stage_cnt,
-                     part, col_xz, cand, cand_cnt, dx, count_iteration);
+  // Sort width: the smallest power of two of at least nchunks * min(M, chunk)
+  // entries (at least 256); 256 threads up to 1024 entries, then four
+  // entries per thread.
+  const int mw = M < kChunk ? M : kChunk;
+  int w = kWG;
+  while (w < nchunks * mw) w <<= 1;
+#define MC_MERGE(E_, T_)                                                                                    \
+  hipLaunchKernelGGL((k_merge<E_, T_>), dim3(k), dim3(T_), 0, s, st, pdet, pslot, tab, M, nchunks, stage,   \
+                     stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration)
+  switch (w) {
+    case 256: MC_MERGE(1, 256); break;
+    case 512: MC_MERGE(2, 256); break;
+    case 1024: MC_MERGE(4, 256); break;
+    case 2048: MC_MERGE(4, 512); break;
+    default: MC_MERGE(4, 1024); break;
+  }
+#undef MC_MERGE
 }
 
 void launch_pick(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
                  const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz) {
-  hipLaunchKernelGGL(k_pick, dim3(1), dim3(1024), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot, px, pz);
+  // Sort width: the smallest power of two of at least k * M candidates (and
+  // at least one wave); four entries per thread from 256 entries up.
+  int w = 64;
+  while (w < k * M) w <<= 1;
+  const int E = w >= 256 ? 4 : (w >= 128 ? 2 : 1);
+  const int threads = w / E;
+  switch (E) {
+    case 1:
+      hipLaunchKernelGGL(k_pick<1>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
+                         px, pz);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_pick<2>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
+                         px, pz);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pick<4>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
+                         px, pz);
+      break;
+  }
 }
 
 void launch_init_ref(hipStream_t s, DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,

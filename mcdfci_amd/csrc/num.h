@@ -98,6 +98,18 @@ MC_HD dd dd_div_dbl(dd a, double b) {
   return fast_two_sum(q1, q2);
 }
 
+// double-double divided by a double b, given rb = 1/b rounded: the two
+// quotient digits of dd_div_dbl, each formed by a multiplication with rb and
+// corrected through the exact remainder.
+MC_HD dd dd_div_rcp(dd a, double b, double rb) {
+  double q1 = a.hi * rb;
+  dd p = two_prod(q1, b);
+  dd s = two_sum(a.hi, -p.hi);
+  double sl = (s.lo - p.lo) + a.lo;
+  double q2 = (s.hi + sl) * rb;
+  return fast_two_sum(q1, q2);
+}
+
 MC_HD dd dd_sqrt(dd a) {
   if (!(a.hi > 0.0)) return dd{__builtin_sqrt(a.hi), 0.0};
   double r = 1.0 / __builtin_sqrt(a.hi);

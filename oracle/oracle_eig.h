@@ -7,10 +7,18 @@
 // (Makefile:25), so this restatement fixes one deterministic algorithm that
 // the device solver reproduces operation for operation:
 //   1. Householder tridiagonalisation (LAPACK dsytd2 conventions),
-//   2. Sturm-count multisection (256 points per round) for the lowest
-//      eigenvalue of the tridiagonal matrix,
-//   3. three steps of inverse iteration on the tridiagonal matrix in double,
-//      sharing one LU factorisation with partial pivoting (dgttrf style),
+// with every matrix-vector row sum (here and in the subspace update) taken
+// as four interleaved partial sums, c = j mod 4, combined as
+// (S0 + S1) + (S2 + S3) (dot4),
+//   2. Laguerre's iteration on det(T - lambda I) from the Gershgorin lower
+//      bound for the lowest eigenvalue of the tridiagonal matrix T (for a
+//      polynomial with real roots it rises monotonically to the smallest
+//      root, cubically near it); each pass is one leading-minor recurrence
+//      with its first and second derivatives (no division on the chain),
+//      and lambda stays a point where every leading minor is positive,
+//   3. three steps of inverse iteration on T in double, sharing one LU
+//      factorisation with partial pivoting (dgttrf style); back substitution
+//      multiplies by the rounded reciprocals of the pivots,
 //   4. the reference's extended-precision refinement step: one more solve
 //      with shift lambda - 1e-14 (solver.h:439) using double LU factors and
 //      a double-double vector (as the reference uses double LU factors with
@@ -30,6 +38,19 @@ struct EigPair {
   double lambda = 0.0;  // bisection eigenvalue of the tridiagonal form
   std::vector<dd> q;    // refined, normalised eigenvector (q[0] >= 0)
 };
+
+// Row sum in the canonical four-way order: S_j = sum over c = j mod 4 in
+// ascending c, combined as (S0 + S1) + (S2 + S3).
+template <class F>
+static inline double dot4(int m, F term) {
+  double S[4];
+  for (int j = 0; j < 4; ++j) {
+    double acc = 0.0;
+    for (int c = j; c < m; c += 4) acc = acc + term(c);
+    S[j] = acc;
+  }
+  return (S[0] + S[1]) + (S[2] + S[3]);
+}
 
 static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
 static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
@@ -83,7 +104,13 @@ static TriLU tri_factor(int n, const double* d, const double* e, double shift) {
   return F;
 }
 
-static void tri_solve_d(int n, const TriLU& F, double* b) {
+static std::vector<double> tri_rcp(int n, const TriLU& F) {
+  std::vector<double> r(n);
+  for (int i = 0; i < n; ++i) r[i] = 1.0 / F.dg[i];
+  return r;
+}
+
+static void tri_solve_d(int n, const TriLU& F, const double* rdg, double* b) {
   for (int i = 0; i + 1 < n; ++i) {
     if (!F.swp[i]) {
       b[i + 1] = b[i + 1] - F.mul[i] * b[i];
@@ -93,18 +120,18 @@ static void tri_solve_d(int n, const TriLU& F, double* b) {
       b[i + 1] = bi - F.mul[i] * bi1;
     }
   }
-  b[n - 1] = b[n - 1] / F.dg[n - 1];
-  if (n >= 2) b[n - 2] = (b[n - 2] - F.du[n - 2] * b[n - 1]) / F.dg[n - 2];
+  b[n - 1] = b[n - 1] * rdg[n - 1];
+  if (n >= 2) b[n - 2] = (b[n - 2] - F.du[n - 2] * b[n - 1]) * rdg[n - 2];
   for (int i = n - 3; i >= 0; --i) {
     double t = b[i] - F.du[i] * b[i + 1];
     t = t - F.du2[i] * b[i + 2];
-    b[i] = t / F.dg[i];
+    b[i] = t * rdg[i];
   }
 }
 
 // Substitution with double factors and a double-double right-hand side
 // (the reference's double LU with binary128 vectors, src/solver.h:459-478).
-static void tri_solve_dd(int n, const TriLU& F, dd* b) {
+static void tri_solve_dd(int n, const TriLU& F, const double* rdg, dd* b) {
   for (int i = 0; i + 1 < n; ++i) {
     if (!F.swp[i]) {
       b[i + 1] = dd_sub(b[i + 1], dd_mul_d(b[i], F.mul[i]));
@@ -114,37 +141,58 @@ static void tri_solve_dd(int n, const TriLU& F, dd* b) {
       b[i + 1] = dd_sub(bi, dd_mul_d(bi1, F.mul[i]));
     }
   }
-  b[n - 1] = dd_div_dbl(b[n - 1], F.dg[n - 1]);
-  if (n >= 2) b[n - 2] = dd_div_dbl(dd_sub(b[n - 2], dd_mul_d(b[n - 1], F.du[n - 2])), F.dg[n - 2]);
+  b[n - 1] = dd_div_rcp(b[n - 1], F.dg[n - 1], rdg[n - 1]);
+  if (n >= 2)
+    b[n - 2] = dd_div_rcp(dd_sub(b[n - 2], dd_mul_d(b[n - 1], F.du[n - 2])), F.dg[n - 2], rdg[n - 2]);
   for (int i = n - 3; i >= 0; --i) {
     dd t = dd_sub(b[i], dd_mul_d(b[i + 1], F.du[i]));
     t = dd_sub(t, dd_mul_d(b[i + 2], F.du2[i]));
-    b[i] = dd_div_dbl(t, F.dg[i]);
+    b[i] = dd_div_rcp(t, F.dg[i], rdg[i]);
   }
 }
 
-static int sturm_count(int n, const double* d, const double* e2, double pivmin, double x) {
-  double q = d[0] - x;
-  if (std::fabs(q) < pivmin) q = -pivmin;
-  int c = q < 0.0;
+// One pass of the leading-minor recurrence of T - lam*I,
+//   p_i = (d_i - lam) p_{i-1} - e_{i-1}^2 p_{i-2},
+// with its first and second lam-derivatives, rescaled by exact powers of two
+// (2^-300 / 2^300) whenever p_i leaves [2^-300, 2^300].  Returns false as
+// soon as some p_i is not positive (T - lam*I is not positive definite in
+// floating point); otherwise G = f'/f and Hh = (f'/f)^2 - f''/f for
+// f = det(T - lam*I).
+static bool minor_pass(int n, const double* d, const double* e2, double lam, double& G, double& Hh) {
+  const double big = 0x1p300, small = 0x1p-300;
+  double pm = 1.0, dpm = 0.0, ddpm = 0.0;
+  double p = d[0] - lam, dp = -1.0, ddp = 0.0;
+  if (!(p > 0.0)) return false;
   for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
-    if (std::fabs(q) < pivmin) q = -pivmin;
-    c += q < 0.0;
+    const double a = d[i] - lam, b = e2[i - 1];
+    double pn = a * p - b * pm;
+    double dpn = (a * dp - b * dpm) - p;
+    double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
+    if (!(pn > 0.0)) return false;
+    if (pn > big) {
+      pn *= small, dpn *= small, ddpn *= small, p *= small, dp *= small, ddp *= small;
+    } else if (pn < small) {
+      pn *= big, dpn *= big, ddpn *= big, p *= big, dp *= big, ddp *= big;
+    }
+    pm = p, dpm = dp, ddpm = ddp;
+    p = pn, dp = dpn, ddp = ddpn;
   }
-  return c;
+  G = dp / p;
+  Hh = G * G - ddp / p;
+  return true;
 }
 
-// Lowest eigenvalue of tridiag(e, d, e) by 256-point multisection.
+// Lowest eigenvalue of tridiag(e, d, e) by Laguerre's iteration from the
+// Gershgorin lower bound.  lam always passes minor_pass (a lower bound).  In
+// exact arithmetic the iteration never passes the smallest root, so a
+// candidate that fails is within rounding of it: the point
+// 1e-13 * (|cand| + 1) below it is tried once, and if it passes it is the
+// answer; otherwise the failed points bound the bisection that follows.  At
+// most 100 iterations.
 static double tri_lowest(int n, const double* d, const double* e) {
   if (n == 1) return d[0];
   std::vector<double> e2(n - 1);
-  double emax = 1.0;
-  for (int i = 0; i + 1 < n; ++i) {
-    e2[i] = e[i] * e[i];
-    emax = std::max(emax, e2[i]);
-  }
-  double pivmin = 2.2250738585072014e-308 * emax;
+  for (int i = 0; i + 1 < n; ++i) e2[i] = e[i] * e[i];
   double lo = d[0], hi = d[0];
   for (int i = 0; i < n; ++i) {
     double r = 0.0;
@@ -153,26 +201,33 @@ static double tri_lowest(int n, const double* d, const double* e) {
     lo = std::min(lo, d[i] - r);
     hi = std::max(hi, d[i] + r);
   }
-  double pad = (hi - lo) * 1e-6 + 1e-200;
-  lo = lo - pad;
+  const double pad = (hi - lo) * 1e-6 + 1e-200;
+  double lam = lo - pad;
   hi = hi + pad;
-  for (int round = 0; round < 16; ++round) {
-    double w = (hi - lo) / 257.0;
-    int j = 256;
-    double xs[256];
-    for (int t = 0; t < 256; ++t) xs[t] = lo + (double)(t + 1) * w;
-    for (int t = 0; t < 256; ++t)
-      if (sturm_count(n, d, e2.data(), pivmin, xs[t]) >= 1) {
-        j = t;
-        break;
-      }
-    double nlo = (j == 0) ? lo : xs[j - 1];
-    double nhi = (j == 256) ? hi : xs[j];
-    if (nlo == lo && nhi == hi) break;
-    lo = nlo;
-    hi = nhi;
+  const double nn = (double)n;
+  double G, Hh;
+  if (!minor_pass(n, d, e2.data(), lam, G, Hh)) return lam;
+  for (int it = 0; it < 100; ++it) {
+    double disc = (nn - 1.0) * (nn * Hh - G * G);
+    if (!(disc > 0.0)) disc = 0.0;
+    double cand = lam - nn / (G - std::sqrt(disc));
+    if (!(cand < hi)) cand = lam + 0.5 * (hi - lam);
+    if (!(cand > lam) || !(cand < hi)) break;
+    double G2, H2;
+    if (minor_pass(n, d, e2.data(), cand, G2, H2)) {
+      lam = cand, G = G2, Hh = H2;
+      continue;
+    }
+    hi = cand;
+    const double t = cand - 1e-13 * (std::fabs(cand) + 1.0);
+    if (!(t > lam)) break;
+    if (minor_pass(n, d, e2.data(), t, G2, H2)) {
+      lam = t;
+      break;
+    }
+    hi = t;
   }
-  return lo + 0.5 * (hi - lo);
+  return lam;
 }
 
 // A: n x n symmetric, row-major, both triangles filled with identical bits.
@@ -209,10 +264,8 @@ static EigPair lowest_eigpair(int n, std::vector<double> A) {
     e[s] = beta;
     vs[s].assign(v.begin(), v.begin() + m);
     for (int r = 0; r < m; ++r) {
-      double acc = 0.0;
       const double* row = &A[(s + 1 + r) * n + (s + 1)];
-      for (int c = 0; c < m; ++c) acc = acc + row[c] * v[c];
-      p[r] = t * acc;
+      p[r] = t * dot4(m, [&](int c) { return row[c] * v[c]; });
     }
     double K = (-0.5 * t) * red64(m, [&](int i) { return p[i] * v[i]; });
     for (int r = 0; r < m; ++r) w[r] = p[r] + K * v[r];
@@ -232,8 +285,9 @@ static EigPair lowest_eigpair(int n, std::vector<double> A) {
   std::vector<double> y(n, 1.0);
   {
     TriLU F1 = tri_factor(n, d.data(), e.data(), lam);
+    std::vector<double> r1 = tri_rcp(n, F1);
     for (int it = 0; it < 3; ++it) {
-      tri_solve_d(n, F1, y.data());
+      tri_solve_d(n, F1, r1.data(), y.data());
       double mx = 0.0;
       for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(y[i]));
       for (int i = 0; i < n; ++i) y[i] = y[i] / mx;
@@ -245,7 +299,8 @@ static EigPair lowest_eigpair(int n, std::vector<double> A) {
   for (int i = 0; i < n; ++i) q[i] = dd_make(y[i]);
   {
     TriLU F2 = tri_factor(n, d.data(), e.data(), lam - 1e-14);
-    tri_solve_dd(n, F2, q.data());
+    std::vector<double> r2 = tri_rcp(n, F2);
+    tri_solve_dd(n, F2, r2.data(), q.data());
   }
   for (int s = n - 3; s >= 0; --s) {
     if (tau[s] == 0.0) continue;
@@ -290,11 +345,7 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
   for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
   for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
-  for (int r = 0; r < k; ++r) {
-    double acc = 0.0;
-    for (int c = 0; c < k; ++c) acc = acc + H(r, c) * x[c];
-    Hx[r] = acc;
-  }
+  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c) * x[c]; });
   for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
   double yy = dd_add_d(xx, -xTx).hi;
   double y1 = std::sqrt(yy);
@@ -321,11 +372,7 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
       ev[i] = ep.q[i].hi;
       dl[i] = ep.q[i].lo;
     }
-    for (int r = 0; r < n; ++r) {
-      double acc = 0.0;
-      for (int c = 0; c < n; ++c) acc = acc + A[r * n + c] * ev[c];
-      Ae[r] = acc;
-    }
+    for (int r = 0; r < n; ++r) Ae[r] = dot4(n, [&](int c) { return A[r * n + c] * ev[c]; });
     double res2 = red64(n, [&](int i) {
       double t = Ae[i] + (-lam) * ev[i];
       return t * t;

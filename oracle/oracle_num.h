@@ -99,6 +99,19 @@ static inline dd dd_div_dbl(dd a, double b) {
   return quick_two_sum(q1, q2);
 }
 
+// double-double divided by a double b, given rb = 1/b rounded: the same
+// two quotient digits as dd_div_dbl, each formed by a multiplication with
+// rb and corrected through the exact remainder.
+static inline dd dd_div_rcp(dd a, double b, double rb) {
+  double q1 = a.hi * rb;
+  dd p = two_prod(q1, b);
+  dd s = two_sum(a.hi, -p.hi);
+  double sl = s.lo - p.lo;
+  sl = sl + a.lo;
+  double q2 = (s.hi + sl) * rb;
+  return quick_two_sum(q1, q2);
+}
+
 static inline dd dd_sqrt(dd a) {
   if (!(a.hi > 0.0)) return dd{std::sqrt(a.hi), 0.0};  // 0 stays 0, negative gives NaN
   double x = 1.0 / std::sqrt(a.hi);
