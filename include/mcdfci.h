@@ -148,6 +148,22 @@ int mcdfci_table_dump(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int
 /* The candidate list passed to the next pick (sub_xz, src/wavefunction.h:567-589). */
 int mcdfci_candidates(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);
 
+/* Adjoint of the energy.  The reference has no counterpart (it only
+ * computes energies, src/solver.h:672-829); the Hamiltonian terms
+ * differentiated are those of src/hamiltonian.h:257-410.  For the current
+ * wavefunction x (normalised c = x/|x|) returns the gradient of the
+ * variational energy E = c.Hc / c.c with respect to every unique FCIDUMP
+ * parameter, at its canonical position:
+ *   grad_h[p*norb + q], p >= q                 (d E / d h_pq, h_pq = h_qp)
+ *   grad_eri[((p*norb + q)*norb + r)*norb + s]  (d E / d (pq|rs) for the
+ *       8-fold orbit; p >= q, r >= s, pair(pq) >= pair(rs))
+ * with all other entries 0, and energy = core + sum theta * dE/dtheta (E at
+ * fixed c, exact by linearity).  At an eigenvector this is the
+ * Hellmann-Feynman gradient; it equals the spin-summed 1-RDM and 2-RDM
+ * folded onto the unique parameters.  grad_h has norb^2 entries and
+ * grad_eri norb^4; energy may be NULL. */
+int mcdfci_energy_gradient(mcdfci_ctx* ctx, double* grad_h, double* grad_eri, double* energy);
+
 #ifdef __cplusplus
 }
 #endif

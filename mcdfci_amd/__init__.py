@@ -95,6 +95,7 @@ SIGNATURES = {
     "mcdfci_table_dump": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]),
     "mcdfci_candidates": (C.c_int, [C.c_void_p, _u64p, _f64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
+    "mcdfci_energy_gradient": (C.c_int, [C.c_void_p, _f64p, _f64p, C.POINTER(C.c_double)]),
 }
 
 
@@ -139,6 +140,7 @@ class Cdfci:
         self._check(self.lib.mcdfci_info(self.h, info))
         self.nwords, self.nso, self.nelec, self.ms2, self.n_enum, self.k, self.has_orbital_energy = (
             int(v) for v in info[:7])
+        self.norb = self.nso // 2
         self.estimate = int(estimate)
 
     def _check(self, rc):
@@ -249,3 +251,15 @@ class Cdfci:
         self._check(self.lib.mcdfci_candidates(self.h, dets, x, z, cap, C.byref(n)))
         m = min(n.value, cap)
         return dets[: m * self.nwords].reshape(m, self.nwords), x[:m], z[:m]
+
+    def energy_gradient(self):
+        """(E, dE/dh, dE/d(pq|rs)) of the current wavefunction: the gradient of
+        the variational energy with respect to the unique integrals at their
+        canonical positions (h: p >= q; eri: p >= q, r >= s, pair(pq) >=
+        pair(rs)); other entries are 0."""
+        n = self.norb
+        gh = np.zeros(n * n)
+        ge = np.zeros(n ** 4)
+        e = C.c_double()
+        self._check(self.lib.mcdfci_energy_gradient(self.h, gh, ge, C.byref(e)))
+        return e.value, gh.reshape(n, n), ge.reshape(n, n, n, n)

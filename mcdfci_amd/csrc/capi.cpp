@@ -15,6 +15,7 @@
 #include "../../include/mcdfci.h"
 #include "device.h"
 #include "fcidump.h"
+#include "grad.h"
 #include "kernels.h"
 
 using namespace mc;
@@ -776,3 +777,83 @@ int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
 }
 
 }  // extern "C"
+
+// Gradient of the variational energy of the current wavefunction with
+// respect to the unique integrals (see grad.hip).
+int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, double* energy) {
+  if (!c || !grad_h || !grad_eri) return fail(c, MCDFCI_E_INPUT, "null argument");
+  if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (s.error) return check_state_error(c, s);
+  const int M = c->ham.norb;
+  const size_t nh = (size_t)M * M, ne4 = nh * nh;
+  long long ub = (long long)std::min<uint64_t>(c->nslots, (uint64_t)(s.x_size + c->k + 1024));
+  uint64_t* od = nullptr;
+  double* ox = nullptr;
+  unsigned long long* cnt = nullptr;
+  int64_t *acc_h = nullptr, *acc_e = nullptr, *acc_q = nullptr;
+  int rc;
+  auto release = [&]() {
+    for (void* p : {(void*)od, (void*)ox, (void*)cnt, (void*)acc_h, (void*)acc_e, (void*)acc_q}) dfree(c, p);
+  };
+  unsigned long long m = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if ((rc = dalloc(c, &od, ub)) || (rc = dalloc(c, &ox, ub)) || (rc = dalloc(c, &cnt, 1))) {
+      release();
+      return rc;
+    }
+    HIPCHECK(c, hipMemset(cnt, 0, 8));
+    launch_compact_x(c->stream, c->tab, c->nslots, od, ox, cnt, ub);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
+    if ((long long)m <= ub) break;
+    for (void* p : {(void*)od, (void*)ox, (void*)cnt}) dfree(c, p);
+    od = nullptr, ox = nullptr, cnt = nullptr;
+    ub = (long long)m;
+  }
+  if ((rc = dalloc(c, &acc_h, 2 * nh)) || (rc = dalloc(c, &acc_e, 2 * ne4)) || (rc = dalloc(c, &acc_q, 2))) {
+    release();
+    return rc;
+  }
+  HIPCHECK(c, hipMemset(acc_h, 0, 16 * nh));
+  HIPCHECK(c, hipMemset(acc_e, 0, 16 * ne4));
+  HIPCHECK(c, hipMemset(acc_q, 0, 16));
+  int64_t q[2];
+  // |x|^2 in fixed point, then the normalised sum c.c (exactly, for the
+  // denominator of the Rayleigh quotient).
+  launch_sum_sq(c->stream, ox, (long long)m, 1.0, acc_q);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpy(q, acc_q, 16, hipMemcpyDeviceToHost));
+  const double xx = fix_to_d(fix_join(q[0], q[1]));
+  if (!(xx > 0.0)) {
+    release();
+    return fail(c, MCDFCI_E_NUMERIC, "the wavefunction is zero");
+  }
+  const double scale = 1.0 / std::sqrt(xx);
+  HIPCHECK(c, hipMemset(acc_q, 0, 16));
+  launch_sum_sq(c->stream, ox, (long long)m, scale, acc_q);
+  launch_energy_grad(c->stream, c->dh, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc_h, acc_e);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  std::vector<int64_t> hh(2 * nh), he(2 * ne4);
+  HIPCHECK(c, hipMemcpy(q, acc_q, 16, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hh.data(), acc_h, 16 * nh, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(he.data(), acc_e, 16 * ne4, hipMemcpyDeviceToHost));
+  release();
+  const dd den = fix_to_dd(fix_join(q[0], q[1]));
+  double e = c->ham.core;
+  for (size_t i = 0; i < nh; ++i) {
+    grad_h[i] = dd_div(fix_to_dd(fix_join(hh[2 * i], hh[2 * i + 1])), den).hi;
+    e += grad_h[i] * c->ham.h[i];
+  }
+  for (size_t i = 0; i < ne4; ++i) {
+    grad_eri[i] = dd_div(fix_to_dd(fix_join(he[2 * i], he[2 * i + 1])), den).hi;
+    e += grad_eri[i] * c->ham.eri[i];
+  }
+  if (energy) *energy = e;
+  return MCDFCI_OK;
+}

@@ -1045,4 +1045,136 @@ void oc_dd_ops(double ah, double al, double bh, double bl, double* out) {
   out[7] = r.lo;
 }
 
+// Gradient of the variational energy of the wavefunction (dets, x) with
+// respect to the unique integrals (canonical positions; see
+// include/mcdfci.h, mcdfci_energy_gradient).  An independent restatement:
+// excitations are enumerated over spin-orbitals (every single and double of
+// each determinant, no threshold), the matrix-element terms are the
+// Slater-Condon expressions of src/hamiltonian.h:257-410 written with
+// spatial chemists' integrals, and all sums are exact fixed-point integers.
+// Returns 0, or 1 when x is zero.
+int oc_energy_gradient(void* p, const uint64_t* dets, const double* x, long n, double* grad_h,
+                       double* grad_eri, double* energy) {
+  Ctx* c = (Ctx*)p;
+  const Fcidump& f = c->f;
+  const int ns = f.norb, M = ns / 2, nw = c->h.nw;
+  const size_t nh = (size_t)M * M, n4 = nh * nh;
+  std::vector<Det> D(n);
+  for (long i = 0; i < n; ++i) det_in(dets + (size_t)i * nw, nw, D[i]);
+  std::unordered_map<uint64_t, std::vector<std::pair<Det, double>>> xm;
+  for (long i = 0; i < n; ++i) xm[det_hash(D[i])].push_back({D[i], x[i]});
+  auto lookup = [&](const Det& d) -> double {
+    auto it = xm.find(det_hash(d));
+    if (it == xm.end()) return 0.0;
+    for (auto& e : it->second)
+      if (e.first == d) return e.second;
+    return 0.0;
+  };
+  i128 s1 = 0;
+  for (long i = 0; i < n; ++i) s1 += to_fixed(x[i] * x[i]);
+  const double xx = fixed_to_double(s1);
+  if (!(xx > 0.0)) return 1;
+  const double scale = 1.0 / std::sqrt(xx);
+  i128 q = 0;
+  for (long i = 0; i < n; ++i) {
+    const double ci = x[i] * scale;
+    q += to_fixed(ci * ci);
+  }
+  std::vector<i128> gh(nh, 0), ge(n4, 0);
+  auto hi = [&](int a, int b) { return a >= b ? (size_t)a * M + b : (size_t)b * M + a; };
+  auto ei = [&](int a, int b, int cc, int d) {
+    int p1 = std::max(a, b), q1 = std::min(a, b), p2 = std::max(cc, d), q2 = std::min(cc, d);
+    if (p1 * (p1 + 1) / 2 + q1 < p2 * (p2 + 1) / 2 + q2) {
+      std::swap(p1, p2);
+      std::swap(q1, q2);
+    }
+    return (((size_t)p1 * M + q1) * M + p2) * M + q2;
+  };
+  for (long di = 0; di < n; ++di) {
+    const Det& d = D[di];
+    const double cD = x[di] * scale;
+    const std::vector<int> occ = occupied(d, ns);
+    std::vector<int> vir;
+    for (int s = 0; s < ns; ++s)
+      if (!d.occ(s)) vir.push_back(s);
+    {
+      const i128 w = to_fixed(cD * cD);
+      for (size_t a = 0; a < occ.size(); ++a) {
+        const int I = occ[a];
+        gh[hi(I >> 1, I >> 1)] += w;
+        for (size_t b = a + 1; b < occ.size(); ++b) {
+          const int J = occ[b];
+          ge[ei(I >> 1, I >> 1, J >> 1, J >> 1)] += w;
+          if ((I & 1) == (J & 1)) ge[ei(I >> 1, J >> 1, J >> 1, I >> 1)] -= w;
+        }
+      }
+    }
+    for (int I : occ)
+      for (int A : vir) {
+        if ((I & 1) != (A & 1)) continue;
+        Det e = d;
+        e.clr(I);
+        const int sg = parity(e, I, A);
+        e.set(A);
+        const double xo = lookup(e);
+        if (!(std::fabs(xo) != 0.0)) continue;
+        i128 w = to_fixed(cD * (xo * scale));
+        if (sg) w = -w;
+        const int pI = I >> 1, pA = A >> 1;
+        gh[hi(pI, pA)] += w;
+        for (int K : occ) {
+          ge[ei(pI, pA, K >> 1, K >> 1)] += w;
+          if ((K & 1) == (I & 1)) ge[ei(pI, K >> 1, K >> 1, pA)] -= w;
+        }
+      }
+    for (size_t a = 0; a < occ.size(); ++a)
+      for (size_t b = a + 1; b < occ.size(); ++b) {
+        const int I = occ[a], J = occ[b];
+        for (size_t u = 0; u < vir.size(); ++u)
+          for (size_t v = u + 1; v < vir.size(); ++v) {
+            int A = vir[u], B = vir[v];
+            const int sI = I & 1, sJ = J & 1;
+            if (sI == sJ) {
+              if ((A & 1) != sI || (B & 1) != sI) continue;
+            } else {
+              if ((A & 1) == (B & 1)) continue;
+              if ((A & 1) != sI) std::swap(A, B);  // A pairs with I by spin
+            }
+            Det e = d;
+            e.clr(I);
+            int sg = parity(e, I, A);
+            e.set(A);
+            e.clr(J);
+            sg ^= parity(e, J, B);
+            e.set(B);
+            const double xo = lookup(e);
+            if (!(std::fabs(xo) != 0.0)) continue;
+            i128 w = to_fixed(cD * (xo * scale));
+            if (sg) w = -w;
+            ge[ei(I >> 1, A >> 1, J >> 1, B >> 1)] += w;
+            if (sI == sJ) ge[ei(I >> 1, B >> 1, J >> 1, A >> 1)] -= w;
+          }
+      }
+  }
+  const dd den = fixed_to_dd(q);
+  double e = c->h.core;
+  for (int a = 0; a < M; ++a)
+    for (int b = 0; b < M; ++b) {
+      const size_t i = (size_t)a * M + b;
+      grad_h[i] = dd_div(fixed_to_dd(gh[i]), den).hi;
+      e += grad_h[i] * f.h1[(size_t)(2 * a) * ns + 2 * b];
+    }
+  for (size_t i = 0; i < n4; ++i) {
+    grad_eri[i] = dd_div(fixed_to_dd(ge[i]), den).hi;
+    if (grad_eri[i] != 0.0) {
+      const int a = (int)(i / (nh * M)), b = (int)(i / nh % M), cc = (int)(i / M % M), d = (int)(i % M);
+      // (ab|cd) = <ac|bd>; the store keeps <ij|..> with i <= j (<ij|ab> = <ji|ba>)
+      const double v = a <= cc ? f.get2(2 * a, 2 * cc, 2 * b, 2 * d) : f.get2(2 * cc, 2 * a, 2 * d, 2 * b);
+      e += grad_eri[i] * v;
+    }
+  }
+  if (energy) *energy = e;
+  return 0;
+}
+
 }  // extern "C"

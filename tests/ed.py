@@ -134,3 +134,75 @@ def ground_state(path):
     dets, H = hamiltonian(h, eri, core, na, nb)
     w, v = np.linalg.eigh(H)
     return dets, H, w[0], v[:, 0]
+
+
+def _apply(dets, vec, ops):
+    """Apply a string of (kind, spin-orbital) operators, rightmost first, to
+    the vector vec over dets; returns (target dets, coefficients) of the
+    surviving terms."""
+    D = dets.copy()
+    valid = np.ones(len(dets), bool)
+    sign = np.ones(len(dets))
+    for kind, s in reversed(ops):
+        if kind == "a":
+            D, valid, sign = _annihilate(D, valid, sign, s)
+        else:
+            D, valid, sign = _create(D, valid, sign, s)
+    return D[valid], (sign * vec)[valid]
+
+
+def _expect(dets, c, ops):
+    Dn, coef = _apply(dets, c, ops)
+    idx = np.searchsorted(dets, Dn)
+    ok = (idx < len(dets))
+    ok[ok] = dets[idx[ok]] == Dn[ok]
+    return float(np.dot(c[idx[ok]], coef[ok]))
+
+
+def rdm12(dets, c, norb):
+    """Spin-summed 1-RDM g[p,q] = sum_s <a+_ps a_qs> and 2-RDM (chemists'
+    order) G[p,q,r,s] = sum_{st} <a+_ps a+_rt a_st a_qs> of the normalised
+    vector c over the sorted determinant list dets, by second quantisation."""
+    c = np.asarray(c, dtype=np.float64) / np.linalg.norm(c)
+    g = np.zeros((norb, norb))
+    G = np.zeros((norb, norb, norb, norb))
+    for p in range(norb):
+        for q in range(norb):
+            for sp in (0, 1):
+                g[p, q] += _expect(dets, c, [("c", 2 * p + sp), ("a", 2 * q + sp)])
+    for p, q, r, s in itertools.product(range(norb), repeat=4):
+        v = 0.0
+        for sp in (0, 1):
+            for st in (0, 1):
+                v += _expect(dets, c, [("c", 2 * p + sp), ("c", 2 * r + st), ("a", 2 * s + st), ("a", 2 * q + sp)])
+        G[p, q, r, s] = v
+    return g, G
+
+
+def orbit(p, q, r, s):
+    """The distinct index tuples of the 8-fold symmetry orbit of (pq|rs)."""
+    return sorted({(p, q, r, s), (q, p, r, s), (p, q, s, r), (q, p, s, r),
+                   (r, s, p, q), (s, r, p, q), (r, s, q, p), (s, r, q, p)})
+
+
+def canonical(p, q, r, s):
+    a, b = max(p, q), min(p, q)
+    c, d = max(r, s), min(r, s)
+    if a * (a + 1) // 2 + b < c * (c + 1) // 2 + d:
+        a, b, c, d = c, d, a, b
+    return a, b, c, d
+
+
+def fold_gradient(g, G):
+    """dE/dtheta for the unique parameters of E = sum h g + 1/2 sum (pq|rs) G
+    + core, at the canonical positions (p >= q; canonical (pq|rs))."""
+    n = g.shape[0]
+    gh = np.zeros((n, n))
+    for p in range(n):
+        for q in range(p + 1):
+            gh[p, q] = g[p, q] + g[q, p] if p != q else g[p, p]
+    ge = np.zeros((n, n, n, n))
+    for p, q, r, s in itertools.product(range(n), repeat=4):
+        if (p, q, r, s) == canonical(p, q, r, s):
+            ge[p, q, r, s] = 0.5 * sum(G[t] for t in orbit(p, q, r, s))
+    return gh, ge

@@ -58,6 +58,8 @@ def oracle_lib():
         _sig(lib, "oc_wf_set_scalars", None, [C.c_void_p, _f64p, C.c_double])
         _sig(lib, "oc_wf_state", None, [C.c_void_p, _f64p])
         _sig(lib, "oc_wf_update", None, [C.c_void_p, C.c_int, _u64p, _f64p])
+        _sig(lib, "oc_energy_gradient", C.c_int,
+             [C.c_void_p, _u64p, _f64p, C.c_long, _f64p, _f64p, C.POINTER(C.c_double)])
         _sig(lib, "oc_pick", C.c_int, [C.c_void_p, C.c_long, _u64p, _f64p, _f64p, _f64p, C.c_int, _u64p])
         _oracle = lib
     return _oracle
@@ -175,6 +177,20 @@ class Oracle:
         n = self.lib.oc_candidates(self.h, dets, x, z, cap)
         return dets[: n * self.nw].reshape(n, self.nw), x[:n], z[:n]
 
+    def energy_gradient(self, dets, x):
+        """(E, dE/dh, dE/d(pq|rs)) of the wavefunction (dets, x) at the
+        canonical positions of the unique integrals."""
+        m = self.nso // 2
+        dets = np.ascontiguousarray(np.asarray(dets, dtype=np.uint64).reshape(-1, self.nw))
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        gh = np.zeros(m * m)
+        ge = np.zeros(m ** 4)
+        e = C.c_double()
+        rc = self.lib.oc_energy_gradient(self.h, dets.reshape(-1), x, len(x), gh, ge, C.byref(e))
+        if rc:
+            raise RuntimeError("zero wavefunction")
+        return e.value, gh.reshape(m, m), ge.reshape(m, m, m, m)
+
 
 class RefHam:
     """The reference's HamiltonianMolecule<N> built from its own sources."""
@@ -222,3 +238,4 @@ class RefHam:
         H = np.zeros(k * k)
         self.lib.ref_submatrix(self.h, k, dets, H)
         return H.reshape(k, k)
+
