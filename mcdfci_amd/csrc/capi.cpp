@@ -270,7 +270,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->col_xz, (size_t)2 * k))) return rc;
   if ((rc = dalloc(c, &c->cand, (size_t)k * M))) return rc;
   if ((rc = dalloc(c, &c->cand_cnt, k))) return rc;
-  HIPCHECK(c, hipMemset(c->cand_cnt, 0, sizeof(int) * k));
+  HIPCHECK(c, hipMemsetAsync(c->cand_cnt, 0, sizeof(int) * k, c->stream));
   return MCDFCI_OK;
 }
 
@@ -599,7 +599,7 @@ int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* va
   if ((rc = dalloc(c, &dv, ncap))) return rc;
   if ((rc = dalloc(c, &cnt, 1))) return rc;
   HIPCHECK(c, hipMemcpy(dd_, det, 8, hipMemcpyHostToDevice));
-  HIPCHECK(c, hipMemset(cnt, 0, 8));
+  HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
   launch_submatrix(c->stream, H, nullptr, dd_, 1, dh_, 1);
   double diag;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -659,7 +659,7 @@ int mcdfci_submatrix(mcdfci_ctx* c, int k, const uint64_t* dets, double* Hout) {
   if ((rc = dalloc(c, &d, k))) return rc;
   if ((rc = dalloc(c, &h, (size_t)k * k))) return rc;
   HIPCHECK(c, hipMemcpy(d, dets, 8 * (size_t)k, hipMemcpyHostToDevice));
-  HIPCHECK(c, hipMemset(h, 0, 8 * (size_t)k * k));
+  HIPCHECK(c, hipMemsetAsync(h, 0, 8 * (size_t)k * k, c->stream));
   launch_submatrix(c->stream, c->dh, nullptr, d, k, h, k);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -722,7 +722,7 @@ int mcdfci_table_dump(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
   if ((rc = dalloc(c, &ox, ub))) return rc;
   if ((rc = dalloc(c, &oz, 2 * ub))) return rc;
   if ((rc = dalloc(c, &cnt, 1))) return rc;
-  HIPCHECK(c, hipMemset(cnt, 0, 8));
+  HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
   launch_table_compact(c->stream, c->tab, c->nslots, od, ox, oz, cnt, ub);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -805,7 +805,7 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
       release();
       return rc;
     }
-    HIPCHECK(c, hipMemset(cnt, 0, 8));
+    HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
     launch_compact_x(c->stream, c->tab, c->nslots, od, ox, cnt, ub);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -819,14 +819,15 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
     release();
     return rc;
   }
-  HIPCHECK(c, hipMemset(acc_h, 0, 16 * nh));
-  HIPCHECK(c, hipMemset(acc_e, 0, 16 * ne4));
-  HIPCHECK(c, hipMemset(acc_q, 0, 16));
+  HIPCHECK(c, hipMemsetAsync(acc_h, 0, 16 * nh, c->stream));
+  HIPCHECK(c, hipMemsetAsync(acc_e, 0, 16 * ne4, c->stream));
+  HIPCHECK(c, hipMemsetAsync(acc_q, 0, 16, c->stream));
   int64_t q[2];
   // |x|^2 in fixed point, then the normalised sum c.c (exactly, for the
   // denominator of the Rayleigh quotient).
   launch_sum_sq(c->stream, ox, (long long)m, 1.0, acc_q);
   HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));  // the copy below is on the null stream
   HIPCHECK(c, hipMemcpy(q, acc_q, 16, hipMemcpyDeviceToHost));
   const double xx = fix_to_d(fix_join(q[0], q[1]));
   if (!(xx > 0.0)) {
@@ -834,7 +835,7 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
     return fail(c, MCDFCI_E_NUMERIC, "the wavefunction is zero");
   }
   const double scale = 1.0 / std::sqrt(xx);
-  HIPCHECK(c, hipMemset(acc_q, 0, 16));
+  HIPCHECK(c, hipMemsetAsync(acc_q, 0, 16, c->stream));
   launch_sum_sq(c->stream, ox, (long long)m, scale, acc_q);
   launch_energy_grad(c->stream, c->dh, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc_h, acc_e);
   HIPCHECK(c, hipGetLastError());
