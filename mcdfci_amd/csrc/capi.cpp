@@ -72,6 +72,10 @@ struct mcdfci_ctx {
   int64_t* col_xz = nullptr;
   CandEnt* cand = nullptr;
   int* cand_cnt = nullptr;
+  PickEnt* pk0 = nullptr;  // pick lists for k * M above one workgroup's sort
+  PickEnt* pk1 = nullptr;
+  int* pc0 = nullptr;
+  int* pc1 = nullptr;
   size_t iter_bytes = 0;
   bool ready = false;
   bool timing = false;
@@ -240,21 +244,16 @@ void table_size(double max_memory, int nwords, uint64_t& slots, double& max_size
 int alloc_iteration(mcdfci_ctx* c) {
   for (void* p : {(void*)c->pdet, (void*)c->pslot, (void*)c->px, (void*)c->pz, (void*)c->dx, (void*)c->Hs,
                   (void*)c->Aglob, (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
-                  (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt})
+                  (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt, (void*)c->pk0, (void*)c->pk1,
+                  (void*)c->pc0, (void*)c->pc1})
     dfree(c, p);
   const int k = c->k, M = c->M;
   const DevHam& H = c->dh;
   c->nchunks = (H.n_enum + kChunk - 1) / kChunk;
   c->vis_words = c->nchunks * (kChunk / 64);
   const int mw = std::min(M, kChunk);
-  if (c->nchunks > 1024) return fail(c, MCDFCI_E_UNSUPPORTED, "column too long for the merge kernel");
-  if ((long long)c->nchunks * mw > kSortCap)
-    return fail(c, MCDFCI_E_UNSUPPORTED,
-                "estimate * column chunks exceeds the per-column merge capacity (" + std::to_string(kSortCap) +
-                    "); reduce estimate");
-  if ((long long)k * M > kSortCap)
-    return fail(c, MCDFCI_E_UNSUPPORTED,
-                "num_of_coordinate * estimate exceeds the pick capacity (" + std::to_string(kSortCap) + ")");
+  if (c->nchunks > 1023) return fail(c, MCDFCI_E_UNSUPPORTED, "column too long for the merge kernel");
+  if (M > kChunk) return fail(c, MCDFCI_E_UNSUPPORTED, "estimate above " + std::to_string(kChunk));
   int rc;
   if ((rc = dalloc(c, &c->pdet, k))) return rc;
   if ((rc = dalloc(c, &c->pslot, k))) return rc;
@@ -270,6 +269,11 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->col_xz, (size_t)2 * k))) return rc;
   if ((rc = dalloc(c, &c->cand, (size_t)k * M))) return rc;
   if ((rc = dalloc(c, &c->cand_cnt, k))) return rc;
+  const int L = std::min(M, k);
+  if ((rc = dalloc(c, &c->pk0, (size_t)k * L))) return rc;
+  if ((rc = dalloc(c, &c->pk1, (size_t)k * L))) return rc;
+  if ((rc = dalloc(c, &c->pc0, k))) return rc;
+  if ((rc = dalloc(c, &c->pc1, k))) return rc;
   HIPCHECK(c, hipMemsetAsync(c->cand_cnt, 0, sizeof(int) * k, c->stream));
   return MCDFCI_OK;
 }
@@ -316,7 +320,8 @@ void launch_iteration(mcdfci_ctx* c) {
   hipStream_t s = c->stream;
   const int k = c->k;
   if (c->timing) (void)hipEventRecord(c->ev[0], s);
-  launch_pick(s, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz);
+  launch_pick(s, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz, c->pk0, c->pk1,
+              c->pc0, c->pc1);
   if (c->timing) (void)hipEventRecord(c->ev[1], s);
   launch_submatrix(s, c->dh, c->st, c->pdet, k, c->Hs, k);
   if (c->timing) (void)hipEventRecord(c->ev[2], s);

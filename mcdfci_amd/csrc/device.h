@@ -99,4 +99,12 @@ struct CandEnt {
   uint32_t pad;
 };
 
+// A refetched candidate of the pick: key = bits of |x*xss + z|.
+struct PickEnt {
+  uint64_t key;
+  uint64_t det;
+  uint32_t slot;
+  uint32_t pad;
+};
+
 }  // namespace mc

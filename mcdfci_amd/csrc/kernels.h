@@ -22,7 +22,8 @@ void launch_merge(hipStream_t s, DevState* st, const uint64_t* pdet, const uint3
                   const int64_t* part, int64_t* col_xz, CandEnt* cand, int* cand_cnt, const double* dx,
                   int count_iteration);
 void launch_pick(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
-                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz);
+                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0,
+                 PickEnt* pk1, int* pc0, int* pc1);
 void launch_init_ref(hipStream_t s, DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
                      uint32_t* pslot, double* px, double* pz, double* dx);
 void launch_column_dump(hipStream_t s, const DevHam& H, uint64_t det, double diag, uint64_t* odet,

@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "device.h"
 #include "kernels.h"
 #include "sort.h"
@@ -540,7 +542,12 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
 // the determinants present in the table (the refetch of
 // src/wavefunction.h:567-575), reinsert z_t := new_z_t, and let the last
 // workgroup update x.z and the counters.
-template <int E, int T>
+// STREAM = false: all chunk lists of the column are sorted at once (at most
+// T*E entries).  STREAM = true: the window of T*E = 2*Mp entries (Mp >= M,
+// a power of two) holds the running top Mp in its first half; each further
+// chunk list is loaded reversed into the second half and one bitonic merge
+// level keeps the top Mp (chunk lists are sorted descending by pass 2).
+template <int E, int T, bool STREAM>
 __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet, const uint32_t* pslot,
                                                Slot* tab, int M, int nchunks, const StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
@@ -576,32 +583,62 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
   }
   __syncthreads();
   const int n = total;
-  if (n > T * E) {
-    if (threadIdx.x == 0) st->error = ERR_CAP;
-    return;
-  }
   uint64_t A[E], B[E];
   uint32_t P[E];
+  if constexpr (!STREAM) {
+    if (n > T * E) {
+      if (threadIdx.x == 0) st->error = ERR_CAP;
+      return;
+    }
 #pragma unroll
-  for (int r = 0; r < E; ++r) {
-    const int g = threadIdx.x * E + r;
-    A[r] = 0;
-    B[r] = 0;
-    P[r] = 0xffffffffu;
-    if (g < n) {
-      int lo = 0, hi = nchunks;
-      while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (offs[mid] <= g) lo = mid;
-        else hi = mid;
+    for (int r = 0; r < E; ++r) {
+      const int g = threadIdx.x * E + r;
+      A[r] = 0;
+      B[r] = 0;
+      P[r] = 0xffffffffu;
+      if (g < n) {
+        int lo = 0, hi = nchunks;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (offs[mid] <= g) lo = mid;
+          else hi = mid;
+        }
+        const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
+        A[r] = e.val;
+        B[r] = e.det;
+        P[r] = e.slot;
       }
-      const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
-      A[r] = e.val;
-      B[r] = e.det;
-      P[r] = e.slot;
+    }
+    block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
+  } else {
+    const int half = T * E / 2;  // >= mw
+    auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint32_t& p) {
+      a = 0;
+      b = 0;
+      p = 0xffffffffu;
+      if (chunk < nchunks && j < offs[chunk + 1] - offs[chunk]) {
+        const StageEnt& e = stage[((size_t)t * nchunks + chunk) * mw + j];
+        a = e.val;
+        b = e.det;
+        p = e.slot;
+      }
+    };
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const int i = threadIdx.x * E + r;
+      if (i < half) fetch(0, i, A[r], B[r], P[r]);
+      else fetch(1, 2 * half - 1 - i, A[r], B[r], P[r]);
+    }
+    block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
+    for (int chunk = 2; chunk < nchunks; ++chunk) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const int i = threadIdx.x * E + r;
+        if (i >= half) fetch(chunk, 2 * half - 1 - i, A[r], B[r], P[r]);
+      }
+      block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
     }
   }
-  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
   // First M entries; keep those present in the table (order preserved).
   const int m = n < M ? n : M;
   {
@@ -768,6 +805,125 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   }
 }
 
+// ------------------------------------------------------------------ pick for k*M > 4096
+// The first k distinct candidates of the full (key desc, det desc) order,
+// found without one global sort: every column list is sorted and cut to
+// L = min(M, k) entries (an entry below position k of its own list has k
+// distinct determinants above it), then lists are merged pairwise, adjacent
+// duplicates dropped and the result cut to L distinct entries, until one
+// list remains.  Same result as sorting all candidates at once.
+template <int E, int T>
+__global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, const CandEnt* cand,
+                                                 const int* cand_cnt, const Slot* tab, PickEnt* out,
+                                                 int* out_cnt) {
+  __shared__ uint64_t ka[T * E];
+  __shared__ uint64_t kb[T * E];
+  __shared__ uint32_t pl[T * E];
+  if (st->error) return;
+  const int t = blockIdx.x;
+  const int n = t < st->k_prev ? min(cand_cnt[t], M) : 0;
+  const dd s = load_xss(st);
+  uint64_t A[E], B[E];
+  uint32_t P[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int g = threadIdx.x * E + r;
+    A[r] = 0;
+    B[r] = 0;
+    P[r] = 0xffffffffu;
+    if (g < n) {
+      const CandEnt ce = cand[(size_t)t * M + g];
+      const Slot& sl = tab[ce.slot];
+      const double z = fix_to_d(fix_join(sl.za, sl.zb));
+      A[r] = bitsd(grad_key(s, sl.x, z));
+      B[r] = ce.det;
+      P[r] = ce.slot;
+    }
+  }
+  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
+  const int keep = n < L ? n : L;
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    if (i < keep) out[(size_t)t * L + i] = PickEnt{A[r], B[r], P[r], 0};
+  }
+  if (threadIdx.x == 0) out_cnt[t] = keep;
+}
+
+template <int E, int T>
+__global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, const PickEnt* in,
+                                                  const int* in_cnt, int nin, PickEnt* out, int* out_cnt) {
+  __shared__ uint64_t ka[T * E];
+  __shared__ uint64_t kb[T * E];
+  __shared__ uint32_t pl[T * E];
+  __shared__ int wtot[T / 64 + 1];
+  if (st->error) return;
+  const int j = blockIdx.x;
+  const int la = 2 * j, lb = 2 * j + 1;
+  const int na = in_cnt[la], nb = lb < nin ? in_cnt[lb] : 0;
+  const int half = T * E / 2;  // >= L
+  uint64_t A[E], B[E];
+  uint32_t P[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    A[r] = 0;
+    B[r] = 0;
+    P[r] = 0xffffffffu;
+    if (i < half) {
+      if (i < na) {
+        const PickEnt e = in[(size_t)la * L + i];
+        A[r] = e.key, B[r] = e.det, P[r] = e.slot;
+      }
+    } else {
+      const int q = 2 * half - 1 - i;
+      if (q < nb) {
+        const PickEnt e = in[(size_t)lb * L + q];
+        A[r] = e.key, B[r] = e.det, P[r] = e.slot;
+      }
+    }
+  }
+  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < E; ++r) kb[threadIdx.x * E + r] = B[r];
+  __syncthreads();
+  bool first[E];
+  int cnt = 0;
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    first[r] = B[r] != 0 && (i == 0 || kb[i - 1] != B[r]);
+    cnt += first[r] ? 1 : 0;
+  }
+  int distinct;
+  int pos = block_excl_scan(cnt, wtot, distinct);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    if (!first[r]) continue;
+    if (pos < L) out[(size_t)j * L + pos] = PickEnt{A[r], B[r], P[r], 0};
+    ++pos;
+  }
+  if (threadIdx.x == 0) out_cnt[j] = distinct < L ? distinct : L;
+}
+
+__global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* in_cnt, const Slot* tab,
+                            uint64_t* pdet, uint32_t* pslot, double* px, double* pz) {
+  if (st->error) return;
+  if (in_cnt[0] < k) {
+    if (threadIdx.x == 0) st->error = ERR_CANDIDATES;
+    return;
+  }
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    const PickEnt e = in[i];
+    pdet[i] = e.det;
+    pslot[i] = e.slot;
+    const Slot& sl = tab[e.slot];
+    px[i] = sl.x;
+    pz[i] = fix_to_d(fix_join(sl.za, sl.zb));
+  }
+}
+
 // ------------------------------------------------------------------ init
 // update_x of the reference determinant with dx = 1 (src/solver.h:864-872).
 __global__ void k_init_ref(DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
@@ -893,47 +1049,100 @@ void launch_merge(hipStream_t s, DevState* st, const uint64_t* pdet, const uint3
                   int M, int k, int nchunks, const StageEnt* stage, const int* stage_cnt,
                   const int64_t* part, int64_t* col_xz, CandEnt* cand, int* cand_cnt, const double* dx,
                   int count_iteration) {
-  // Sort width: the smallest power of two of at least nchunks * min(M, chunk)
-  // entries (at least 256); 256 threads up to 1024 entries, then four
-  // entries per thread.
+  // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
+  // entries, then four entries per thread).  Beyond: the streaming merge
+  // with a window of 2 * Mp entries.
   const int mw = M < kChunk ? M : kChunk;
-  int w = kWG;
-  while (w < nchunks * mw) w <<= 1;
-#define MC_MERGE(E_, T_)                                                                                    \
-  hipLaunchKernelGGL((k_merge<E_, T_>), dim3(k), dim3(T_), 0, s, st, pdet, pslot, tab, M, nchunks, stage,   \
-                     stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration)
-  switch (w) {
-    case 256: MC_MERGE(1, 256); break;
-    case 512: MC_MERGE(2, 256); break;
-    case 1024: MC_MERGE(4, 256); break;
-    case 2048: MC_MERGE(4, 512); break;
-    default: MC_MERGE(4, 1024); break;
+#define MC_MERGE(E_, T_, S_)                                                                              \
+  hipLaunchKernelGGL((k_merge<E_, T_, S_>), dim3(k), dim3(T_), 0, s, st, pdet, pslot, tab, M, nchunks,    \
+                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration)
+  if ((long long)nchunks * mw <= kSortCap) {
+    int w = kWG;
+    while (w < nchunks * mw) w <<= 1;
+    switch (w) {
+      case 256: MC_MERGE(1, 256, false); break;
+      case 512: MC_MERGE(2, 256, false); break;
+      case 1024: MC_MERGE(4, 256, false); break;
+      case 2048: MC_MERGE(4, 512, false); break;
+      default: MC_MERGE(4, 1024, false); break;
+    }
+  } else {
+    int w = kWG;
+    while (w < 2 * mw) w <<= 1;
+    switch (w) {
+      case 256: MC_MERGE(1, 256, true); break;
+      case 512: MC_MERGE(2, 256, true); break;
+      case 1024: MC_MERGE(4, 256, true); break;
+      default: MC_MERGE(8, 256, true); break;
+    }
   }
 #undef MC_MERGE
 }
 
 void launch_pick(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
-                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz) {
-  // Sort width: the smallest power of two of at least k * M candidates (and
-  // at least one wave); four entries per thread from 256 entries up.
-  int w = 64;
-  while (w < k * M) w <<= 1;
-  const int E = w >= 256 ? 4 : (w >= 128 ? 2 : 1);
-  const int threads = w / E;
-  switch (E) {
-    case 1:
-      hipLaunchKernelGGL(k_pick<1>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
-                         px, pz);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_pick<2>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
-                         px, pz);
-      break;
-    default:
-      hipLaunchKernelGGL(k_pick<4>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet, pslot,
-                         px, pz);
-      break;
+                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0,
+                 PickEnt* pk1, int* pc0, int* pc1) {
+  if ((long long)k * M <= kSortCap) {
+    // One workgroup sorts all candidates: the smallest power of two of at
+    // least k * M entries (at least one wave); four entries per thread from
+    // 256 entries up.
+    int w = 64;
+    while (w < k * M) w <<= 1;
+    const int E = w >= 256 ? 4 : (w >= 128 ? 2 : 1);
+    const int threads = w / E;
+    switch (E) {
+      case 1:
+        hipLaunchKernelGGL(k_pick<1>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
+                           pslot, px, pz);
+        break;
+      case 2:
+        hipLaunchKernelGGL(k_pick<2>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
+                           pslot, px, pz);
+        break;
+      default:
+        hipLaunchKernelGGL(k_pick<4>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
+                           pslot, px, pz);
+        break;
+    }
+    return;
   }
+  const int L = M < k ? M : k;
+  // Column sorts: width next power of two >= M (<= 1024).
+  int wc = 64;
+  while (wc < M) wc <<= 1;
+#define MC_COLS(E_, T_) \
+  hipLaunchKernelGGL((k_pick_cols<E_, T_>), dim3(k), dim3(T_), 0, s, st, M, L, cand, cand_cnt, tab, pk0, pc0)
+  switch (wc) {
+    case 64: MC_COLS(1, 64); break;
+    case 128: MC_COLS(1, 128); break;
+    case 256: MC_COLS(1, 256); break;
+    case 512: MC_COLS(2, 256); break;
+    default: MC_COLS(4, 256); break;
+  }
+#undef MC_COLS
+  // Merge levels: window 2 * next power of two >= L.
+  int wm = 64;
+  while (wm < 2 * L) wm <<= 1;
+  PickEnt *in = pk0, *out = pk1;
+  int *ic = pc0, *oc = pc1;
+  int nin = k;
+  while (nin > 1) {
+    const int nout = (nin + 1) / 2;
+#define MC_PM(E_, T_) \
+  hipLaunchKernelGGL((k_pick_merge<E_, T_>), dim3(nout), dim3(T_), 0, s, st, L, in, ic, nin, out, oc)
+    switch (wm) {
+      case 64: MC_PM(1, 64); break;
+      case 128: MC_PM(1, 128); break;
+      case 256: MC_PM(1, 256); break;
+      case 512: MC_PM(2, 256); break;
+      default: MC_PM(4, 256); break;
+    }
+#undef MC_PM
+    std::swap(in, out);
+    std::swap(ic, oc);
+    nin = nout;
+  }
+  hipLaunchKernelGGL(k_pick_emit, dim3(1), dim3(256), 0, s, st, k, in, ic, tab, pdet, pslot, px, pz);
 }
 
 void launch_init_ref(hipStream_t s, DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,

@@ -78,13 +78,17 @@ __device__ __forceinline__ void sort_keep(uint64_t& a, uint64_t& b, uint32_t& p,
   p = take ? pp : p;
 }
 
+// first_size = 2 sorts any input; first_size = n runs only the last level,
+// which sorts a bitonic input (the first half descending, the second
+// ascending) -- the merge of two sorted lists.
 template <int E>
-__device__ __forceinline__ void block_sort_desc(uint64_t (&a)[E], uint64_t (&b)[E], uint32_t (&p)[E], SortScratch x) {
+__device__ __forceinline__ void block_sort_desc(uint64_t (&a)[E], uint64_t (&b)[E], uint32_t (&p)[E], SortScratch x,
+                                                int first_size = 2) {
   const int n = blockDim.x * E;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int base = tid * E;
-  for (int size = 2; size <= n; size <<= 1) {
+  for (int size = first_size; size <= n; size <<= 1) {
     for (int stride = size >> 1; stride >= E; stride >>= 1) {
       const bool desc = (base & size) == 0;
       if (stride < 64 * E) {

@@ -91,9 +91,9 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
     g = Cdfci(path, num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=max_memory, threshold=0.0)
     g.init()
     o = Oracle(path)
-    o.solver_init(k, M, z_thr, g.report()["max_size"], nthreads=4)
+    o.solver_init(k, M, z_thr, g.report()["max_size"], nthreads=8)
     done = 0
-    for n in list(checkpoints) + [iters]:
+    for n in [c for c in checkpoints if c < iters] + [iters]:
         step = n - done
         if step <= 0:
             continue
@@ -124,11 +124,17 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
     ("c3_hubbard", 4, 8, 0.0, 200),
     ("h2o", 8, 8, 0.0, 300),
     ("c2", 8, 8, 1e-7, 60),
+    # k * M above one workgroup's sort: per-column sorts and a merge tree in the pick
+    ("c3_synth", 128, 128, 0.0, 60),
+    # chunk lists above one workgroup's sort: streaming merge of the column top-M
+    ("c2", 16, 1024, 1e-7, 12),
+    # the norb-24 molecule at k = M = 256
+    ("c4", 256, 256, 1e-6, 4),
 ])
 def test_trajectory_bit_exact(name, k, M, z_thr, iters, workloads, h2o_fcidump):
     print("case", name, k, M, z_thr, iters, flush=True)
     path = _path(name, workloads, h2o_fcidump)
-    mem = 8.0 if name == "c2" else 0.05
+    mem = 8.0 if name in ("c2", "c4") else 0.05
     _run_pair(path, k, M, z_thr, iters, max_memory=mem)
 
 
