@@ -62,7 +62,6 @@ struct mcdfci_ctx {
   double* pz = nullptr;
   double* dx = nullptr;
   double* Hs = nullptr;
-  double* Aglob = nullptr;
   uint64_t* vis = nullptr;
   int vis_words = 0;
   int nchunks = 0;
@@ -243,7 +242,7 @@ void table_size(double max_memory, int nwords, uint64_t& slots, double& max_size
 
 int alloc_iteration(mcdfci_ctx* c) {
   for (void* p : {(void*)c->pdet, (void*)c->pslot, (void*)c->px, (void*)c->pz, (void*)c->dx, (void*)c->Hs,
-                  (void*)c->Aglob, (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
+                  (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
                   (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt, (void*)c->pk0, (void*)c->pk1,
                   (void*)c->pc0, (void*)c->pc1})
     dfree(c, p);
@@ -261,7 +260,6 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->pz, k))) return rc;
   if ((rc = dalloc(c, &c->dx, k))) return rc;
   if ((rc = dalloc(c, &c->Hs, (size_t)k * k))) return rc;
-  if ((rc = dalloc(c, &c->Aglob, (size_t)(k + 1) * (k + 1)))) return rc;
   if ((rc = dalloc(c, &c->vis, (size_t)k * c->vis_words))) return rc;
   if ((rc = dalloc(c, &c->stage, (size_t)k * c->nchunks * mw))) return rc;
   if ((rc = dalloc(c, &c->stage_cnt, (size_t)k * c->nchunks))) return rc;
@@ -325,7 +323,7 @@ void launch_iteration(mcdfci_ctx* c) {
   if (c->timing) (void)hipEventRecord(c->ev[1], s);
   launch_submatrix(s, c->dh, c->st, c->pdet, k, c->Hs, k);
   if (c->timing) (void)hipEventRecord(c->ev[2], s);
-  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, c->Aglob, c->tab, c->pslot, 0);
+  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, c->tab, c->pslot, 0);
   if (c->timing) (void)hipEventRecord(c->ev[3], s);
   launch_update_z(c, k, 1);
   if (c->timing) {
@@ -635,12 +633,25 @@ int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* va
 }
 
 // Diagnostic (not part of the public header): phase timestamps of the last
-// k_eig launch in wall-clock ticks.
+// k_eig launch in wall-clock ticks (slots 0-3, 7-9), and the Lanczos vectors
+// of its first attempt (10), its attempts (11) and the last attempt's
+// vectors (12).
 int mcdfci_diag_eig_profile(mcdfci_ctx* c, int64_t* out16) {
   if (!c || !out16) return MCDFCI_E_INPUT;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   DevState s;
   HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 16; ++i) out16[i] = s.prof[i];
+  return MCDFCI_OK;
+}
+
+// Diagnostic (not part of the public header): the profile slots of the
+// last mcdfci_eig_update call.
+int mcdfci_diag_hook_profile(mcdfci_ctx* c, int64_t* out16) {
+  if (!c || !out16) return MCDFCI_E_INPUT;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st_scratch, sizeof(s), hipMemcpyDeviceToHost));
   for (int i = 0; i < 16; ++i) out16[i] = s.prof[i];
   return MCDFCI_OK;
 }
@@ -678,14 +689,17 @@ int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, 
                       double* dxo) {
   if (!c || !Hin || !x || !z || !s || !dxo || k < 1 || k > kMaxK) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   HIPCHECK(c, hipSetDevice(c->device));
-  double *h = nullptr, *dxx = nullptr, *dz = nullptr, *ddx = nullptr, *A = nullptr;
+  double *h = nullptr, *dxx = nullptr, *dz = nullptr, *ddx = nullptr;
   int rc;
   if ((rc = dalloc(c, &h, (size_t)k * k))) return rc;
   if ((rc = dalloc(c, &dxx, k))) return rc;
   if ((rc = dalloc(c, &dz, k))) return rc;
   if ((rc = dalloc(c, &ddx, k))) return rc;
-  if ((rc = dalloc(c, &A, (size_t)(k + 1) * (k + 1)))) return rc;
-  HIPCHECK(c, hipMemcpy(h, Hin, 8 * (size_t)k * k, hipMemcpyHostToDevice));
+  // The update reads the upper triangle (j >= i); the kernel takes both.
+  std::vector<double> hsym((size_t)k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = i; j < k; ++j) hsym[(size_t)i * k + j] = hsym[(size_t)j * k + i] = Hin[(size_t)i * k + j];
+  HIPCHECK(c, hipMemcpy(h, hsym.data(), 8 * (size_t)k * k, hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemcpy(dxx, x, 8 * (size_t)k, hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemcpy(dz, z, 8 * (size_t)k, hipMemcpyHostToDevice));
   DevState st{};
@@ -697,12 +711,12 @@ int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, 
   st.xz_hi = (int64_t)(xz >> 64);
   st.xz_lo = (uint64_t)xz;
   HIPCHECK(c, hipMemcpy(c->st_scratch, &st, sizeof(st), hipMemcpyHostToDevice));
-  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, A, nullptr, nullptr, 1);
+  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, nullptr, nullptr, 1);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(&st, c->st_scratch, sizeof(st), hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(dxo, ddx, 8 * (size_t)k, hipMemcpyDeviceToHost));
-  for (void* p : {(void*)h, (void*)dxx, (void*)dz, (void*)ddx, (void*)A}) dfree(c, p);
+  for (void* p : {(void*)h, (void*)dxx, (void*)dz, (void*)ddx}) dfree(c, p);
   s[0] = st.xx_hi;
   s[1] = st.xx_lo;
   s[2] = st.sc_hi;

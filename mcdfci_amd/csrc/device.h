@@ -81,6 +81,7 @@ struct DevState {
   double dx_max;
   double max_residual, curr_max_abs_evec, sum_diff_inv, max_diff_inv, curr_diff_inv;
   long long sum_iterations, sum_inv_iter, curr_inv_iter;
+  int lz_prev;     // Lanczos vectors used by the previous subspace solve
   long long prof[16];  // phase timestamps of the last k_eig (wall clock ticks)
 };
 

@@ -1,24 +1,35 @@
 // Subspace update of the CDFCI iteration (src/solver.h:333-609) and
 // update_x (src/wavefunction.h:425-451) in one workgroup of 256 threads.
 //
-// The (k+1)-dimensional problem [alpha b^T; b H_sub] is solved for its
-// lowest eigenpair by
-//   1. Householder tridiagonalisation (dsytd2 conventions; the trailing
-//      block is updated column-per-lane, which keeps it bitwise symmetric,
-//      and the four waves take the columns c = w mod 4),
-//   2. Laguerre's iteration on det(T - lambda I) from the Gershgorin lower
-//      bound (monotone and cubically convergent for the smallest root),
-//      safeguarded by a bracket,
-//   3. three inverse-iteration solves on the tridiagonal matrix (double)
-//      with one LU factorisation and reciprocal pivots,
-//   4. the extended-precision refinement step of the reference
-//      (src/solver.h:437-503) as one double-double inverse-iteration solve
-//      with shift lambda - 1e-14, back-transformed in double-double.
-// Row sums of matrix-vector products use four interleaved partial sums
-// (c = j mod 4, ascending, combined as (S0 + S1) + (S2 + S3)); the other
-// parallel sums use the 64-lane order (strided partials, then xor
-// butterfly), evaluated redundantly by every wave so that all threads hold
-// the same bits.  Sequential chains run on thread 0.
+// The (k+1)-dimensional problem A = [alpha b^T; b H_sub] is solved for its
+// lowest eigenpair by Lanczos from the current state (sqrt(yy), x_I),
+// normalised and perturbed by a fixed 2^-16 pseudo-random vector:
+//   w = A q_j - beta_{j-1} q_{j-1};  alpha_j = q_j.w;  w -= alpha_j q_j;
+//   w -= sum_i (q_i.w) q_i;  beta_j = |w|;  q_{j+1} = w / beta_j.
+// A is read in place: from LDS when n <= kEigLdsN, otherwise straight from
+// the symmetric H_sub in HBM/L2 (k_submatrix fills both triangles) with the
+// border alpha, b in LDS.  Each wave takes the columns c = w mod 4 of every
+// row (four interleaved partial sums, combined (S0 + S1) + (S2 + S3)); after
+// the barrier every wave combines all rows redundantly, so the dot products
+// (64-lane order: strided partials, xor butterfly) and the next Lanczos
+// vector are known to all waves, and each wave keeps its own LDS copy of
+// the q_j entries it multiplies.
+// After the alpha step one classical Gram-Schmidt pass against all previous
+// Lanczos vectors (held in LDS; thread i forms q_i.w as four interleaved
+// sums over the rows) keeps them orthogonal, so a convergence test made
+// late costs steps but not accuracy.  The LDS holds lz_capacity(n) vectors
+// (12288 doubles shared with A when A is resident); a solve reaching that
+// many stops and restarts from its Ritz vector.  At scheduled steps (from three less than the previous
+// solve's count, then every second step) thread 0 finds the lowest
+// eigenvalue of the Lanczos matrix T_m by Laguerre's iteration on its
+// leading minors (warm-started from the previous check) and its eigenvector
+// by two inverse-iteration solves; the solve stops when
+// beta_j |y_m| <= 1e-15 |T_m|_inf |y|, at breakdown or at m = n.  The Ritz
+// vector is normalised in double-double; the solve is accepted when
+// |A v - rq v| <= 8e-15 |T_m|_inf (rq the Rayleigh quotient) and restarted
+// from v otherwise (at most five times).  The update follows
+// src/solver.h:510-538.  The first iteration (only the reference
+// determinant carries weight) starts from the pseudo-random vector alone.
 
 #include <hip/hip_runtime.h>
 
@@ -27,17 +38,43 @@
 
 namespace mc {
 
-constexpr int kEigLdsN = 90;  // A lives in LDS up to this dimension
+constexpr int kEigLdsN = 90;    // A lives in LDS up to this dimension
+constexpr int kQBudget = 12288;  // LDS doubles for A (n <= kEigLdsN) and the Lanczos basis
 constexpr int kNMax = kMaxK + 1;
-constexpr int kEigThreads = kWG;  // four waves
+constexpr int kRows = (kNMax + 63) / 64;  // rows per lane
+constexpr int kEigThreads = kWG;          // four waves
+constexpr double kLzTol = 1e-15;
+constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accepts a solve
+constexpr double kLzPerturb = 0x1p-16;
+constexpr int kLzAttempts = 6;
+
+// Lanczos vectors the LDS holds for dimension n; a solve that reaches this
+// many stops and restarts from its Ritz vector.
+__device__ __forceinline__ int lz_capacity(int n) {
+  const int avail = kQBudget - (n <= kEigLdsN ? n * n : 0);
+  const int c = avail / n;
+  return c < n ? c : n;
+}
+
+__device__ __forceinline__ double bfly_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ dd bfly_sum_dd(dd v) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    dd o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
+    v = dd_add(v, o);
+  }
+  return v;
+}
 
 template <class F>
 __device__ __forceinline__ double red64(int n, F term) {
   const int lane = threadIdx.x & 63;
   double acc = 0.0;
   for (int i = lane; i < n; i += 64) acc = acc + term(i);
-  for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
-  return acc;
+  return bfly_sum(acc);
 }
 
 template <class F>
@@ -45,28 +82,27 @@ __device__ __forceinline__ dd red64_dd(int n, F term) {
   const int lane = threadIdx.x & 63;
   dd acc{0.0, 0.0};
   for (int i = lane; i < n; i += 64) acc = dd_add(acc, term(i));
-  for (int off = 32; off >= 1; off >>= 1) {
-    dd o{__shfl_xor(acc.hi, off, 64), __shfl_xor(acc.lo, off, 64)};
-    acc = dd_add(acc, o);
-  }
-  return acc;
+  return bfly_sum_dd(acc);
 }
 
-// max |y_i| over i < n (exact, so any order gives the same bits).
-__device__ __forceinline__ double wave_absmax(int n, const double* y) {
-  const int lane = threadIdx.x & 63;
-  double m = 0.0;
-  for (int i = lane; i < n; i += 64) m = fmax(m, __builtin_fabs(y[i]));
-  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  return m;
+// Fixed pseudo-random value in [-1, 1) for index i (integer hash, exact).
+__device__ __forceinline__ double lz_hash_unit(int i) {
+  uint32_t h = (uint32_t)(i + 1) * 0x9E3779B1u;
+  h ^= h >> 15;
+  h *= 0x85EBCA77u;
+  h ^= h >> 13;
+  h *= 0xC2B2AE3Du;
+  h ^= h >> 16;
+  return (double)(h >> 8) * 0x1p-23 - 1.0;
 }
 
 struct EigLds {
   double x[kMaxK], z[kMaxK], Hx[kMaxK], b0[kMaxK];
-  double v[kNMax], p[kNMax], w[kNMax], col[kNMax];
-  double d[kNMax], e[kNMax], e2[kNMax], tau[kNMax], y[kNMax];
-  double part[4][kNMax];
-  double dg[kNMax], du[kNMax], du2[kNMax], mul[kNMax], rdg[kNMax];
+  double v0[kNMax];
+  double part[2][4][kNMax];
+  double qw[4][(kNMax + 3) / 4];  // wave w's copy of q_j[4i + w]
+  double alpha[kNMax], beta[kNMax], e2[kNMax], hbuf[kNMax], wv[kNMax];
+  double dg[kNMax], du[kNMax], du2[kNMax], mul[kNMax], rdg[kNMax], y[kNMax];
   int sw[kNMax];
   dd q[kNMax];
   double ev[kNMax], dl[kNMax], Ae[kNMax];
@@ -74,22 +110,56 @@ struct EigLds {
   int iv[8];
 };
 
-// Four-way partial row sums: thread (wave w, lane) accumulates the columns
-// c = w mod 4 of rows lane, lane + 64, ...; part[w][r] receives S_w(r).
-template <class F>
-__device__ __forceinline__ void dot4_partials(EigLds& L, int rows, int cols, F term) {
+__device__ __forceinline__ double dot4_combine(double (*part)[kNMax], int r) {
+  return (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+}
+
+// Subspace matrix entries A(r, c).
+struct MatLds {  // symmetric, column-major in LDS
+  const double* a;
+  int n;
+  __device__ __forceinline__ double operator()(int r, int c) const { return a[c * n + r]; }
+};
+
+// Four-way partial row sums of A v: wave w accumulates the columns c = w
+// mod 4 (ascending) of the rows lane, lane + 64, ... (held in registers)
+// and stores S_w(r) in part[w][r].  vw(i) returns v[4i + w].
+template <class Mat, class V>
+__device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (*part)[kNMax]) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = lane; r < rows; r += 64) {
-    double acc = 0.0;
-#pragma unroll 4
-    for (int c = w; c < cols; c += 4) acc = acc + term(r, c);
-    L.part[w][r] = acc;
+  double acc[kRows];
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
+  int i = 0;
+  for (int c = w; c < n; c += 4, ++i) {
+    const double vc = vw(i);
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) acc[t] = acc[t] + A(r, c) * vc;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    const int r = lane + 64 * t;
+    if (r < n) part[w][r] = acc[t];
   }
 }
 
-__device__ __forceinline__ double dot4_combine(const EigLds& L, int r) {
-  return (L.part[0][r] + L.part[1][r]) + (L.part[2][r] + L.part[3][r]);
-}
+struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
+  const double* hs;
+  int ldh;
+  const double* b0;
+  double a00;
+  int o;
+  __device__ __forceinline__ double operator()(int r, int c) const {
+    if (o) {
+      if (c == 0) return r == 0 ? a00 : b0[r - 1];
+      if (r == 0) return b0[c - 1];
+    }
+    return hs[(size_t)(c - o) * ldh + (r - o)];
+  }
+};
 
 // LU factorisation with partial pivoting of T - shift*I (dgttrf
 // conventions), by one thread; factors land in dg/du/du2/mul/sw.  The
@@ -160,36 +230,6 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const doub
   }
 }
 
-// Double factors, double-double right-hand side (the reference's double LU
-// with binary128 vectors, src/solver.h:459-478).
-__device__ __forceinline__ void tri_solve_dd(int n, const double* dg, const double* rdg, const double* du,
-                                             const double* du2, const double* mul, const int* swp, dd* b) {
-  dd cur = b[0];
-  for (int i = 0; i + 1 < n; ++i) {
-    const dd nb = b[i + 1];
-    if (!swp[i]) {
-      b[i] = cur;
-      cur = dd_sub(nb, dd_mul_d(cur, mul[i]));
-    } else {
-      b[i] = nb;
-      cur = dd_sub(cur, dd_mul_d(nb, mul[i]));
-    }
-  }
-  dd b1 = dd_div_rcp(cur, dg[n - 1], rdg[n - 1]);
-  b[n - 1] = b1;
-  if (n < 2) return;
-  dd b0 = dd_div_rcp(dd_sub(b[n - 2], dd_mul_d(b1, du[n - 2])), dg[n - 2], rdg[n - 2]);
-  b[n - 2] = b0;
-  for (int i = n - 3; i >= 0; --i) {
-    dd t = dd_sub(b[i], dd_mul_d(b0, du[i]));
-    t = dd_sub(t, dd_mul_d(b1, du2[i]));
-    const dd nb = dd_div_rcp(t, dg[i], rdg[i]);
-    b[i] = nb;
-    b1 = b0;
-    b0 = nb;
-  }
-}
-
 // Leading-minor recurrence of T - lam*I, p_i = (d_i - lam) p_{i-1} -
 // e_{i-1}^2 p_{i-2}, with first and second lam-derivatives, rescaled by
 // exact powers of two whenever p_i leaves [2^-300, 2^300] (no division on
@@ -227,11 +267,14 @@ __device__ __forceinline__ bool minor_pass(int n, const double* d, const double*
 }
 
 // Lowest eigenvalue of tridiag(e, d, e): Laguerre from the Gershgorin lower
-// bound; lam always passes minor_pass.  A failed candidate is within
+// bound, or from `hint` when it lies above that bound and passes
+// minor_pass; lam always passes minor_pass.  A failed candidate is within
 // rounding of the root (the exact iteration never passes it): the point
 // 1e-13 * (|cand| + 1) below it is tried once and, if it passes, is the
 // answer; otherwise the failed points bound a bisection.
-__device__ __forceinline__ double tri_lowest(int n, const double* d, const double* e, const double* e2) {
+__device__ __forceinline__ double tri_lowest(int n, const double* d, const double* e, const double* e2,
+                                             double hint) {
+  if (n == 1) return d[0];
   double lo = d[0], hi = d[0];
   for (int i = 0; i < n; ++i) {
     double r = 0.0;
@@ -245,7 +288,11 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
   hi = hi + pad;
   const double nn = (double)n;
   double G, Hh;
-  if (!minor_pass(n, d, e2, lam, G, Hh)) return lam;
+  if (hint > lam && hint < hi && minor_pass(n, d, e2, hint, G, Hh)) {
+    lam = hint;
+  } else if (!minor_pass(n, d, e2, lam, G, Hh)) {
+    return lam;
+  }
   for (int it = 0; it < 100; ++it) {
     double disc = (nn - 1.0) * (nn * Hh - G * G);
     if (!(disc > 0.0)) disc = 0.0;
@@ -274,146 +321,218 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
     if (threadIdx.x == 0 && prof) prof[i] = (long long)wall_clock64(); \
   } while (0)
 
-// Lowest eigenpair of the symmetric n x n matrix in A (destroyed).  Leaves
-// lambda in L.sc[0] and the normalised dd eigenvector in L.q.
-__device__ __forceinline__ void lowest_eigpair(EigLds& L, double* A, int n, long long* prof) {
-  const int tid = threadIdx.x;
-  const int bs = blockDim.x;
-  if (n == 1) {
-    if (tid == 0) {
-      L.sc[0] = A[0];
-      L.q[0] = dd{1.0, 0.0};
-    }
-    __syncthreads();
-    return;
-  }
-  for (int s = 0; s <= n - 3; ++s) {
-    const int m = n - s - 1;
-    for (int i = tid; i < m; i += bs) L.col[i] = A[s * n + s + 1 + i];
-    __syncthreads();
-    const double alpha = L.col[0];
-    const double xnorm2 = red64(m - 1, [&](int i) {
-      double t = L.col[1 + i];
-      return t * t;
-    });
-    if (tid == 0) L.d[s] = A[s * n + s];
-    if (xnorm2 == 0.0) {
-      if (tid == 0) {
-        L.tau[s] = 0.0;
-        L.e[s] = alpha;
-      }
-      __syncthreads();
-      continue;
-    }
-    const double beta = -__builtin_copysign(__builtin_sqrt(alpha * alpha + xnorm2), alpha);
-    const double tt = (beta - alpha) / beta;
-    const double scl = 1.0 / (alpha - beta);
-    for (int i = tid; i < m; i += bs) {
-      double vi = (i == 0) ? 1.0 : L.col[i] * scl;
-      L.v[i] = vi;
-      A[s * n + s + 1 + i] = vi;
-    }
-    if (tid == 0) {
-      L.tau[s] = tt;
-      L.e[s] = beta;
-    }
-    __syncthreads();
-    const double* B = A + (size_t)(s + 1) * n + (s + 1);
-    dot4_partials(L, m, m, [&](int r, int c) { return B[(size_t)c * n + r] * L.v[c]; });
-    __syncthreads();
-    for (int r = tid; r < m; r += bs) L.p[r] = tt * dot4_combine(L, r);
-    __syncthreads();
-    const double K = (-0.5 * tt) * red64(m, [&](int i) { return L.p[i] * L.v[i]; });
-    for (int r = tid; r < m; r += bs) L.w[r] = L.p[r] + K * L.v[r];
-    __syncthreads();
-    {
-      const int w = tid >> 6, lane = tid & 63;
-      double* Bw = A + (size_t)(s + 1) * n + (s + 1);
-      for (int r = lane; r < m; r += 64) {
-        const double vr = L.v[r], wr = L.w[r];
-#pragma unroll 4
-        for (int c = w; c < m; c += 4) {
-          double a = Bw[(size_t)c * n + r];
-          Bw[(size_t)c * n + r] = a - (L.v[c] * wr + L.w[c] * vr);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  EIG_STAMP(3);
-  if (tid == 0) {
-    L.d[n - 2] = A[(n - 2) * n + (n - 2)];
-    L.d[n - 1] = A[(n - 1) * n + (n - 1)];
-    L.e[n - 2] = A[(n - 1) * n + (n - 2)];
-  }
-  __syncthreads();
-  for (int i = tid; i + 1 < n; i += bs) L.e2[i] = L.e[i] * L.e[i];
-  __syncthreads();
-  if (tid == 0) {
-    const double lam = tri_lowest(n, L.d, L.e, L.e2);
-    L.sc[0] = lam;
-    tri_factor(n, L.d, L.e, lam, L.dg, L.du, L.du2, L.mul, L.sw);
-  }
-  __syncthreads();
-  EIG_STAMP(4);
-  for (int i = tid; i < n; i += bs) {
+// Eigenvector of the Lanczos matrix T_m for eigenvalue lam (thread 0; lam
+// within rounding of the root): two inverse-iteration solves from the
+// all-ones vector on one factorisation, each followed by division by the
+// largest magnitude.  Leaves y in L.y.
+__device__ void tri_vector(EigLds& L, int m, double lam) {
+  tri_factor(m, L.alpha, L.beta, lam, L.dg, L.du, L.du2, L.mul, L.sw);
+  for (int i = 0; i < m; ++i) {
     L.rdg[i] = 1.0 / L.dg[i];
     L.y[i] = 1.0;
   }
-  __syncthreads();
-  for (int it = 0; it < 3; ++it) {
-    if (tid == 0) tri_solve_d(n, L.rdg, L.du, L.du2, L.mul, L.sw, L.y);
-    __syncthreads();
-    const double mx = wave_absmax(n, L.y);
-    __syncthreads();
-    for (int i = tid; i < n; i += bs) L.y[i] = L.y[i] / mx;
-    __syncthreads();
+  for (int it = 0; it < 2; ++it) {
+    tri_solve_d(m, L.rdg, L.du, L.du2, L.mul, L.sw, L.y);
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i) mx = fmax(mx, __builtin_fabs(L.y[i]));
+    for (int i = 0; i < m; ++i) L.y[i] = L.y[i] / mx;
   }
-  for (int i = tid; i < n; i += bs) L.q[i] = mkdd(L.y[i]);
-  if (tid == 0) tri_factor(n, L.d, L.e, L.sc[0] - 1e-14, L.dg, L.du, L.du2, L.mul, L.sw);
-  __syncthreads();
-  for (int i = tid; i < n; i += bs) L.rdg[i] = 1.0 / L.dg[i];
-  __syncthreads();
-  if (tid == 0) tri_solve_dd(n, L.dg, L.rdg, L.du, L.du2, L.mul, L.sw, L.q);
-  __syncthreads();
-  EIG_STAMP(5);
-  // Back-transform q <- H_0 ... H_{n-3} q in double-double.
-  for (int s = n - 3; s >= 0; --s) {
-    const double ts = L.tau[s];
-    if (ts == 0.0) continue;
-    const int m = n - s - 1;
-    const double* vs = A + (size_t)s * n + s + 1;
-    const dd r = red64_dd(m, [&](int i) { return dd_mul_d(L.q[s + 1 + i], vs[i]); });
-    const dd t = dd_mul_d(r, ts);
-    __syncthreads();
-    for (int i = tid; i < m; i += bs) L.q[s + 1 + i] = dd_sub(L.q[s + 1 + i], dd_mul_d(t, vs[i]));
-    __syncthreads();
+}
+
+// Lowest eigenpair of the n x n matrix A by Lanczos from L.v0 (normalised,
+// plus perturb times the fixed pseudo-random vector).  Q (LDS) holds the
+// Lanczos vectors, row i at Q + i * n, at most lz_capacity(n) of them.
+// Leaves the eigenvalue in L.sc[0], |T_m|_inf in L.sc[1], the normalised
+// double-double Ritz vector (first nonzero entry positive) in L.q and the
+// number of Lanczos vectors in L.iv[2].
+template <class Mat>
+__device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, double perturb, double* Q,
+                               long long* prof) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int mcap = lz_capacity(n);
+  double QC[kRows], QP[kRows], W[kRows];
+  {
+    double p0 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) p0 = p0 + L.v0[r] * L.v0[r];
+    }
+    const double s0 = __builtin_sqrt(bfly_sum(p0));
+    double p1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      QP[t] = 0.0;
+      QC[t] = 0.0;
+      if (r < n) {
+        const double q = (s0 > 0.0 ? L.v0[r] / s0 : 0.0) + perturb * lz_hash_unit(r);
+        QC[t] = q;
+        p1 = p1 + q * q;
+      }
+    }
+    const double s1 = __builtin_sqrt(bfly_sum(p1));
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) {
+        QC[t] = QC[t] / s1;
+        if ((lane & 3) == w) L.qw[w][r >> 2] = QC[t];
+        if ((t & 3) == w) Q[r] = QC[t];
+      }
+    }
   }
-  EIG_STAMP(6);
-  const dd nrm2 = red64_dd(n, [&](int i) { return dd_mul(L.q[i], L.q[i]); });
-  const dd nrm = dd_sqrt(nrm2);
-  __syncthreads();
-  for (int i = tid; i < n; i += bs) L.q[i] = dd_div(L.q[i], nrm);
+  double aprev = 0.0, bprev = 0.0, bprev2 = 0.0, tn_done = 0.0, tn = 0.0;
+  double hint = -__builtin_huge_val();  // thread 0: warm start of the next check
+  int m = 0;
+  for (int j = 0; j < n; ++j) {
+    mat_partials(A, n, [&](int i) { return L.qw[w][i]; }, L.part[j & 1]);
+    __syncthreads();
+    double(*P)[kNMax] = L.part[j & 1];
+    double pa = 0.0;
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      W[t] = 0.0;
+      if (r < n) {
+        double s = (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
+        if (j > 0) s = s - bprev * QP[t];
+        W[t] = s;
+        pa = pa + QC[t] * s;
+      }
+    }
+    const double alpha = bfly_sum(pa);
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      W[t] = W[t] - alpha * QC[t];
+      if (r < n && w == 0) L.wv[r] = W[t];
+    }
+    __syncthreads();
+    // Full reorthogonalisation, one classical Gram-Schmidt pass: thread i
+    // forms h_i = q_i.w as four interleaved sums over the rows, then every
+    // wave subtracts sum_i h_i q_i in ascending i.
+    for (int i = tid; i <= j; i += blockDim.x) {
+      const double* qi = Q + (size_t)i * n;
+      double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
+      int r = 0;
+      for (; r + 3 < n; r += 4) {
+        S0 = S0 + qi[r] * L.wv[r];
+        S1 = S1 + qi[r + 1] * L.wv[r + 1];
+        S2 = S2 + qi[r + 2] * L.wv[r + 2];
+        S3 = S3 + qi[r + 3] * L.wv[r + 3];
+      }
+      if (r < n) S0 = S0 + qi[r] * L.wv[r];
+      if (r + 1 < n) S1 = S1 + qi[r + 1] * L.wv[r + 1];
+      if (r + 2 < n) S2 = S2 + qi[r + 2] * L.wv[r + 2];
+      L.hbuf[i] = (S0 + S1) + (S2 + S3);
+    }
+    __syncthreads();
+    double pb = 0.0;
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) {
+        double acc = W[t];
+        for (int i = 0; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
+        W[t] = acc;
+        pb = pb + acc * acc;
+      }
+    }
+    const double beta = __builtin_sqrt(bfly_sum(pb));
+    if (j > 0) {
+      double rj = __builtin_fabs(aprev);
+      if (j > 1) rj = rj + __builtin_fabs(bprev2);
+      rj = rj + __builtin_fabs(bprev);
+      tn_done = fmax(tn_done, rj);
+    }
+    double last = __builtin_fabs(alpha);
+    if (j > 0) last = last + __builtin_fabs(bprev);
+    tn = fmax(tn_done, last);
+    m = j + 1;
+    if (tid == 0) {
+      L.alpha[j] = alpha;
+      L.beta[j] = beta;
+      L.e2[j] = beta * beta;
+    }
+    const bool breakdown = !(beta > kLzTol * tn);
+    if (m == n || breakdown || m == mcap || (m >= first_check && ((m - first_check) & 1) == 0)) {
+      if (tid == 0) {
+        const double theta = tri_lowest(m, L.alpha, L.beta, L.e2, hint);
+        tri_vector(L, m, theta);
+        double ss = 0.0;
+        for (int i = 0; i < m; ++i) ss = ss + L.y[i] * L.y[i];
+        const double rn = __builtin_sqrt(ss);
+        const double est = beta * __builtin_fabs(L.y[m - 1]);
+        const bool cv = m == n || breakdown || est <= (kLzTol * tn) * rn;
+        L.iv[0] = (cv || m == mcap) ? 1 : 0;
+        L.sc[0] = theta;
+        // Some eigenvalue of A lies within est / |y| of theta, and the next
+        // lowest Ritz value lies in [lambda_min(A), theta].
+        hint = theta - est / rn;
+      }
+      __syncthreads();
+      if (L.iv[0]) break;
+    }
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      QP[t] = QC[t];
+      if (r < n) {
+        const double q = W[t] / beta;
+        QC[t] = q;
+        if ((lane & 3) == w) L.qw[w][r >> 2] = q;
+        if ((t & 3) == w) Q[(size_t)(j + 1) * n + r] = q;
+      }
+    }
+    aprev = alpha;
+    bprev2 = bprev;
+    bprev = beta;
+  }
+  EIG_STAMP(3);
+  // Ritz vector v = sum_i y_i q_i (every wave, all rows), normalised in
+  // double-double.
+  double V[kRows];
+  dd pn{0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    const int r = lane + 64 * t;
+    V[t] = 0.0;
+    if (r < n) {
+      double acc = 0.0;
+      for (int i = 0; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
+      V[t] = acc;
+      pn = dd_add(pn, two_prod(acc, acc));
+    }
+  }
+  const dd nrm = dd_sqrt(bfly_sum_dd(pn));
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    const int r = lane + 64 * t;
+    if (r < n && w == 0) L.q[r] = dd_div(mkdd(V[t]), nrm);
+  }
   __syncthreads();
   if (tid == 0) {
     int lead = 0;
     while (lead < n - 1 && dd_zero_p(L.q[lead])) ++lead;
     L.iv[1] = dd_neg_p(L.q[lead]) ? 1 : 0;
+    L.iv[2] = m;
+    L.sc[1] = tn;
   }
   __syncthreads();
   if (L.iv[1])
-    for (int i = tid; i < n; i += bs) L.q[i] = dd_neg(L.q[i]);
+    for (int i = tid; i < n; i += blockDim.x) L.q[i] = dd_neg(L.q[i]);
   __syncthreads();
 }
 
 // mode 0: solver iteration (reads picked x/z, writes dx, updates the state
 // and the table: update_x).  mode 1: test hook (state holds the inputs; no
-// table access).
+// table access).  Hs holds both triangles of H_sub.
 __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const double* Hs,
-                                             const double* px, const double* pz, double* dx,
-                                             double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
+                                             const double* px, const double* pz, double* dx, Slot* tab,
+                                             const uint32_t* pslot, int mode) {
   __shared__ EigLds L;
-  __shared__ double Alds[kEigLdsN * kEigLdsN];
+  __shared__ double Qlds[kQBudget];  // A (n <= kEigLdsN), then the Lanczos basis
   if (st->error) return;
   long long* prof = st->prof;
   EIG_STAMP(0);
@@ -436,17 +555,11 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   const bool normal = __builtin_fabs(yy) > 1e-14;
   const int n = normal ? k + 1 : k;
   const int o = normal ? 1 : 0;
-  double* A = (n <= kEigLdsN) ? Alds : Aglob;
-  // H_sub into the trailing block (upper triangle of Hs mirrored).
-  for (int idx = tid; idx < k * k; idx += bs) {
-    const int r = idx / k, c = idx - r * k;
-    A[(size_t)(r + o) * n + (c + o)] = r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r];
-  }
+  const bool in_lds = n <= kEigLdsN;
+  // H x (H read from Hs in place, both triangles).
+  mat_partials(MatSub{Hs, ldh, nullptr, 0.0, 0}, k, [&](int i) { return L.x[4 * i + (tid >> 6)]; }, L.part[0]);
   __syncthreads();
-  const double* Hb = A + (size_t)o * n + o;
-  dot4_partials(L, k, k, [&](int r, int c) { return Hb[(size_t)c * n + r] * L.x[c]; });
-  __syncthreads();
-  for (int r = tid; r < k; r += bs) L.Hx[r] = dot4_combine(L, r);
+  for (int r = tid; r < k; r += bs) L.Hx[r] = dot4_combine(L.part[0], r);
   __syncthreads();
   if (normal) {
     if (tid == 0) {
@@ -455,53 +568,81 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
       dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
       dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
-      const double a = dd_div_d(a00, yy).hi;
-      L.sc[6] = a;
-      A[0] = a;
+      L.sc[6] = dd_div_d(a00, yy).hi;
+      L.v0[0] = y1;
     }
     for (int i = tid; i < k; i += bs) {
-      const double b = (L.z[i] - L.Hx[i]) / y1;
-      L.b0[i] = b;
-      A[i + 1] = b;
-      A[(size_t)(i + 1) * n] = b;
+      L.b0[i] = (L.z[i] - L.Hx[i]) / y1;
+      L.v0[i + 1] = L.x[i];
     }
+  } else {
+    // x is the reference determinant alone, which a symmetry can hide from
+    // the lowest eigenvector: start from the fixed pseudo-random vector.
+    for (int i = tid; i < k; i += bs) L.v0[i] = 0.0;
   }
   __syncthreads();
-  const double a00 = L.sc[6];
-  // Entries of the original matrix (A itself is overwritten below).
-  auto Aorig = [&](int r, int c) -> double {
-    if (!normal) return r <= c ? Hs[r * ldh + c] : Hs[c * ldh + r];
-    if (r == 0 && c == 0) return a00;
-    if (r == 0) return L.b0[c - 1];
-    if (c == 0) return L.b0[r - 1];
-    return (r - 1) <= (c - 1) ? Hs[(r - 1) * ldh + (c - 1)] : Hs[(c - 1) * ldh + (r - 1)];
-  };
+  const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
+  if (in_lds) {
+    for (int idx = tid; idx < n * n; idx += bs) {
+      const int c = idx / n, r = idx - c * n;
+      Qlds[idx] = Ms(r, c);
+    }
+    __syncthreads();
+  }
+  const int first0 = st->lz_prev - 3 > 4 ? st->lz_prev - 3 : 4;
   EIG_STAMP(1);
   EIG_STAMP(2);
-  // Separate inlined calls keep the LDS case on ds_* instructions.
-  if (n <= kEigLdsN)
-    lowest_eigpair(L, Alds, n, prof);
-  else
-    lowest_eigpair(L, Aglob, n, prof);
-  EIG_STAMP(7);
-  const double lam = L.sc[0];
-  if (normal) {
+  const MatLds Ml{Qlds, n};
+  double* const Qb = Qlds + (in_lds ? n * n : 0);
+  // Solve, then accept when |A v - rq v| <= kLzAccept |T|_inf; otherwise
+  // restart from the Ritz vector (checks from 4 vectors on).
+  int first = first0, attempts = 0;
+  double perturb = kLzPerturb;
+  dd rq{0.0, 0.0};
+  double res2 = 0.0;
+  for (;;) {
+    // Separate inlined calls keep the LDS case on ds_* instructions.
+    if (in_lds)
+      lanczos_lowest(L, Ml, n, first, perturb, Qb, prof);
+    else
+      lanczos_lowest(L, Ms, n, first, perturb, Qb, prof);
     for (int i = tid; i < n; i += bs) {
       L.ev[i] = L.q[i].hi;
       L.dl[i] = L.q[i].lo;
     }
     __syncthreads();
-    dot4_partials(L, n, n, [&](int r, int c) { return Aorig(r, c) * L.ev[c]; });
+    if (in_lds)
+      mat_partials(Ml, n, [&](int i) { return L.ev[4 * i + (tid >> 6)]; }, L.part[0]);
+    else
+      mat_partials(Ms, n, [&](int i) { return L.ev[4 * i + (tid >> 6)]; }, L.part[0]);
     __syncthreads();
-    for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L, r);
+    for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L.part[0], r);
     __syncthreads();
-    const double res2 = red64(n, [&](int i) {
-      double t = L.Ae[i] + (-lam) * L.ev[i];
-      return t * t;
-    });
     const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
     const dd t2 = red64_dd(n, [&](int i) { return two_prod(L.dl[i], L.Ae[i]); });
-    const dd rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    const double lrq = rq.hi;
+    res2 = red64(n, [&](int i) {
+      double t = L.Ae[i] + (-lrq) * L.ev[i];
+      return t * t;
+    });
+    ++attempts;
+    // Diagnostics: Lanczos vectors of the first and last attempt, attempts.
+    if (tid == 0 && prof) {
+      if (attempts == 1) prof[10] = L.iv[2];
+      prof[11] = attempts;
+      prof[12] = L.iv[2];
+    }
+    if (__builtin_sqrt(res2) <= kLzAccept * L.sc[1] || attempts == kLzAttempts) break;
+    for (int i = tid; i < n; i += bs) L.v0[i] = L.ev[i];
+    __syncthreads();
+    first = 4;
+    perturb = 0.0;
+  }
+  EIG_STAMP(7);
+  if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
+  const double lam = rq.hi;
+  if (normal) {
     const dd lam_q = dd_neg(rq);
     if (!(lam_q.hi > 0.0)) {
       if (tid == 0) st->error = ERR_NUMERIC;
@@ -531,15 +672,15 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       st->xx_lo = xx.lo;
     }
   } else {
-    if (!(lam < 0.0)) {
+    if (!(rq.hi < 0.0)) {
       if (tid == 0) st->error = ERR_NUMERIC;
       return;
     }
-    const double r = __builtin_sqrt(-lam);
-    for (int i = tid; i < k; i += bs) dx[i] = L.q[i].hi * r - L.x[i];
+    const double r = __builtin_sqrt(-rq.hi);
+    for (int i = tid; i < k; i += bs) dx[i] = L.ev[i] * r - L.x[i];
     if (tid == 0) {
-      st->xx_hi = -lam;
-      st->xx_lo = 0.0;
+      st->xx_hi = -rq.hi;
+      st->xx_lo = -rq.lo;
       st->sc_hi = 1.0;
       st->sc_lo = 0.0;
     }
@@ -590,9 +731,9 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
 }
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
-                const double* pz, double* dx, double* Aglob, Slot* tab, const uint32_t* pslot, int mode) {
+                const double* pz, double* dx, Slot* tab, const uint32_t* pslot, int mode) {
   // The four-way partial sums assume exactly four waves.
-  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, Aglob, tab, pslot, mode);
+  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tab, pslot, mode);
 }
 
 }  // namespace mc

@@ -10,7 +10,7 @@ namespace mc {
 void launch_submatrix(hipStream_t s, const DevHam& H, const DevState* st, const uint64_t* dets, int k,
                       double* Hs, int ldh);
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
-                const double* pz, double* dx, double* Aglob, Slot* tab, const uint32_t* pslot, int mode);
+                const double* pz, double* dx, Slot* tab, const uint32_t* pslot, int mode);
 void launch_scatter(hipStream_t s, const DevHam& H, DevState* st, const uint64_t* pdet, const double* dx,
                     const double* Hs, int ldh, Slot* tab, uint64_t mask, uint64_t* vis, int vis_words,
                     int k, int nchunks);

@@ -273,7 +273,8 @@ __device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
 }
 
 // ------------------------------------------------------------------ k_submatrix
-// One workgroup per row i; H[i*ldh + j] for j >= i.
+// One workgroup per row i computes H[i][j] for j >= i and stores it in both
+// triangles (the subspace solver reads columns of the symmetric matrix).
 __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const uint64_t* dets,
                                                    int k, double* Hs, int ldh) {
   if (st && st->error) return;
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st,
   for (int j = i + threadIdx.x; j < k; j += blockDim.x) {
     double v = (j == i) ? diagonal_of(H, Di) : off_diagonal_of(H, Di, dets[j]);
     Hs[i * ldh + j] = v;
+    Hs[j * ldh + i] = v;
   }
 }
 

@@ -791,10 +791,15 @@ long oc_column(void* p, const uint64_t* det, uint64_t* dets, double* vals, long 
 }
 
 // Canonical lowest eigenpair (for pinning against LAPACK/numpy).
+// The subspace solve from e_0 with no previous solve (convergence tested
+// from 4 Lanczos vectors on); lambda is the Rayleigh quotient.
 void oc_lowest_eig(int n, const double* A, double* lambda, double* q_hi, double* q_lo) {
-  std::vector<double> M(A, A + (size_t)n * n);
-  EigPair ep = lowest_eigpair(n, M);
-  *lambda = ep.lambda;
+  std::vector<double> v0(n, 0.0);
+  v0[0] = 1.0;
+  EigStats st;
+  EigSolve S = solve_lowest(n, [&](int r, int c) { return A[(size_t)r * n + c]; }, v0, st);
+  const EigPair& ep = S.ep;
+  *lambda = S.rq.hi;
   for (int i = 0; i < n; ++i) {
     q_hi[i] = ep.q[i].hi;
     q_lo[i] = ep.q[i].lo;
@@ -819,6 +824,47 @@ int oc_eig_update(int k, const double* Hs, const double* x, const double* z, dou
     stats[2] = st.curr_diff_inv_iter;
   }
   return rc;
+}
+
+// Diagnostics of one normal-branch subspace solve on given inputs (fresh
+// schedule): out = {first attempt's Lanczos vectors, attempts, final
+// attempt's vectors, alpha_0, theta, |T|_inf, residual}.
+void oc_eig_trace(int k, const double* Hs, const double* x, const double* z, const double* s, double* out) {
+  auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
+  double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
+  for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
+  for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
+  std::vector<double> Hx(k);
+  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c) * x[c]; });
+  for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
+  dd xx{s[0], s[1]};
+  dd xz = fixed_to_dd(to_fixed(s[4]) + to_fixed(s[5]));
+  const double yy = dd_add_d(xx, -xTx).hi, y1 = std::sqrt(yy);
+  const int n = k + 1;
+  const double a00 = dd_div_d(dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx), yy).hi;
+  std::vector<double> b(k), v0(n);
+  for (int i = 0; i < k; ++i) b[i] = (z[i] - Hx[i]) / y1;
+  auto A = [&](int r, int c) -> double {
+    if (r == 0) return c == 0 ? a00 : b[c - 1];
+    if (c == 0) return b[r - 1];
+    return H(r - 1, c - 1);
+  };
+  v0[0] = y1;
+  for (int i = 0; i < k; ++i) v0[i + 1] = x[i];
+  EigStats st;
+  EigSolve S = solve_lowest(n, A, v0, st);
+  out[0] = S.steps_first;
+  out[1] = S.attempts;
+  out[2] = S.ep.steps;
+  out[3] = S.ep.alpha0;
+  out[4] = S.ep.lambda;
+  out[5] = S.ep.tn;
+  out[6] = std::sqrt(S.res2);
+  out[7] = S.Ae[0];
+  out[8] = S.ev[0];
+  out[9] = S.rq.hi;
+  out[10] = S.Ae[n - 1];
+  out[11] = S.ev[n - 1];
 }
 
 // Solver. ref_det may be NULL (Hartree-Fock).

@@ -6,37 +6,70 @@
 // (src/solver.h:333-609).  LAPACK is an external, unversioned dependency
 // (Makefile:25), so this restatement fixes one deterministic algorithm that
 // the device solver reproduces operation for operation:
-//   1. Householder tridiagonalisation (LAPACK dsytd2 conventions),
-// with every matrix-vector row sum (here and in the subspace update) taken
-// as four interleaved partial sums, c = j mod 4, combined as
-// (S0 + S1) + (S2 + S3) (dot4),
-//   2. Laguerre's iteration on det(T - lambda I) from the Gershgorin lower
-//      bound for the lowest eigenvalue of the tridiagonal matrix T (for a
-//      polynomial with real roots it rises monotonically to the smallest
-//      root, cubically near it); each pass is one leading-minor recurrence
-//      with its first and second derivatives (no division on the chain),
-//      and lambda stays a point where every leading minor is positive,
-//   3. three steps of inverse iteration on T in double, sharing one LU
-//      factorisation with partial pivoting (dgttrf style); back substitution
-//      multiplies by the rounded reciprocals of the pivots,
-//   4. the reference's extended-precision refinement step: one more solve
-//      with shift lambda - 1e-14 (solver.h:439) using double LU factors and
-//      a double-double vector (as the reference uses double LU factors with
-//      binary128 vectors, solver.h:437-503), back-transformed with the
-//      Householder vectors in double-double and normalised.
-// Reductions use the canonical 64-lane order of oracle_num.h.
+//   1. Lanczos on the (k+1)-dimensional matrix A from the current state
+//      (sqrt(yy), x_I) (normalised, plus a fixed 2^-16 pseudo-random
+//      perturbation so that no eigenvector is missed by symmetry):
+//        w = A q_j - beta_{j-1} q_{j-1};  alpha_j = q_j.w;
+//        w = w - alpha_j q_j;  w = w - sum_i (q_i.w) q_i (one classical
+//        Gram-Schmidt pass over all previous vectors, each q_i.w a four-way
+//        sum over the rows);
+//        beta_j = |w|;  q_{j+1} = w / beta_j,
+//      every matrix-vector row sum taken as four interleaved partial sums,
+//      c = j mod 4, combined as (S0 + S1) + (S2 + S3) (dot4), every dot
+//      product in the canonical 64-lane order (red64);
+//   2. at scheduled steps (from three less than the previous solve's
+//      count, then every second step) the lowest eigenvalue theta of the Lanczos matrix
+//      T_m by Laguerre's iteration on det(T_m - theta I) from the
+//      Gershgorin lower bound, or from theta - est of the previous check
+//      when that point is certified below the root (monotone and cubically
+//      convergent for the
+//      smallest root; each pass is one division-free leading-minor
+//      recurrence), and its eigenvector y by two inverse-iteration
+//      solves on one LU factorisation with partial pivoting (dgttrf
+//      style; back substitution by reciprocal pivots);
+//   3. stop when beta_j |y_m| <= 1e-15 |T_m|_inf |y|, at breakdown
+//      (beta_j <= 1e-15 |T_m|_inf), at m = n or at the on-chip capacity
+//      lz_capacity(n); the Ritz vector
+//      v = sum_i y_i q_i is normalised in double-double;
+//   4. the solve is accepted when |A v - rq v| <= 8e-15 |T_m|_inf (rq the
+//      Rayleigh quotient); otherwise it restarts from v (at most five
+//      times).
+// The subspace update then follows src/solver.h:510-538 in double-double:
+// the first-order Rayleigh correction rq = v.Av + 2 dv.Av, gamma, scale,
+// dx and xx.  The reference's refinement step (one binary128 inverse
+// iteration on double LU factors) improves the vector only to the accuracy
+// of the double factorisation, which the converged Lanczos vector already
+// has (residual ~ 1e-15 |A|).
 #pragma once
 
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "oracle_num.h"
 
 namespace oc {
 
+constexpr double kLzTol = 1e-15;
+constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accepts a solve
+constexpr double kLzPerturb = 0x1p-16;
+constexpr int kLzAttempts = 6;
+// The device keeps the Lanczos basis (and A, up to n = 90) in 12288 doubles
+// of on-chip memory; a solve stops at that many vectors and restarts.
+constexpr int kEigLdsN = 90;
+constexpr int kQBudget = 12288;
+
+static inline int lz_capacity(int n) {
+  const int avail = kQBudget - (n <= kEigLdsN ? n * n : 0);
+  return std::min(n, avail / n);
+}
+
 struct EigPair {
-  double lambda = 0.0;  // bisection eigenvalue of the tridiagonal form
-  std::vector<dd> q;    // refined, normalised eigenvector (q[0] >= 0)
+  double lambda = 0.0;  // Laguerre eigenvalue of the Lanczos matrix
+  std::vector<dd> q;    // normalised Ritz vector, first nonzero entry > 0
+  int steps = 0;        // Lanczos vectors used
+  double tn = 0.0;      // |T_m|_inf at the stop
+  double alpha0 = 0.0;  // first Lanczos coefficient (diagnostics)
 };
 
 // Row sum in the canonical four-way order: S_j = sum over c = j mod 4 in
@@ -55,6 +88,17 @@ static inline double dot4(int m, F term) {
 static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
 static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
 static inline bool dd_ge(dd a, dd b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
+
+// Fixed pseudo-random value in [-1, 1) for index i (integer hash, exact).
+static inline double lz_hash_unit(int i) {
+  uint32_t h = (uint32_t)(i + 1) * 0x9E3779B1u;
+  h ^= h >> 15;
+  h *= 0x85EBCA77u;
+  h ^= h >> 13;
+  h *= 0xC2B2AE3Du;
+  h ^= h >> 16;
+  return (double)(h >> 8) * 0x1p-23 - 1.0;
+}
 
 // LU factorisation with partial pivoting of T - shift*I, T = tridiag(e, d, e)
 // (LAPACK dgttrf conventions: row i and i+1 swap when |sub| > |diag|).
@@ -129,28 +173,6 @@ static void tri_solve_d(int n, const TriLU& F, const double* rdg, double* b) {
   }
 }
 
-// Substitution with double factors and a double-double right-hand side
-// (the reference's double LU with binary128 vectors, src/solver.h:459-478).
-static void tri_solve_dd(int n, const TriLU& F, const double* rdg, dd* b) {
-  for (int i = 0; i + 1 < n; ++i) {
-    if (!F.swp[i]) {
-      b[i + 1] = dd_sub(b[i + 1], dd_mul_d(b[i], F.mul[i]));
-    } else {
-      dd bi = b[i], bi1 = b[i + 1];
-      b[i] = bi1;
-      b[i + 1] = dd_sub(bi, dd_mul_d(bi1, F.mul[i]));
-    }
-  }
-  b[n - 1] = dd_div_rcp(b[n - 1], F.dg[n - 1], rdg[n - 1]);
-  if (n >= 2)
-    b[n - 2] = dd_div_rcp(dd_sub(b[n - 2], dd_mul_d(b[n - 1], F.du[n - 2])), F.dg[n - 2], rdg[n - 2]);
-  for (int i = n - 3; i >= 0; --i) {
-    dd t = dd_sub(b[i], dd_mul_d(b[i + 1], F.du[i]));
-    t = dd_sub(t, dd_mul_d(b[i + 2], F.du2[i]));
-    b[i] = dd_div_rcp(t, F.dg[i], rdg[i]);
-  }
-}
-
 // One pass of the leading-minor recurrence of T - lam*I,
 //   p_i = (d_i - lam) p_{i-1} - e_{i-1}^2 p_{i-2},
 // with its first and second lam-derivatives, rescaled by exact powers of two
@@ -183,13 +205,14 @@ static bool minor_pass(int n, const double* d, const double* e2, double lam, dou
 }
 
 // Lowest eigenvalue of tridiag(e, d, e) by Laguerre's iteration from the
-// Gershgorin lower bound.  lam always passes minor_pass (a lower bound).  In
+// Gershgorin lower bound, or from `hint` when hint lies above that bound and
+// passes minor_pass.  lam always passes minor_pass (a lower bound).  In
 // exact arithmetic the iteration never passes the smallest root, so a
 // candidate that fails is within rounding of it: the point
 // 1e-13 * (|cand| + 1) below it is tried once, and if it passes it is the
 // answer; otherwise the failed points bound the bisection that follows.  At
 // most 100 iterations.
-static double tri_lowest(int n, const double* d, const double* e) {
+static double tri_lowest(int n, const double* d, const double* e, double hint) {
   if (n == 1) return d[0];
   std::vector<double> e2(n - 1);
   for (int i = 0; i + 1 < n; ++i) e2[i] = e[i] * e[i];
@@ -206,7 +229,11 @@ static double tri_lowest(int n, const double* d, const double* e) {
   hi = hi + pad;
   const double nn = (double)n;
   double G, Hh;
-  if (!minor_pass(n, d, e2.data(), lam, G, Hh)) return lam;
+  if (hint > lam && hint < hi && minor_pass(n, d, e2.data(), hint, G, Hh)) {
+    lam = hint;
+  } else if (!minor_pass(n, d, e2.data(), lam, G, Hh)) {
+    return lam;
+  }
   for (int it = 0; it < 100; ++it) {
     double disc = (nn - 1.0) * (nn * Hh - G * G);
     if (!(disc > 0.0)) disc = 0.0;
@@ -230,96 +257,107 @@ static double tri_lowest(int n, const double* d, const double* e) {
   return lam;
 }
 
-// A: n x n symmetric, row-major, both triangles filled with identical bits.
-static EigPair lowest_eigpair(int n, std::vector<double> A) {
+// Eigenvector of tridiag(e, d, e) for eigenvalue lam (lam within rounding
+// of the root, so each solve shrinks the other components by ~1e-12): two
+// inverse-iteration solves from the all-ones vector on one factorisation,
+// each followed by division by the largest magnitude.
+static std::vector<double> tri_vector(int m, const double* d, const double* e, double lam) {
+  std::vector<double> y(m, 1.0);
+  TriLU F = tri_factor(m, d, e, lam);
+  std::vector<double> r = tri_rcp(m, F);
+  for (int it = 0; it < 2; ++it) {
+    tri_solve_d(m, F, r.data(), y.data());
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i) mx = std::max(mx, std::fabs(y[i]));
+    for (int i = 0; i < m; ++i) y[i] = y[i] / mx;
+  }
+  return y;
+}
+
+// Lowest eigenpair of the symmetric n x n matrix A(r, c) by Lanczos from v0
+// (unnormalised).  first_check: the smallest number of Lanczos vectors at
+// which convergence is tested (then every step).
+template <class Aop>
+static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int first_check, double perturb) {
   EigPair out;
+  std::vector<double> Q((size_t)(n + 1) * n), w(n), h(n), alpha(n), beta(n), y;
+  {
+    const double s0 = std::sqrt(red64(n, [&](int i) { return v0[i] * v0[i]; }));
+    double* q = Q.data();
+    for (int i = 0; i < n; ++i) q[i] = (s0 > 0.0 ? v0[i] / s0 : 0.0) + perturb * lz_hash_unit(i);
+    const double s1 = std::sqrt(red64(n, [&](int i) { return q[i] * q[i]; }));
+    for (int i = 0; i < n; ++i) q[i] = q[i] / s1;
+  }
+  const int mcap = lz_capacity(n);
+  double tn_done = 0.0;  // max row sum of the completed rows of T
+  double theta = 0.0, hint = -HUGE_VAL, tn = 0.0;
+  int m = 0;
+  for (int j = 0; j < n; ++j) {
+    const double* qj = &Q[(size_t)j * n];
+    for (int r = 0; r < n; ++r) w[r] = dot4(n, [&](int c) { return A(r, c) * qj[c]; });
+    if (j > 0)
+      for (int r = 0; r < n; ++r) w[r] = w[r] - beta[j - 1] * Q[(size_t)(j - 1) * n + r];
+    alpha[j] = red64(n, [&](int i) { return qj[i] * w[i]; });
+    for (int r = 0; r < n; ++r) w[r] = w[r] - alpha[j] * qj[r];
+    // Full reorthogonalisation: one classical Gram-Schmidt pass against
+    // q_0..q_j (coefficients as four-way sums over the rows, then
+    // subtracted in ascending order).
+    for (int i = 0; i <= j; ++i) h[i] = dot4(n, [&](int r) { return Q[(size_t)i * n + r] * w[r]; });
+    for (int r = 0; r < n; ++r) {
+      double acc = w[r];
+      for (int i = 0; i <= j; ++i) acc = acc - h[i] * Q[(size_t)i * n + r];
+      w[r] = acc;
+    }
+    beta[j] = std::sqrt(red64(n, [&](int i) { return w[i] * w[i]; }));
+    m = j + 1;
+    if (j > 0) {
+      double rj = std::fabs(alpha[j - 1]);
+      if (j > 1) rj = rj + std::fabs(beta[j - 2]);
+      rj = rj + std::fabs(beta[j - 1]);
+      tn_done = std::max(tn_done, rj);
+    }
+    double last = std::fabs(alpha[j]);
+    if (j > 0) last = last + std::fabs(beta[j - 1]);
+    tn = std::max(tn_done, last);
+    const bool breakdown = !(beta[j] > kLzTol * tn);
+    if (m == n || breakdown || m == mcap || (m >= first_check && (m - first_check) % 2 == 0)) {
+      theta = tri_lowest(m, alpha.data(), beta.data(), hint);
+      y = tri_vector(m, alpha.data(), beta.data(), theta);
+      double ss = 0.0;
+      for (int i = 0; i < m; ++i) ss = ss + y[i] * y[i];
+      const double rn = std::sqrt(ss);
+      const double est = beta[j] * std::fabs(y[m - 1]);
+      const bool conv = m == n || breakdown || est <= (kLzTol * tn) * rn;
+      if (conv || m == mcap) break;
+      // Some eigenvalue of A lies within est / |y| of theta, and the next
+      // lowest Ritz value lies in [lambda_min(A), theta].
+      hint = theta - est / rn;
+    }
+    double* qn = &Q[(size_t)(j + 1) * n];
+    for (int r = 0; r < n; ++r) qn[r] = w[r] / beta[j];
+  }
+  std::vector<double> v(n);
+  for (int r = 0; r < n; ++r) {
+    double acc = 0.0;
+    for (int i = 0; i < m; ++i) acc = acc + y[i] * Q[(size_t)i * n + r];
+    v[r] = acc;
+  }
+  const dd nrm = dd_sqrt(red64_dd(n, [&](int i) { return two_prod(v[i], v[i]); }));
   out.q.assign(n, dd{0.0, 0.0});
-  if (n == 1) {
-    out.lambda = A[0];
-    out.q[0] = dd{1.0, 0.0};
-    return out;
-  }
-  std::vector<double> d(n), e(n - 1), tau(n, 0.0);
-  std::vector<std::vector<double>> vs(n);
-  std::vector<double> v(n), p(n), w(n);
-  for (int s = 0; s <= n - 3; ++s) {
-    int m = n - s - 1;
-    double alpha = A[(s + 1) * n + s];
-    double xnorm2 = red64(m - 1, [&](int i) {
-      double t = A[(s + 2 + i) * n + s];
-      return t * t;
-    });
-    d[s] = A[s * n + s];
-    if (xnorm2 == 0.0) {
-      tau[s] = 0.0;
-      e[s] = alpha;
-      continue;
-    }
-    double beta = -std::copysign(std::sqrt(alpha * alpha + xnorm2), alpha);
-    double t = (beta - alpha) / beta;
-    double scal = 1.0 / (alpha - beta);
-    v[0] = 1.0;
-    for (int i = 1; i < m; ++i) v[i] = A[(s + 1 + i) * n + s] * scal;
-    tau[s] = t;
-    e[s] = beta;
-    vs[s].assign(v.begin(), v.begin() + m);
-    for (int r = 0; r < m; ++r) {
-      const double* row = &A[(s + 1 + r) * n + (s + 1)];
-      p[r] = t * dot4(m, [&](int c) { return row[c] * v[c]; });
-    }
-    double K = (-0.5 * t) * red64(m, [&](int i) { return p[i] * v[i]; });
-    for (int r = 0; r < m; ++r) w[r] = p[r] + K * v[r];
-    for (int r = 0; r < m; ++r) {
-      double* row = &A[(s + 1 + r) * n + (s + 1)];
-      for (int c = 0; c < m; ++c) row[c] = row[c] - (v[r] * w[c] + w[r] * v[c]);
-    }
-  }
-  d[n - 2] = A[(n - 2) * n + (n - 2)];
-  d[n - 1] = A[(n - 1) * n + (n - 1)];
-  e[n - 2] = A[(n - 1) * n + (n - 2)];
-
-  double lam = tri_lowest(n, d.data(), e.data());
-  out.lambda = lam;
-
-  // Inverse iteration on T in double: one factorisation, three solves.
-  std::vector<double> y(n, 1.0);
-  {
-    TriLU F1 = tri_factor(n, d.data(), e.data(), lam);
-    std::vector<double> r1 = tri_rcp(n, F1);
-    for (int it = 0; it < 3; ++it) {
-      tri_solve_d(n, F1, r1.data(), y.data());
-      double mx = 0.0;
-      for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(y[i]));
-      for (int i = 0; i < n; ++i) y[i] = y[i] / mx;
-    }
-  }
-
-  // Extended-precision refinement step (shift lambda - 1e-14, solver.h:439).
-  std::vector<dd> q(n);
-  for (int i = 0; i < n; ++i) q[i] = dd_make(y[i]);
-  {
-    TriLU F2 = tri_factor(n, d.data(), e.data(), lam - 1e-14);
-    std::vector<double> r2 = tri_rcp(n, F2);
-    tri_solve_dd(n, F2, r2.data(), q.data());
-  }
-  for (int s = n - 3; s >= 0; --s) {
-    if (tau[s] == 0.0) continue;
-    int m = n - s - 1;
-    const std::vector<double>& vv = vs[s];
-    dd r = red64_dd(m, [&](int i) { return dd_mul_d(q[s + 1 + i], vv[i]); });
-    dd t = dd_mul_d(r, tau[s]);
-    for (int i = 0; i < m; ++i) q[s + 1 + i] = dd_sub(q[s + 1 + i], dd_mul_d(t, vv[i]));
-  }
-  dd nrm2 = red64_dd(n, [&](int i) { return dd_mul(q[i], q[i]); });
-  dd nrm = dd_sqrt(nrm2);
-  for (int i = 0; i < n; ++i) q[i] = dd_div(q[i], nrm);
+  for (int i = 0; i < n; ++i) out.q[i] = dd_div(dd{v[i], 0.0}, nrm);
   int lead = 0;
-  while (lead < n - 1 && dd_is_zero(q[lead])) ++lead;
-  if (dd_lt0(q[lead]))
-    for (int i = 0; i < n; ++i) q[i] = dd_neg(q[i]);
-  out.q = q;
+  while (lead < n - 1 && dd_is_zero(out.q[lead])) ++lead;
+  if (dd_lt0(out.q[lead]))
+    for (int i = 0; i < n; ++i) out.q[i] = dd_neg(out.q[i]);
+  out.lambda = theta;
+  out.steps = m;
+  out.tn = tn;
+  out.alpha0 = alpha[0];
   return out;
 }
+
+// First convergence test of a solve, from the previous solve's count.
+static inline int lz_first_check(int prev) { return std::max(4, prev - 3); }
 
 struct EigStats {
   double max_residual = 0.0;
@@ -330,9 +368,59 @@ struct EigStats {
   double sum_diff_inv_iter = 0.0;
   double max_diff_inv_iter = 0.0;
   double curr_diff_inv_iter = 0.0;
+  int lz_prev = 0;  // Lanczos vectors used by the previous solve
 };
 
 enum { EIG_OK = 0, EIG_NONNEG = 1 };
+
+// One subspace solve with the a-posteriori check: the Ritz vector q (hi
+// parts ev, lo parts dl), Ae = A ev, the double-double Rayleigh quotient
+// rq = ev.Ae + 2 dl.Ae and res2 = |Ae - rq ev|^2.  A solve whose residual
+// exceeds kLzAccept |T|_inf restarts from its Ritz vector, checking from
+// 4 vectors on; the next solve then checks from half the first point.
+struct EigSolve {
+  EigPair ep;
+  std::vector<double> ev, dl, Ae;
+  dd rq{0.0, 0.0};
+  double res2 = 0.0;
+  int attempts = 0;
+  int steps_first = 0;
+};
+
+template <class Aop>
+static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st) {
+  EigSolve S;
+  int first = lz_first_check(st.lz_prev);
+  const int first0 = first;
+  double pert = kLzPerturb;
+  for (int attempt = 0; attempt < kLzAttempts; ++attempt) {
+    S.ep = lanczos_lowest(n, A, v0, first, pert);
+    S.ev.assign(n, 0.0);
+    S.dl.assign(n, 0.0);
+    S.Ae.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      S.ev[i] = S.ep.q[i].hi;
+      S.dl[i] = S.ep.q[i].lo;
+    }
+    for (int r = 0; r < n; ++r) S.Ae[r] = dot4(n, [&](int c) { return A(r, c) * S.ev[c]; });
+    dd t1 = red64_dd(n, [&](int i) { return two_prod(S.ev[i], S.Ae[i]); });
+    dd t2 = red64_dd(n, [&](int i) { return two_prod(S.dl[i], S.Ae[i]); });
+    S.rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    const double lam = S.rq.hi;
+    S.res2 = red64(n, [&](int i) {
+      double t = S.Ae[i] + (-lam) * S.ev[i];
+      return t * t;
+    });
+    S.attempts = attempt + 1;
+    if (attempt == 0) S.steps_first = S.ep.steps;
+    if (std::sqrt(S.res2) <= kLzAccept * S.ep.tn) break;
+    v0 = S.ev;
+    first = 4;
+    pert = 0.0;
+  }
+  st.lz_prev = S.attempts > 1 ? std::max(4, first0 / 2) : S.ep.steps;
+  return S;
+}
 
 // The CDFCI subspace update (src/solver.h:333-609), canonical form.
 // Hs: k x k row-major; only Hs[i*k+j] with j >= i is read (the reference's
@@ -352,44 +440,31 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
 
   if (std::fabs(yy) > 1e-14) {
     int n = k + 1;
-    std::vector<double> A((size_t)n * n);
-    dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
-    A[0] = dd_div_d(a00, yy).hi;
-    for (int i = 0; i < k; ++i) {
-      double b = (z[i] - Hx[i]) / y1;
-      A[(i + 1) * n] = b;
-      A[i + 1] = b;
-    }
-    for (int i = 0; i < k; ++i)
-      for (int j = i; j < k; ++j) {
-        A[(i + 1) * n + (j + 1)] = Hs[i * k + j];
-        A[(j + 1) * n + (i + 1)] = Hs[i * k + j];
-      }
-    EigPair ep = lowest_eigpair(n, A);
-    double lam = ep.lambda;
-    std::vector<double> ev(n), dl(n), Ae(n);
-    for (int i = 0; i < n; ++i) {
-      ev[i] = ep.q[i].hi;
-      dl[i] = ep.q[i].lo;
-    }
-    for (int r = 0; r < n; ++r) Ae[r] = dot4(n, [&](int c) { return A[r * n + c] * ev[c]; });
-    double res2 = red64(n, [&](int i) {
-      double t = Ae[i] + (-lam) * ev[i];
-      return t * t;
-    });
-    st.max_residual = std::max(st.max_residual, std::sqrt(res2));
+    dd a00d = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+    const double a00 = dd_div_d(a00d, yy).hi;
+    std::vector<double> b(k);
+    for (int i = 0; i < k; ++i) b[i] = (z[i] - Hx[i]) / y1;
+    auto A = [&](int r, int c) -> double {
+      if (r == 0) return c == 0 ? a00 : b[c - 1];
+      if (c == 0) return b[r - 1];
+      return H(r - 1, c - 1);
+    };
+    std::vector<double> v0(n);
+    v0[0] = y1;
+    for (int i = 0; i < k; ++i) v0[i + 1] = x[i];
+    EigSolve S = solve_lowest(n, A, v0, st);
+    const std::vector<dd>& q = S.ep.q;
+    const double lam = S.rq.hi;
+    st.max_residual = std::max(st.max_residual, std::sqrt(S.res2));
     double mev = 0.0;
-    for (int i = 1; i < n; ++i) mev = std::max(mev, std::fabs(ev[i]));
+    for (int i = 1; i < n; ++i) mev = std::max(mev, std::fabs(S.ev[i]));
     st.curr_max_abs_evec = mev;
-    dd t1 = red64_dd(n, [&](int i) { return two_prod(ev[i], Ae[i]); });
-    dd t2 = red64_dd(n, [&](int i) { return two_prod(dl[i], Ae[i]); });
-    dd rq = dd_add(t1, dd_mul_d(t2, 2.0));
-    dd lam_q = dd_neg(rq);
+    dd lam_q = dd_neg(S.rq);
     if (!(lam_q.hi > 0.0)) return EIG_NONNEG;
-    dd gam = dd_div_d(ep.q[0], y1);
+    dd gam = dd_div_d(q[0], y1);
     dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
-    for (int i = 0; i < k; ++i) dx[i] = dd_add_d(dd_div(ep.q[i + 1], gam), -x[i]).hi;
-    double sc_d = ev[0] / y1 * std::sqrt(-lam);
+    for (int i = 0; i < k; ++i) dx[i] = dd_add_d(dd_div(q[i + 1], gam), -x[i]).hi;
+    double sc_d = S.ev[0] / y1 * std::sqrt(-lam);
     dd diff = dd_add_d(dd_abs(sc_q), -std::fabs(sc_d));
     double ad = std::fabs(diff.hi);
     st.sum_iterations += 1;
@@ -402,16 +477,17 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
     xx = dd_div(dd_div(lam_q, sc_q), sc_q);
     return EIG_OK;
   }
-  // First iteration: only the reference determinant carries weight.
-  std::vector<double> A((size_t)k * k);
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < k; ++j) A[i * k + j] = H(i, j);
-  EigPair ep = lowest_eigpair(k, A);
-  double lam = ep.lambda;
-  if (!(lam < 0.0)) return EIG_NONNEG;
-  double r = std::sqrt(-lam);
-  for (int i = 0; i < k; ++i) dx[i] = ep.q[i].hi * r - x[i];
-  xx = dd_make(-lam);
+  // First iteration: only the reference determinant carries weight; the
+  // eigenpair of H_sub itself (src/solver.h:552-598), with the eigenvalue
+  // taken as the double-double Rayleigh quotient of the normalised vector.
+  // x is the reference determinant alone, which a symmetry can hide from
+  // the lowest eigenvector: start from the fixed pseudo-random vector.
+  std::vector<double> v0(k, 0.0);
+  EigSolve S = solve_lowest(k, H, v0, st);
+  if (!(S.rq.hi < 0.0)) return EIG_NONNEG;
+  double r = std::sqrt(-S.rq.hi);
+  for (int i = 0; i < k; ++i) dx[i] = S.ev[i] * r - x[i];
+  xx = dd_neg(S.rq);
   sc = dd_make(1.0);
   return EIG_OK;
 }

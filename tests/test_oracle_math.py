@@ -120,6 +120,81 @@ def test_lowest_eigpair_vs_lapack(n):
     assert abs(np.dot(qh, qh) - 1.0) < 1e-14
 
 
+def _lowest(A, n):
+    import ctypes as C
+    lam = C.c_double()
+    qh, ql = np.zeros(n), np.zeros(n)
+    oracle_lib().oc_lowest_eig(n, np.ascontiguousarray(A.reshape(-1)), C.byref(lam), qh, ql)
+    return lam.value, qh, ql
+
+
+def test_lanczos_finds_eigenvalue_hidden_from_start_vector():
+    """The lowest eigenvector lives in a block that the start vector e_0 does
+    not touch (an exact symmetry); the fixed perturbation of the start vector
+    still finds it, as dsyevr would."""
+    rng = np.random.default_rng(3)
+    n1, n2 = 20, 30
+    B1 = rng.normal(size=(n1, n1)) * 0.3
+    B2 = rng.normal(size=(n2, n2)) * 0.3
+    A = np.zeros((n1 + n2, n1 + n2))
+    A[:n1, :n1] = B1 + B1.T - 2.0 * np.eye(n1)
+    A[n1:, n1:] = B2 + B2.T - 6.0 * np.eye(n2)
+    w, V = np.linalg.eigh(A)
+    assert np.all(np.abs(V[:n1, 0]) == 0.0)
+    lam, qh, ql = _lowest(A, n1 + n2)
+    assert abs(lam - w[0]) < 1e-12 * abs(w[0])
+    assert np.linalg.norm(A @ qh - w[0] * qh) < 1e-12 * np.abs(A).sum(1).max()
+
+
+def test_lanczos_degenerate_and_breakdown():
+    # doubly degenerate lowest eigenvalue: any vector of the eigenspace
+    rng = np.random.default_rng(5)
+    n = 24
+    Qm, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    ev = np.concatenate([[-3.0, -3.0], np.linspace(-1.0, 2.0, n - 2)])
+    A = (Qm * ev) @ Qm.T
+    A = (A + A.T) / 2
+    lam, qh, _ = _lowest(A, n)
+    assert abs(lam + 3.0) < 1e-12
+    assert np.linalg.norm(A @ qh + 3.0 * qh) < 1e-12
+    assert abs(qh @ qh - 1.0) < 1e-14
+    # breakdown after one step: A = c I keeps the Krylov space one-dimensional
+    A = -2.5 * np.eye(7)
+    lam, qh, _ = _lowest(A, 7)
+    assert abs(lam + 2.5) < 1e-15 and abs(qh @ qh - 1.0) < 1e-15
+
+
+def test_eig_update_on_solver_subproblems_vs_numpy(workloads):
+    """The subspace update inside a real run (norb-12 molecule, k = 64, the
+    Lanczos step count carried from solve to solve) agrees with the numpy
+    restatement of src/solver.h:333-609 on each iteration's inputs."""
+    import ctypes as C
+    path = workloads("c3_synth")["fcidump"]
+    o = Oracle(path)
+    k = 64
+    o.solver_init(k, k, 0.0, 1e7, nthreads=4)
+    lib = o.lib
+    lib.oc_last_step.restype = C.c_int
+    lib.oc_last_step.argtypes = [C.c_void_p] * 7
+    for it in range(8):
+        assert o.step(1) == 0
+        dets = np.zeros(2 * k, np.uint64)
+        x, z, dx = np.zeros(k), np.zeros(k), np.zeros(k)
+        Hs, s6 = np.zeros(k * k), np.zeros(6)
+        lib.oc_last_step(o.h, *[a.ctypes.data_as(C.c_void_p) for a in (dets, x, z, Hs, s6, dx)])
+        xx = s6[0] + s6[1]
+        hi, lo = s6[4:6].view(np.int64)
+        xz = float((int(hi) << 64) + (int(lo) & ((1 << 64) - 1))) * 2.0 ** -64
+        dx_ref, xx_ref, _ = _reference_update_numpy(Hs.reshape(k, k), x, z, xx, xz)
+        if it == 0:
+            # first-iteration branch: x_new = v sqrt(-lambda) with v's global
+            # sign arbitrary (canonical here, LAPACK's choice in the reference)
+            sgn = np.sign((x + dx) @ (x + dx_ref))
+            dx_ref = sgn * (x + dx_ref) - x
+        assert np.allclose(dx, dx_ref, atol=1e-10, rtol=0), it
+        assert abs(o.report()["xx"] - xx_ref) <= 1e-12 * xx_ref
+
+
 def _reference_update_numpy(Hs, x, z, xx, xz):
     """The subspace update of src/solver.h:333-609 written with numpy.eigh."""
     k = len(x)
