@@ -19,7 +19,7 @@
 // sums over the rows) keeps them orthogonal, so a convergence test made
 // late costs steps but not accuracy.  The LDS holds lz_capacity(n) vectors
 // (12288 doubles shared with A when A is resident); a solve reaching that
-// many stops and restarts from its Ritz vector.  At scheduled steps (from three less than the previous
+// many stops and restarts from its Ritz vector.  At scheduled steps (from one less than the previous
 // solve's count, then every second step) thread 0 finds the lowest
 // eigenvalue of the Lanczos matrix T_m by Laguerre's iteration on its
 // leading minors (warm-started from the previous check) and its eigenvector
@@ -126,17 +126,31 @@ struct MatLds {  // symmetric, column-major in LDS
 // and stores S_w(r) in part[w][r].  vw(i) returns v[4i + w].
 template <class Mat, class V>
 __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (*part)[kNMax]) {
+  constexpr int CB = 6;  // columns whose loads are issued together
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
   double acc[kRows];
 #pragma unroll
   for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
-  int i = 0;
-  for (int c = w; c < n; c += 4, ++i) {
-    const double vc = vw(i);
+  for (int i0 = 0; i0 < nc; i0 += CB) {
+    double a[CB][kRows], v[CB];
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int r = lane + 64 * t;
-      if (r < n) acc[t] = acc[t] + A(r, c) * vc;
+    for (int b = 0; b < CB; ++b) {
+      const int i = i0 + b, c = w + 4 * i;
+      v[b] = i < nc ? vw(i) : 0.0;
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) {
+        const int r = lane + 64 * t;
+        a[b][t] = (i < nc && r < n) ? A(r, c) : 0.0;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      if (i0 + b < nc) {
+#pragma unroll
+        for (int t = 0; t < kRows; ++t)
+          if (lane + 64 * t < n) acc[t] = acc[t] + a[b][t] * v[b];
+      }
     }
   }
 #pragma unroll
@@ -166,13 +180,19 @@ struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
 // elimination chain is carried in registers.
 __device__ __forceinline__ void tri_factor(int n, const double* d, const double* e, double shift, double* dg,
                                            double* du, double* du2, double* mul, int* swp) {
+  // Thread 0 alone; the next entries are loaded one step ahead so that the
+  // LDS latency overlaps the elimination chain.
   const double tiny = 1e-290;
   double cur_dg = d[0] - shift;
   double cur_du = n > 1 ? e[0] : 0.0;
+  double e_i = n > 1 ? e[0] : 0.0, d_n = n > 1 ? d[1] : 0.0, e_n = n > 2 ? e[1] : 0.0;
   for (int i = 0; i + 1 < n; ++i) {
-    const double dl = e[i];
-    double next_dg = d[i + 1] - shift;
-    double next_du = (i + 2 < n) ? e[i + 1] : 0.0;
+    const double dl = e_i;
+    double next_dg = d_n - shift;
+    double next_du = (i + 2 < n) ? e_n : 0.0;
+    e_i = e_n;
+    if (i + 2 < n) d_n = d[i + 2];
+    if (i + 3 < n) e_n = e[i + 2];
     if (__builtin_fabs(cur_dg) >= __builtin_fabs(dl)) {
       if (cur_dg == 0.0) cur_dg = tiny;
       const double m = dl / cur_dg;
@@ -205,14 +225,22 @@ __device__ __forceinline__ void tri_factor(int n, const double* d, const double*
 __device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const double* du, const double* du2,
                                             const double* mul, const int* swp, double* b) {
   double cur = b[0];
+  double nb_n = n > 1 ? b[1] : 0.0, mul_n = n > 1 ? mul[0] : 0.0;
+  int sw_n = n > 1 ? swp[0] : 0;
   for (int i = 0; i + 1 < n; ++i) {
-    const double nb = b[i + 1];
-    if (!swp[i]) {
+    const double nb = nb_n, mi = mul_n;
+    const int si = sw_n;
+    if (i + 2 < n) {
+      nb_n = b[i + 2];
+      mul_n = mul[i + 1];
+      sw_n = swp[i + 1];
+    }
+    if (!si) {
       b[i] = cur;
-      cur = nb - mul[i] * cur;
+      cur = nb - mi * cur;
     } else {
       b[i] = nb;
-      cur = cur - mul[i] * nb;
+      cur = cur - mi * nb;
     }
   }
   double b1 = cur * rdg[n - 1];
@@ -220,10 +248,19 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const doub
   if (n < 2) return;
   double b0 = (b[n - 2] - du[n - 2] * b1) * rdg[n - 2];
   b[n - 2] = b0;
+  double bi_n = n > 2 ? b[n - 3] : 0.0, du_n = n > 2 ? du[n - 3] : 0.0, du2_n = n > 2 ? du2[n - 3] : 0.0,
+         r_n = n > 2 ? rdg[n - 3] : 0.0;
   for (int i = n - 3; i >= 0; --i) {
-    double t = b[i] - du[i] * b0;
-    t = t - du2[i] * b1;
-    const double nb = t * rdg[i];
+    const double bi = bi_n, dui = du_n, du2i = du2_n, ri = r_n;
+    if (i > 0) {
+      bi_n = b[i - 1];
+      du_n = du[i - 1];
+      du2_n = du2[i - 1];
+      r_n = rdg[i - 1];
+    }
+    double t = bi - dui * b0;
+    t = t - du2i * b1;
+    const double nb = t * ri;
     b[i] = nb;
     b1 = b0;
     b0 = nb;
@@ -242,8 +279,13 @@ __device__ __forceinline__ bool minor_pass(int n, const double* d, const double*
   double p = d[0] - lam, dp = -1.0, ddp = 0.0;
   if (!(p > 0.0)) return false;
   bool ok = true;
+  double d_n = n > 1 ? d[1] : 0.0, e_n = n > 1 ? e2[0] : 0.0;
   for (int i = 1; i < n; ++i) {
-    const double a = d[i] - lam, b = e2[i - 1];
+    const double a = d_n - lam, b = e_n;
+    if (i + 1 < n) {
+      d_n = d[i + 1];
+      e_n = e2[i];
+    }
     double pn = a * p - b * pm;
     double dpn = (a * dp - b * dpm) - p;
     double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
@@ -416,6 +458,21 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       const double* qi = Q + (size_t)i * n;
       double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
       int r = 0;
+      for (; r + 15 < n; r += 16) {
+        double qa[16], wa[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          qa[u] = qi[r + u];
+          wa[u] = L.wv[r + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u += 4) {
+          S0 = S0 + qa[u] * wa[u];
+          S1 = S1 + qa[u + 1] * wa[u + 1];
+          S2 = S2 + qa[u + 2] * wa[u + 2];
+          S3 = S3 + qa[u + 3] * wa[u + 3];
+        }
+      }
       for (; r + 3 < n; r += 4) {
         S0 = S0 + qi[r] * L.wv[r];
         S1 = S1 + qi[r + 1] * L.wv[r + 1];
@@ -434,7 +491,18 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       const int r = lane + 64 * t;
       if (r < n) {
         double acc = W[t];
-        for (int i = 0; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
+        int i = 0;
+        for (; i + 7 <= j; i += 8) {
+          double qa[8], ha[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            qa[u] = Q[(size_t)(i + u) * n + r];
+            ha[u] = L.hbuf[i + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+        }
+        for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
         W[t] = acc;
         pb = pb + acc * acc;
       }
@@ -500,7 +568,18 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     V[t] = 0.0;
     if (r < n) {
       double acc = 0.0;
-      for (int i = 0; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
+      int i = 0;
+      for (; i + 7 < m; i += 8) {
+        double qa[8], ya[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          qa[u] = Q[(size_t)(i + u) * n + r];
+          ya[u] = L.y[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+      }
+      for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
       V[t] = acc;
       pn = dd_add(pn, two_prod(acc, acc));
     }
@@ -589,7 +668,7 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     }
     __syncthreads();
   }
-  const int first0 = st->lz_prev - 3 > 4 ? st->lz_prev - 3 : 4;
+  const int first0 = st->lz_prev - 1 > 4 ? st->lz_prev - 1 : 4;
   EIG_STAMP(1);
   EIG_STAMP(2);
   const MatLds Ml{Qlds, n};

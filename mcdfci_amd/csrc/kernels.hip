@@ -192,25 +192,6 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e
 }
 
 // ------------------------------------------------------------------ diagonal / off-diagonal
-// get_diagonal (src/hamiltonian.h:257-272): sequential over occupied pairs.
-__device__ double diagonal_of(const DevHam& H, uint64_t D) {
-  double v = 0.0;
-  uint64_t r = D;
-  while (r) {
-    int i = __ffsll((long long)r) - 1;
-    r &= r - 1;
-    v += H.d1[i];
-    uint64_t r2 = r;
-    while (r2) {
-      int j = __ffsll((long long)r2) - 1;
-      r2 &= r2 - 1;
-      v += H.d2[i * H.ns + j];
-    }
-  }
-  v += H.core;
-  return v;
-}
-
 // get_off_diagonal (src/hamiltonian.h:274-328), fp64.
 __device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
   uint64_t x = D1 ^ D2;
@@ -275,13 +256,46 @@ __device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
 // ------------------------------------------------------------------ k_submatrix
 // One workgroup per row i computes H[i][j] for j >= i and stores it in both
 // triangles (the subspace solver reads columns of the symmetric matrix).
+// The diagonal keeps get_diagonal's sequential order (src/hamiltonian.h:
+// 257-272): its terms h_aa, <ab||ab> (b > a) for occupied a ascending are
+// gathered into LDS by all threads, then summed by one.
+constexpr int kMaxDiagTerms = 2 * kMaxOrb + (2 * kMaxOrb) * (2 * kMaxOrb - 1) / 2;
+
 __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const uint64_t* dets,
                                                    int k, double* Hs, int ldh) {
+  __shared__ double terms[kMaxDiagTerms];
+  __shared__ uint8_t occ[64];
+  __shared__ int nocc;
   if (st && st->error) return;
-  int i = blockIdx.x;
-  uint64_t Di = dets[i];
-  for (int j = i + threadIdx.x; j < k; j += blockDim.x) {
-    double v = (j == i) ? diagonal_of(H, Di) : off_diagonal_of(H, Di, dets[j]);
+  const int i = blockIdx.x;
+  const uint64_t Di = dets[i];
+  if (threadIdx.x == 0) {
+    int ne = 0;
+    for (uint64_t r = Di; r; r &= r - 1) occ[ne++] = (uint8_t)(__ffsll((long long)r) - 1);
+    nocc = ne;
+  }
+  __syncthreads();
+  const int ne = nocc;
+  const int nterms = ne + ne * (ne - 1) / 2;
+  for (int p = threadIdx.x; p < nterms; p += blockDim.x) {
+    // row a holds ne - a terms: h_{occ[a]}, then <occ[a] occ[b]||...> for b > a
+    int a = 0, off = 0;
+    while (p >= off + (ne - a)) {
+      off += ne - a;
+      ++a;
+    }
+    const int q = p - off;
+    terms[p] = q == 0 ? H.d1[occ[a]] : H.d2[occ[a] * H.ns + occ[a + q]];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int p = 0; p < nterms; ++p) v += terms[p];
+    v += H.core;
+    Hs[i * ldh + i] = v;
+  }
+  for (int j = i + 1 + threadIdx.x; j < k; j += blockDim.x) {
+    const double v = off_diagonal_of(H, Di, dets[j]);
     Hs[i * ldh + j] = v;
     Hs[j * ldh + i] = v;
   }

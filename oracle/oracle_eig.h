@@ -17,7 +17,7 @@
 //      every matrix-vector row sum taken as four interleaved partial sums,
 //      c = j mod 4, combined as (S0 + S1) + (S2 + S3) (dot4), every dot
 //      product in the canonical 64-lane order (red64);
-//   2. at scheduled steps (from three less than the previous solve's
+//   2. at scheduled steps (from one less than the previous solve's
 //      count, then every second step) the lowest eigenvalue theta of the Lanczos matrix
 //      T_m by Laguerre's iteration on det(T_m - theta I) from the
 //      Gershgorin lower bound, or from theta - est of the previous check
@@ -357,7 +357,7 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
 }
 
 // First convergence test of a solve, from the previous solve's count.
-static inline int lz_first_check(int prev) { return std::max(4, prev - 3); }
+static inline int lz_first_check(int prev) { return std::max(4, prev - 1); }
 
 struct EigStats {
   double max_residual = 0.0;
