@@ -47,8 +47,10 @@ struct mcdfci_ctx {
   uint8_t* d_sok = nullptr;
   double* d_d1 = nullptr;
   double* d_d2 = nullptr;
-  // wavefunction table
-  Slot* tab = nullptr;
+  // wavefunction table: nslots entries of SlotW<nw>
+  void* tab = nullptr;
+  int nw = 1;    // determinant words on the device (1: up to 31 orbitals, 2: up to 63)
+  int nio = 1;   // words per determinant in the C ABI arrays (src/run_wrapper.h:181)
   uint64_t nslots = 0;
   double max_size = 0.0;
   int table_pow = 0;
@@ -56,7 +58,7 @@ struct mcdfci_ctx {
   DevState* st = nullptr;
   DevState* st_scratch = nullptr;
   int k = 1, M = 1;
-  uint64_t* pdet = nullptr;
+  D2* pdet = nullptr;
   uint32_t* pslot = nullptr;
   double* px = nullptr;
   double* pz = nullptr;
@@ -152,9 +154,36 @@ void set_sector(DevHam& H, int na, int nb) {
   H.n_enum = H.off_dab + H.n_dab;
 }
 
-void det_sector(uint64_t det, int& na, int& nb) {
-  na = __builtin_popcountll(det & 0x5555555555555555ULL);
-  nb = __builtin_popcountll(det & 0xAAAAAAAAAAAAAAAAULL);
+void det_sector(const D2& det, int& na, int& nb) {
+  na = __builtin_popcountll(det.lo & 0x5555555555555555ULL) + __builtin_popcountll(det.hi & 0x5555555555555555ULL);
+  nb = __builtin_popcountll(det.lo & 0xAAAAAAAAAAAAAAAAULL) + __builtin_popcountll(det.hi & 0xAAAAAAAAAAAAAAAAULL);
+}
+
+D2 load_det(const uint64_t* p, int nio) { return D2{p[0], nio > 1 ? p[1] : 0}; }
+
+void store_det(uint64_t* p, const D2& d, int nio) {
+  p[0] = d.lo;
+  if (nio > 1) p[1] = d.hi;
+}
+
+bool det_less(const D2& a, const D2& b) { return a.hi != b.hi ? a.hi < b.hi : a.lo < b.lo; }
+
+size_t slot_bytes(int nw) { return nw == 1 ? sizeof(SlotW<1>) : sizeof(SlotW<2>); }
+
+// Checks a determinant against the device limits of its word count: no bits
+// beyond 2 * norb spin-orbitals, and for two words a sector whose alpha or
+// beta count is below 32 (an empty slot's word 0 holds all ones), with 1 to
+// 94 electrons (the empty determinant keys a reserved slot).
+int check_det(mcdfci_ctx* c, const D2& det, const char* what) {
+  const int ns = 2 * c->ham.norb;
+  const bool beyond = ns < 64 ? ((det.lo >> ns) != 0 || det.hi != 0) : ns < 128 && (det.hi >> (ns - 64)) != 0;
+  if (beyond) return fail(c, MCDFCI_E_INPUT, std::string(what) + " has bits beyond NORB");
+  int na, nb;
+  det_sector(det, na, nb);
+  if (c->nw == 2 && (std::min(na, nb) >= 32 || na + nb > 94 || na + nb < 1))
+    return fail(c, MCDFCI_E_UNSUPPORTED,
+                "two-word determinants need 1 to 94 electrons with fewer than 32 alpha or beta electrons");
+  return MCDFCI_OK;
 }
 
 int build_tables(mcdfci_ctx* c) {
@@ -299,15 +328,22 @@ int check_state_error(mcdfci_ctx* c, const DevState& s) {
   }
 }
 
+// The x field of slot 0 and the slot stride in doubles: k_eig writes the
+// new x of each picked coordinate through its slot index.
+double* tab_x(mcdfci_ctx* c) {
+  return c->nw == 1 ? &((SlotW<1>*)c->tab)->x : &((SlotW<2>*)c->tab)->x;
+}
+int slot_doubles(int nw) { return (int)(slot_bytes(nw) / sizeof(double)); }
+
 void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
   hipStream_t s = c->stream;
-  launch_scatter(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, k,
-                 c->nchunks);
+  launch_scatter(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
+                 k, c->nchunks);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[4], s);
-  launch_pass2(s, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words, c->M,
-               k, c->nchunks, c->stage, c->stage_cnt, c->part);
+  launch_pass2(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
+               c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[5], s);
-  launch_merge(s, c->st, c->pdet, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
+  launch_merge(s, c->nw, c->st, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
                c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration);
 }
 
@@ -318,12 +354,12 @@ void launch_iteration(mcdfci_ctx* c) {
   hipStream_t s = c->stream;
   const int k = c->k;
   if (c->timing) (void)hipEventRecord(c->ev[0], s);
-  launch_pick(s, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz, c->pk0, c->pk1,
-              c->pc0, c->pc1);
+  launch_pick(s, c->nw, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz, c->pk0,
+              c->pk1, c->pc0, c->pc1);
   if (c->timing) (void)hipEventRecord(c->ev[1], s);
-  launch_submatrix(s, c->dh, c->st, c->pdet, k, c->Hs, k);
+  launch_submatrix(s, c->nw, c->dh, c->st, c->pdet, k, c->Hs, k);
   if (c->timing) (void)hipEventRecord(c->ev[2], s);
-  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, c->tab, c->pslot, 0);
+  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot, 0);
   if (c->timing) (void)hipEventRecord(c->ev[3], s);
   launch_update_z(c, k, 1);
   if (c->timing) {
@@ -352,9 +388,9 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   if (!ham || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
   *out = nullptr;
   if (ham->norb <= 0 || !ham->h || !ham->eri) return fail(nullptr, MCDFCI_E_INPUT, "invalid Hamiltonian");
-  if (ham->norb > kMaxOrb)
+  if (ham->norb > kMaxOrb2)
     return fail(nullptr, MCDFCI_E_UNSUPPORTED,
-                "the device path supports up to " + std::to_string(kMaxOrb) + " spatial orbitals");
+                "the device path supports up to " + std::to_string(kMaxOrb2) + " spatial orbitals");
   if (opts->num_of_coordinate < 1 || opts->num_of_coordinate > kMaxK)
     return fail(nullptr, MCDFCI_E_UNSUPPORTED, "num_of_coordinate must be in [1, " + std::to_string(kMaxK) + "]");
   if (opts->estimate < 1) return fail(nullptr, MCDFCI_E_INPUT, "estimate must be positive");
@@ -368,6 +404,8 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   c->k = opts->num_of_coordinate;
   c->M = opts->estimate;
   c->device = opts->device;
+  c->nw = ham->norb <= kMaxOrb ? 1 : 2;
+  c->nio = 1 + (2 * ham->norb - 1) / 64;
   SpatialHam& S = c->ham;
   S.norb = ham->norb;
   S.nelec = ham->nelec;
@@ -390,8 +428,12 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   }
   if (!rc) rc = build_tables(c);
   if (!rc) {
-    table_size(opts->max_memory, 1, c->nslots, c->max_size, c->table_pow);
-    rc = dalloc(c, &c->tab, c->nslots);
+    // Slot count and capacity follow the reference's sizing for its word
+    // count; each slot here is 32 (one word) or 40 (two words) bytes.
+    table_size(opts->max_memory, c->nio, c->nslots, c->max_size, c->table_pow);
+    uint8_t* t = nullptr;
+    rc = dalloc(c, &t, c->nslots * slot_bytes(c->nw));
+    c->tab = t;
   }
   if (!rc) rc = dalloc(c, &c->st, 1);
   if (!rc) rc = dalloc(c, &c->st_scratch, 1);
@@ -444,7 +486,7 @@ int mcdfci_destroy(mcdfci_ctx* c) {
 
 int mcdfci_info(const mcdfci_ctx* c, int64_t* info) {
   if (!c || !info) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
-  info[0] = 1;
+  info[0] = c->nio;
   info[1] = 2 * c->ham.norb;
   info[2] = c->ham.nelec;
   info[3] = c->ham.ms2;
@@ -458,23 +500,23 @@ int mcdfci_hartree_fock(mcdfci_ctx* c, uint64_t* det) {
   if (!c || !det) return fail(c, MCDFCI_E_INPUT, "null argument");
   uint64_t d[2];
   hartree_fock(c->ham, d, nullptr);
-  det[0] = d[0];
+  store_det(det, D2{d[0], d[1]}, c->nio);
   return MCDFCI_OK;
 }
 
 int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
   if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
   HIPCHECK(c, hipSetDevice(c->device));
-  uint64_t det;
+  D2 det;
   if (ref_det) {
-    det = ref_det[0];
+    det = load_det(ref_det, c->nio);
   } else {
     uint64_t d[2];
     hartree_fock(c->ham, d, nullptr);
-    det = d[0];
+    det = D2{d[0], d[1]};
   }
-  const int ns = 2 * c->ham.norb;
-  if (ns < 64 && (det >> ns) != 0) return fail(c, MCDFCI_E_INPUT, "reference determinant has bits beyond NORB");
+  int rc = check_det(c, det, "reference determinant");
+  if (rc) return rc;
   int na, nb;
   det_sector(det, na, nb);
   if (na + nb != c->ham.nelec)
@@ -482,15 +524,15 @@ int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
                 "Reference determinant is invalid. The number of valid orbitals does not match the number of "
                 "electrons.");
   set_sector(c->dh, na, nb);
-  int rc = alloc_iteration(c);
+  rc = alloc_iteration(c);
   if (rc) return rc;
-  HIPCHECK(c, hipMemsetAsync(c->tab, 0, c->nslots * sizeof(Slot), c->stream));
+  launch_table_init(c->stream, c->nw, c->tab, c->nslots);
   DevState s{};
   s.z_thr = c->opts.z_threshold;
   s.max_size = c->max_size;
   HIPCHECK(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
-  launch_init_ref(c->stream, c->st, det, c->tab, c->nslots - 1, c->pdet, c->pslot, c->px, c->pz, c->dx);
-  launch_submatrix(c->stream, c->dh, c->st, c->pdet, 1, c->Hs, c->k);
+  launch_init_ref(c->stream, c->nw, c->st, det, c->tab, c->nslots - 1, c->pdet, c->pslot, c->px, c->pz, c->dx);
+  launch_submatrix(c->stream, c->nw, c->dh, c->st, c->pdet, 1, c->Hs, c->k);
   launch_update_z(c, 1, 0);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -586,47 +628,50 @@ int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* va
   if (!c || !det || !n) return fail(c, MCDFCI_E_INPUT, "null argument");
   HIPCHECK(c, hipSetDevice(c->device));
   DevHam H = c->dh;
+  const D2 d0 = load_det(det, c->nio);
+  int rc = check_det(c, d0, "determinant");
+  if (rc) return rc;
   int na, nb;
-  det_sector(det[0], na, nb);
+  det_sector(d0, na, nb);
   set_sector(H, na, nb);
-  uint64_t* dd_ = nullptr;
+  D2* dd_ = nullptr;
   double* dh_ = nullptr;
   double* dv = nullptr;
   unsigned long long* cnt = nullptr;
-  int rc;
   if ((rc = dalloc(c, &dd_, 1))) return rc;
   if ((rc = dalloc(c, &dh_, 1))) return rc;
   const long long ncap = H.n_enum;
   uint64_t* od = nullptr;
-  if ((rc = dalloc(c, &od, ncap))) return rc;
+  if ((rc = dalloc(c, &od, 2 * ncap))) return rc;
   if ((rc = dalloc(c, &dv, ncap))) return rc;
   if ((rc = dalloc(c, &cnt, 1))) return rc;
-  HIPCHECK(c, hipMemcpy(dd_, det, 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(dd_, &d0, sizeof(D2), hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
-  launch_submatrix(c->stream, H, nullptr, dd_, 1, dh_, 1);
+  launch_submatrix(c->stream, c->nw, H, nullptr, dd_, 1, dh_, 1);
   double diag;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(&diag, dh_, 8, hipMemcpyDeviceToHost));
-  launch_column_dump(c->stream, H, det[0], diag, od, dv, cnt, ncap);
+  launch_column_dump(c->stream, c->nw, H, d0, diag, od, dv, cnt, ncap);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   unsigned long long m;
   HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
-  std::vector<uint64_t> hd(m);
+  std::vector<D2> hd(m);
   std::vector<double> hv(m);
-  HIPCHECK(c, hipMemcpy(hd.data(), od, m * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hd.data(), od, m * sizeof(D2), hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(hv.data(), dv, m * 8, hipMemcpyDeviceToHost));
   for (void* p : {(void*)dd_, (void*)dh_, (void*)od, (void*)dv, (void*)cnt}) dfree(c, p);
   std::vector<size_t> order(m);
   for (size_t i = 0; i < m; ++i) order[i] = i;
+  auto same = [&](const D2& a) { return a.lo == d0.lo && a.hi == d0.hi; };
   std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    bool da = hd[a] == det[0], db = hd[b] == det[0];
+    bool da = same(hd[a]), db = same(hd[b]);
     if (da != db) return da;
-    return hd[a] < hd[b];
+    return det_less(hd[a], hd[b]);
   });
   *n = (int64_t)m;
   for (size_t i = 0; i < m && (int64_t)i < cap; ++i) {
-    dets[i] = hd[order[i]];
+    store_det(dets + i * c->nio, hd[order[i]], c->nio);
     vals[i] = hv[order[i]];
   }
   return MCDFCI_OK;
@@ -661,7 +706,9 @@ int mcdfci_diag_hook_profile(mcdfci_ctx* c, int64_t* out16) {
 int mcdfci_diag_last_step(mcdfci_ctx* c, uint64_t* dets, double* dx) {
   if (!c) return MCDFCI_E_INPUT;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  HIPCHECK(c, hipMemcpy(dets, c->pdet, 8 * (size_t)c->k, hipMemcpyDeviceToHost));
+  std::vector<D2> hd(c->k);
+  HIPCHECK(c, hipMemcpy(hd.data(), c->pdet, sizeof(D2) * (size_t)c->k, hipMemcpyDeviceToHost));
+  for (int i = 0; i < c->k; ++i) store_det(dets + (size_t)i * c->nio, hd[i], c->nio);
   HIPCHECK(c, hipMemcpy(dx, c->dx, 8 * (size_t)c->k, hipMemcpyDeviceToHost));
   return MCDFCI_OK;
 }
@@ -669,14 +716,19 @@ int mcdfci_diag_last_step(mcdfci_ctx* c, uint64_t* dets, double* dx) {
 int mcdfci_submatrix(mcdfci_ctx* c, int k, const uint64_t* dets, double* Hout) {
   if (!c || !dets || !Hout || k < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   HIPCHECK(c, hipSetDevice(c->device));
-  uint64_t* d = nullptr;
-  double* h = nullptr;
+  std::vector<D2> hd(k);
   int rc;
+  for (int i = 0; i < k; ++i) {
+    hd[i] = load_det(dets + (size_t)i * c->nio, c->nio);
+    if ((rc = check_det(c, hd[i], "determinant"))) return rc;
+  }
+  D2* d = nullptr;
+  double* h = nullptr;
   if ((rc = dalloc(c, &d, k))) return rc;
   if ((rc = dalloc(c, &h, (size_t)k * k))) return rc;
-  HIPCHECK(c, hipMemcpy(d, dets, 8 * (size_t)k, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(d, hd.data(), sizeof(D2) * (size_t)k, hipMemcpyHostToDevice));
   HIPCHECK(c, hipMemsetAsync(h, 0, 8 * (size_t)k * k, c->stream));
-  launch_submatrix(c->stream, c->dh, nullptr, d, k, h, k);
+  launch_submatrix(c->stream, c->nw, c->dh, nullptr, d, k, h, k);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(Hout, h, 8 * (size_t)k * k, hipMemcpyDeviceToHost));
@@ -711,7 +763,7 @@ int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, 
   st.xz_hi = (int64_t)(xz >> 64);
   st.xz_lo = (uint64_t)xz;
   HIPCHECK(c, hipMemcpy(c->st_scratch, &st, sizeof(st), hipMemcpyHostToDevice));
-  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, nullptr, nullptr, 1);
+  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, nullptr, 0, nullptr, 1);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(&st, c->st_scratch, sizeof(st), hipMemcpyDeviceToHost));
@@ -737,31 +789,31 @@ int mcdfci_table_dump(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
   int64_t* oz = nullptr;
   unsigned long long* cnt = nullptr;
   int rc;
-  if ((rc = dalloc(c, &od, ub))) return rc;
+  if ((rc = dalloc(c, &od, 2 * ub))) return rc;
   if ((rc = dalloc(c, &ox, ub))) return rc;
   if ((rc = dalloc(c, &oz, 2 * ub))) return rc;
   if ((rc = dalloc(c, &cnt, 1))) return rc;
   HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
-  launch_table_compact(c->stream, c->tab, c->nslots, od, ox, oz, cnt, ub);
+  launch_table_compact(c->stream, c->nw, c->tab, c->nslots, od, ox, oz, cnt, ub);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   unsigned long long m;
   HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
   if ((long long)m > ub) m = ub;
-  std::vector<uint64_t> hd(m);
+  std::vector<D2> hd(m);
   std::vector<double> hx(m);
   std::vector<int64_t> hz(2 * m);
-  HIPCHECK(c, hipMemcpy(hd.data(), od, m * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(hd.data(), od, m * sizeof(D2), hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(hx.data(), ox, m * 8, hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(hz.data(), oz, m * 16, hipMemcpyDeviceToHost));
   for (void* p : {(void*)od, (void*)ox, (void*)oz, (void*)cnt}) dfree(c, p);
   std::vector<size_t> order(m);
   for (size_t i = 0; i < m; ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return hd[a] < hd[b]; });
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return det_less(hd[a], hd[b]); });
   *n = (int64_t)m;
   for (size_t i = 0; i < m && (int64_t)i < cap; ++i) {
     size_t j = order[i];
-    if (dets) dets[i] = hd[j];
+    if (dets) store_det(dets + i * c->nio, hd[j], c->nio);
     if (x) x[i] = hx[j];
     if (z) z[i] = fix_to_d(fix_join(hz[2 * j], hz[2 * j + 1]));
   }
@@ -782,12 +834,17 @@ int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
   for (int t = 0; t < s.k_prev; ++t)
     for (int i = 0; i < cnt[t]; ++i) {
       const CandEnt& e = ce[(size_t)t * c->M + i];
-      Slot sl;
-      HIPCHECK(c, hipMemcpy(&sl, c->tab + e.slot, sizeof(Slot), hipMemcpyDeviceToHost));
+      // x, za, zb are the last three fields of either slot layout
+      double xz[3];
+      const size_t sb = slot_bytes(c->nw);
+      HIPCHECK(c, hipMemcpy(xz, (const uint8_t*)c->tab + (size_t)e.slot * sb + sb - 24, 24, hipMemcpyDeviceToHost));
+      int64_t za, zb;
+      std::memcpy(&za, &xz[1], 8);
+      std::memcpy(&zb, &xz[2], 8);
       if (m < cap) {
-        if (dets) dets[m] = e.det;
-        if (x) x[m] = sl.x;
-        if (z) z[m] = fix_to_d(fix_join(sl.za, sl.zb));
+        if (dets) store_det(dets + m * c->nio, D2{e.det, e.det1}, c->nio);
+        if (x) x[m] = xz[0];
+        if (z) z[m] = fix_to_d(fix_join(za, zb));
       }
       ++m;
     }
@@ -810,7 +867,7 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
   const int M = c->ham.norb;
   const size_t nh = (size_t)M * M, ne4 = nh * nh;
   long long ub = (long long)std::min<uint64_t>(c->nslots, (uint64_t)(s.x_size + c->k + 1024));
-  uint64_t* od = nullptr;
+  D2* od = nullptr;
   double* ox = nullptr;
   unsigned long long* cnt = nullptr;
   int64_t *acc_h = nullptr, *acc_e = nullptr, *acc_q = nullptr;
@@ -825,7 +882,7 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
       return rc;
     }
     HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
-    launch_compact_x(c->stream, c->tab, c->nslots, od, ox, cnt, ub);
+    launch_compact_x(c->stream, c->nw, c->tab, c->nslots, od, ox, cnt, ub);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
@@ -856,7 +913,7 @@ int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, doub
   const double scale = 1.0 / std::sqrt(xx);
   HIPCHECK(c, hipMemsetAsync(acc_q, 0, 16, c->stream));
   launch_sum_sq(c->stream, ox, (long long)m, scale, acc_q);
-  launch_energy_grad(c->stream, c->dh, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc_h, acc_e);
+  launch_energy_grad(c->stream, c->nw, c->dh, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc_h, acc_e);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   std::vector<int64_t> hh(2 * nh), he(2 * ne4);

@@ -1,20 +1,24 @@
 // Device-side data layout shared by the kernels and the host engine.
 //
 // HBM layout (one context):
-//   table      : 2^p slots of {key, x, zA, zB} (32 B, 4 slots per 128 B line),
-//                open addressing with bucketed linear probing; key 0 = empty.
-//                z = (zA * 2^32 + zB) * 2^-64 exactly; key bit 63 marks a
-//                slot inserted during the current iteration (cleared in pass 2).
+//   table      : 2^p slots, open addressing with bucketed linear probing.
+//                W = 1 (up to 31 spatial orbitals): {key, x, zA, zB} (32 B,
+//                4 slots per 128 B line), key 0 = empty, bit 63 marks a slot
+//                inserted during the current iteration (cleared in pass 2).
+//                W = 2 (up to 63): {k0, k1, x, zA, zB} (40 B), claim, mark
+//                and empty conventions of det.h.  z = (zA * 2^32 + zB) * 2^-64
+//                exactly.
 //   integrals  : dense spatial (pq|rs) [M^4] and h [M^2] in fp64 and fp32,
 //                single-excitation sums S_same/S_opp [M^3], diagonal terms
 //                per spin-orbital; all read through L2/MALL.
-//   iteration  : picked dets/slots/x/z, dx, H_sub (k x k), per-column
-//                visibility bitmaps, per-chunk top-M staging lists and
-//                exact partial sums, the candidate lists.
+//   iteration  : picked dets (two words each)/slots/x/z, dx, H_sub (k x k),
+//                per-column visibility bitmaps, per-chunk top-M staging lists
+//                and exact partial sums, the candidate lists.
 #pragma once
 
 #include <cstdint>
 
+#include "det.h"
 #include "num.h"
 
 namespace mc {
@@ -24,18 +28,26 @@ constexpr int kEPT = 4;                // column elements per thread
 constexpr int kChunk = kWG * kEPT;     // column elements per workgroup
 constexpr int kSortCap = 4096;         // entries sorted by one workgroup
 constexpr int kMaxK = 256;             // coordinates per iteration
-constexpr int kMaxOrb = 31;            // spatial orbitals (one 64-bit word, bit 63 spare)
-// Bit 63 of a table key marks a determinant inserted during the running
-// iteration (set by the inserting CAS, cleared by the pass that follows).
-constexpr uint64_t kMark = 0x8000000000000000ULL;
-constexpr uint64_t kKeyMask = ~kMark;
+constexpr int kMaxOrb = 31;            // spatial orbitals of one-word determinants (bit 63 spare)
+constexpr int kMaxOrb2 = 63;           // spatial orbitals of two-word determinants (bits 126, 127 spare)
 
-struct Slot {
+template <int W>
+struct SlotW;
+template <>
+struct SlotW<1> {
   uint64_t key;
   double x;
   int64_t za;
   int64_t zb;
 };
+template <>
+struct SlotW<2> {
+  uint64_t k0, k1;
+  double x;
+  int64_t za;
+  int64_t zb;
+};
+using Slot = SlotW<1>;
 
 // Integral tables and column enumeration layout for one (n_alpha, n_beta)
 // sector.  Column element index e:
@@ -89,13 +101,15 @@ enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ER
 
 struct StageEnt {
   uint64_t val;   // bits of |x*xss+z| (non-negative double)
-  uint64_t det;
+  uint64_t det;   // determinant word 0
+  uint64_t det1;  // determinant word 1 (0 for one-word determinants)
   uint32_t slot;  // table slot, or 0xffffffff when absent
   uint32_t pad;
 };
 
 struct CandEnt {
   uint64_t det;
+  uint64_t det1;
   uint32_t slot;
   uint32_t pad;
 };
@@ -104,6 +118,7 @@ struct CandEnt {
 struct PickEnt {
   uint64_t key;
   uint64_t det;
+  uint64_t det1;
   uint32_t slot;
   uint32_t pad;
 };

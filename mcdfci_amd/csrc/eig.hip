@@ -605,11 +605,12 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
 }
 
 // mode 0: solver iteration (reads picked x/z, writes dx, updates the state
-// and the table: update_x).  mode 1: test hook (state holds the inputs; no
+// and the table: update_x through tabx, the x of slot 0, and slot_stride,
+// the slot size in doubles).  mode 1: test hook (state holds the inputs; no
 // table access).  Hs holds both triangles of H_sub.
 __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const double* Hs,
-                                             const double* px, const double* pz, double* dx, Slot* tab,
-                                             const uint32_t* pslot, int mode) {
+                                             const double* px, const double* pz, double* dx, double* tabx,
+                                             int slot_stride, const uint32_t* pslot, int mode) {
   __shared__ EigLds L;
   __shared__ double Qlds[kQBudget];  // A (n <= kEigLdsN), then the Lanczos basis
   if (st->error) return;
@@ -775,10 +776,10 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   long long nx = 0;
   i128 inc = 0;
   for (int t = tid; t < k; t += bs) {
-    Slot* s = &tab[pslot[t]];
-    const double xo = s->x;
+    double* xs = tabx + (size_t)pslot[t] * slot_stride;
+    const double xo = *xs;
     if (!(__builtin_fabs(xo) != 0.0)) ++nx;
-    s->x = xo + dx[t];
+    *xs = xo + dx[t];
     inc += to_fix(L.z[t] * dx[t]);
   }
   for (int off = 32; off >= 1; off >>= 1) {
@@ -810,9 +811,10 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
 }
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
-                const double* pz, double* dx, Slot* tab, const uint32_t* pslot, int mode) {
+                const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
   // The four-way partial sums assume exactly four waves.
-  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tab, pslot, mode);
+  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride, pslot,
+                     mode);
 }
 
 }  // namespace mc

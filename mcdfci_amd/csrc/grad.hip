@@ -51,15 +51,24 @@ __device__ __forceinline__ void acc_add(int64_t* acc, size_t idx, int64_t a, int
 }
 
 // Determinants with x != 0 (any order: the sums below are exact).
-__global__ void k_compact_x(const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox,
-                            unsigned long long* count, long long cap) {
+template <int W>
+__global__ void k_compact_x(const SlotW<W>* tab, uint64_t nslots, D2* odet, double* ox, unsigned long long* count,
+                            long long cap) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  const Slot s = tab[i];
-  if (s.key == 0 || !(__builtin_fabs(s.x) != 0.0)) return;
+  if (i >= nslots || i == 0xffffffffULL) return;
+  const SlotW<W> s = tab[i];
+  D2 d;
+  if constexpr (W == 1) {
+    if (s.key == 0) return;
+    d = D2{s.key & kKeyMask, 0};
+  } else {
+    if (s.k1 == 0) return;
+    d = D2{s.k0, s.k1 & kHiMask};
+  }
+  if (!(__builtin_fabs(s.x) != 0.0)) return;
   const unsigned long long pos = atomicAdd(count, 1ULL);
   if ((long long)pos < cap) {
-    odet[pos] = s.key & kKeyMask;
+    odet[pos] = d;
     ox[pos] = s.x;
   }
 }
@@ -83,15 +92,16 @@ __global__ void k_sum_sq(const double* x, long long n, double scale, int64_t* ac
 
 // One workgroup per (determinant, column chunk); the table supplies x of
 // the connected determinant.
-__global__ __launch_bounds__(kWG) void k_energy_grad(DevHam H, const Slot* tab, uint64_t mask,
-                                                     const uint64_t* dets, const double* xs, long long n,
-                                                     double scale, int64_t* acc_h, int64_t* acc_e) {
-  __shared__ ColCtx C;
+template <int W>
+__global__ __launch_bounds__(kWG) void k_energy_grad(DevHam H, const SlotW<W>* tab, uint64_t mask, const D2* dets,
+                                                     const double* xs, long long n, double scale, int64_t* acc_h,
+                                                     int64_t* acc_e) {
+  __shared__ ColCtxW<W> C;
   const int M = H.M;
   const int chunk = blockIdx.x;
   for (long long di = blockIdx.y; di < n; di += gridDim.y) {
-    setup_ctx(C, H, dets[di]);
-    const uint64_t D = C.det;
+    setup_ctx<W>(C, H, dets[di]);
+    const D2 D = C.det;
     const double cD = xs[di] * scale;
     if (chunk == 0) {
       // Diagonal: h_II for occupied I, (II|JJ) - delta_spin (IJ|JI) for I < J.
@@ -191,15 +201,16 @@ __global__ __launch_bounds__(kWG) void k_energy_grad(DevHam H, const Slot* tab, 
           b = 2 * rr;
         }
       }
-      uint64_t d1 = D ^ (1ULL << i);
-      int sg = parity1(d1, i, a);
-      d1 |= 1ULL << a;
+      D2 d1 = D;
+      dflip<W>(d1, i);
+      int sg = dparity<W>(d1, i, a);
+      dflip<W>(d1, a);
       if (kind != 0) {
-        d1 ^= 1ULL << j;
-        sg ^= parity1(d1, j, b);
-        d1 |= 1ULL << b;
+        dflip<W>(d1, j);
+        sg ^= dparity<W>(d1, j, b);
+        dflip<W>(d1, b);
       }
-      const long long si = find_slot(tab, mask, d1);
+      const long long si = find_slot<W>(tab, mask, d1);
       if (si < 0) continue;
       const double xo = tab[si].x;
       if (!(__builtin_fabs(xo) != 0.0)) continue;
@@ -226,23 +237,32 @@ __global__ __launch_bounds__(kWG) void k_energy_grad(DevHam H, const Slot* tab, 
   }
 }
 
-void launch_compact_x(hipStream_t s, const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox,
+void launch_compact_x(hipStream_t s, int nw, const void* tab, uint64_t nslots, D2* odet, double* ox,
                       unsigned long long* count, long long cap) {
   const uint64_t blocks = (nslots + 255) / 256;
-  hipLaunchKernelGGL(k_compact_x, dim3((unsigned)blocks), dim3(256), 0, s, tab, nslots, odet, ox, count, cap);
+  if (nw == 1)
+    hipLaunchKernelGGL(k_compact_x<1>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<1>*)tab, nslots, odet,
+                       ox, count, cap);
+  else
+    hipLaunchKernelGGL(k_compact_x<2>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<2>*)tab, nslots, odet,
+                       ox, count, cap);
 }
 
 void launch_sum_sq(hipStream_t s, const double* x, long long n, double scale, int64_t* acc) {
   hipLaunchKernelGGL(k_sum_sq, dim3(256), dim3(256), 0, s, x, n, scale, acc);
 }
 
-void launch_energy_grad(hipStream_t s, const DevHam& H, const Slot* tab, uint64_t mask, const uint64_t* dets,
+void launch_energy_grad(hipStream_t s, int nw, const DevHam& H, const void* tab, uint64_t mask, const D2* dets,
                         const double* xs, long long n, double scale, int64_t* acc_h, int64_t* acc_e) {
   if (n <= 0) return;
   const int nchunks = (H.n_enum + kChunk - 1) / kChunk;
   const long long gy = n < 65535 ? n : 65535;
-  hipLaunchKernelGGL(k_energy_grad, dim3(nchunks, (unsigned)gy), dim3(kWG), 0, s, H, tab, mask, dets, xs, n,
-                     scale, acc_h, acc_e);
+  if (nw == 1)
+    hipLaunchKernelGGL(k_energy_grad<1>, dim3(nchunks, (unsigned)gy), dim3(kWG), 0, s, H, (const SlotW<1>*)tab,
+                       mask, dets, xs, n, scale, acc_h, acc_e);
+  else
+    hipLaunchKernelGGL(k_energy_grad<2>, dim3(nchunks, (unsigned)gy), dim3(kWG), 0, s, H, (const SlotW<2>*)tab,
+                       mask, dets, xs, n, scale, acc_h, acc_e);
 }
 
 }  // namespace mc

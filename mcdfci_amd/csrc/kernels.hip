@@ -1,4 +1,5 @@
-// CDFCI device kernels for gfx950 (one 64-bit word per determinant).
+// CDFCI device kernels for gfx950 (determinants of one or two 64-bit words,
+// template parameter W).
 //
 // One iteration of the reference loop (src/solver.h:701-759) is six launches:
 //   k_pick       gcd_grad pick: refetch candidates, sort, take k distinct
@@ -35,7 +36,8 @@ __device__ __forceinline__ uint64_t rload(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void zadd(Slot* s, i128 v) {
+template <class S>
+__device__ __forceinline__ void zadd(S* s, i128 v) {
   int64_t a, b;
   fix_split(v, a, b);
   if (a) __hip_atomic_fetch_add(&s->za, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -101,10 +103,12 @@ __device__ __forceinline__ double same_spin_val(const DevHam& H, int p, int r, i
 
 // Element e >= 1 of the column of C.det.  Returns false when the element is
 // not part of the column (threshold filter of src/hamiltonian.h:138-161,206).
-template <bool MIXED>
-__device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e, uint64_t& out,
-                                         double& V) {
-  const uint64_t D = C.det;
+// Signs: i -> a applied first, then j -> b on the intermediate determinant,
+// each the parity of the occupied spin-orbitals between the two positions
+// (src/hamiltonian.h:349-356,395-403).
+template <bool MIXED, int W>
+__device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtxW<W>& C, int e, D2& out, double& V) {
+  const D2 D = C.det;
   int i, j, a, b;
   double v;
   if (e < H.off_daa) {
@@ -126,9 +130,11 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e
     v = single_value<MIXED>(H, C.occ, C.ne, p, r, spin);
     i = 2 * p + spin;
     a = 2 * r + spin;
-    uint64_t d1 = D ^ (1ULL << i);
-    if (parity1(d1, i, a)) v = -v;
-    out = d1 | (1ULL << a);
+    D2 d1 = D;
+    dflip<W>(d1, i);
+    if (dparity<W>(d1, i, a)) v = -v;
+    dflip<W>(d1, a);
+    out = d1;
     V = v;
     return true;
   }
@@ -179,12 +185,13 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e
       b = 2 * r;
     }
   }
-  uint64_t d1 = D ^ (1ULL << i);
-  int s1 = parity1(d1, i, a);
-  d1 |= 1ULL << a;
-  d1 ^= 1ULL << j;
-  int s2 = parity1(d1, j, b);
-  d1 |= 1ULL << b;
+  D2 d1 = D;
+  dflip<W>(d1, i);
+  int s1 = dparity<W>(d1, i, a);
+  dflip<W>(d1, a);
+  dflip<W>(d1, j);
+  int s2 = dparity<W>(d1, j, b);
+  dflip<W>(d1, b);
   if (s1 ^ s2) v = -v;
   out = d1;
   V = v;
@@ -193,36 +200,37 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtx& C, int e
 
 // ------------------------------------------------------------------ diagonal / off-diagonal
 // get_off_diagonal (src/hamiltonian.h:274-328), fp64.
-__device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
-  uint64_t x = D1 ^ D2;
-  int pc = __popcll(x);
+template <int W>
+__device__ double off_diagonal_of(const DevHam& H, const D2& D1, const D2& D2b) {
+  const D2 x = dxor(D1, D2b);
+  const int pc = dpopc<W>(x);
   if (pc == 2) {
-    int i = __ffsll((long long)(x & D1)) - 1;
-    int a = __ffsll((long long)(x & D2)) - 1;
+    int i = dffs<W>(dand(x, D1));
+    int a = dffs<W>(dand(x, D2b));
     int spin = i & 1;
     int p = i >> 1, r = a >> 1;
     if (!H.single_ok[p * H.M + r]) return 0.0;
     const size_t base = ((size_t)p * H.M + r) * H.M;
     double v = H.h1[p * H.M + r];
-    uint64_t o = D1;
-    while (o) {
-      int k = __ffsll((long long)o) - 1;
-      o &= o - 1;
+    D2 o = D1;
+    for (int k = dffs<W>(o); k >= 0; k = dffs<W>(o)) {
+      dpop_low<W>(o);
       double t = ((k & 1) == spin) ? H.s_same[base + (k >> 1)] : H.s_opp[base + (k >> 1)];
       v = v + t;
     }
-    uint64_t d1 = D1 ^ (1ULL << i);
-    if (parity1(d1, i, a)) v = -v;
+    D2 d1 = D1;
+    dflip<W>(d1, i);
+    if (dparity<W>(d1, i, a)) v = -v;
     return v;
   }
   if (pc == 4) {
-    uint64_t from = x & D1, to = x & D2;
-    int i = __ffsll((long long)from) - 1;
-    from &= from - 1;
-    int j = __ffsll((long long)from) - 1;
-    int a0 = __ffsll((long long)to) - 1;
-    to &= to - 1;
-    int a1 = __ffsll((long long)to) - 1;
+    D2 from = dand(x, D1), to = dand(x, D2b);
+    int i = dffs<W>(from);
+    dpop_low<W>(from);
+    int j = dffs<W>(from);
+    int a0 = dffs<W>(to);
+    dpop_low<W>(to);
+    int a1 = dffs<W>(to);
     int a, b;
     double v;
     if ((i & 1) == (j & 1)) {
@@ -242,11 +250,12 @@ __device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
     }
     if ((i & 1) != (a & 1) || (j & 1) != (b & 1)) return 0.0;
     if (!(__builtin_fabs(v) > H.thr)) return 0.0;
-    uint64_t d1 = D1 ^ (1ULL << i);
-    int s1 = parity1(d1, i, a);
-    d1 |= 1ULL << a;
-    d1 ^= 1ULL << j;
-    int s2 = parity1(d1, j, b);
+    D2 d1 = D1;
+    dflip<W>(d1, i);
+    int s1 = dparity<W>(d1, i, a);
+    dflip<W>(d1, a);
+    dflip<W>(d1, j);
+    int s2 = dparity<W>(d1, j, b);
     if (s1 ^ s2) v = -v;
     return v;
   }
@@ -258,20 +267,25 @@ __device__ double off_diagonal_of(const DevHam& H, uint64_t D1, uint64_t D2) {
 // triangles (the subspace solver reads columns of the symmetric matrix).
 // The diagonal keeps get_diagonal's sequential order (src/hamiltonian.h:
 // 257-272): its terms h_aa, <ab||ab> (b > a) for occupied a ascending are
-// gathered into LDS by all threads, then summed by one.
-constexpr int kMaxDiagTerms = 2 * kMaxOrb + (2 * kMaxOrb) * (2 * kMaxOrb - 1) / 2;
-
-__global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const uint64_t* dets,
-                                                   int k, double* Hs, int ldh) {
-  __shared__ double terms[kMaxDiagTerms];
-  __shared__ uint8_t occ[64];
+// gathered into LDS by all threads, then summed by one.  Two-word sectors
+// hold at most 94 electrons (63 of one spin, 31 of the other).
+template <int W>
+__global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const D2* dets, int k,
+                                                   double* Hs, int ldh) {
+  constexpr int kNe = W == 1 ? 2 * kMaxOrb : 94;
+  __shared__ double terms[kNe + kNe * (kNe - 1) / 2];
+  __shared__ uint8_t occ[kNe];
   __shared__ int nocc;
   if (st && st->error) return;
   const int i = blockIdx.x;
-  const uint64_t Di = dets[i];
+  const D2 Di = dets[i];
   if (threadIdx.x == 0) {
     int ne = 0;
-    for (uint64_t r = Di; r; r &= r - 1) occ[ne++] = (uint8_t)(__ffsll((long long)r) - 1);
+    D2 r = Di;
+    for (int s = dffs<W>(r); s >= 0; s = dffs<W>(r)) {
+      dpop_low<W>(r);
+      occ[ne++] = (uint8_t)s;
+    }
     nocc = ne;
   }
   __syncthreads();
@@ -295,7 +309,7 @@ __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st,
     Hs[i * ldh + i] = v;
   }
   for (int j = i + 1 + threadIdx.x; j < k; j += blockDim.x) {
-    const double v = off_diagonal_of(H, Di, dets[j]);
+    const double v = off_diagonal_of<W>(H, Di, dets[j]);
     Hs[i * ldh + j] = v;
     Hs[j * ldh + i] = v;
   }
@@ -347,16 +361,17 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int& total) {
 }
 
 // ------------------------------------------------------------------ k_scatter
-template <bool MIXED>
-__global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const uint64_t* pdet,
-                                                 const double* dx, const double* Hs, int ldh,
-                                                 Slot* tab, uint64_t mask, uint64_t* vis,
-                                                 int vis_words) {
-  __shared__ ColCtx C;
+// Column generation (src/hamiltonian.h:336-418) fused with the scatter-add
+// z += H dx and the insert gate (src/wavefunction.h:499-528).
+template <bool MIXED, int W>
+__global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D2* pdet, const double* dx,
+                                                 const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
+                                                 uint64_t* vis, int vis_words) {
+  __shared__ ColCtxW<W> C;
   if (st->error) return;
   const int t = blockIdx.y;
   const int chunk = blockIdx.x;
-  setup_ctx(C, H, pdet[t]);
+  setup_ctx<W>(C, H, pdet[t]);
   const double dxt = dx[t];
   const dd sc{st->sc_hi, st->sc_lo};
   const double zthr = st->z_thr;
@@ -368,7 +383,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
     const int e = chunk * kChunk + r * kWG + threadIdx.x;
     bool visible = false;
     if (e < H.n_enum) {
-      uint64_t D = 0;
+      D2 D{0, 0};
       double V = 0.0;
       bool valid;
       if (e == 0) {
@@ -376,43 +391,90 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
         V = diag;
         valid = true;
       } else {
-        valid = gen_elem<MIXED>(H, C, e, D, V);
+        valid = gen_elem<MIXED, W>(H, C, e, D, V);
       }
       if (valid) {
         ++n_valid;
         const double dz = V * dxt;
         const bool gate = dd_abs_gt(dd_mul_d(sc, dz), zthr);
-        Slot* hit = nullptr;
+        SlotW<W>* hit = nullptr;
         bool mine = false, marked = false;
-        uint64_t b = mix64(D) & mask & ~3ULL;
+        uint64_t b = dhash<W>(D) & mask & ~3ULL;
         bool done = false;
-        for (uint64_t n = 0; n <= mask && !done; n += 4) {
-          for (int jj = 0; jj < 4 && !done; ++jj) {
-            Slot* s = &tab[b + jj];
-            uint64_t k = rload(&s->key);
-            if ((k & kKeyMask) == D) {
-              hit = s;
-              marked = (k & kMark) != 0;
-              done = true;
-            } else if (k == 0) {
+        if constexpr (W == 1) {
+          for (uint64_t n = 0; n <= mask && !done; n += 4) {
+            for (int jj = 0; jj < 4 && !done; ++jj) {
+              SlotW<W>* s = &tab[b + jj];
+              uint64_t k = rload(&s->key);
+              if ((k & kKeyMask) == D.lo) {
+                hit = s;
+                marked = (k & kMark) != 0;
+                done = true;
+              } else if (k == 0) {
+                if (!gate) {
+                  done = true;
+                } else {
+                  uint64_t expect = 0;
+                  bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D.lo | kMark, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  if (ok) {
+                    hit = s;
+                    mine = true;
+                    done = true;
+                  } else if ((expect & kKeyMask) == D.lo) {
+                    hit = s;
+                    done = true;
+                  }
+                }
+              }
+            }
+            b = (b + 4) & mask;
+          }
+        } else {
+          // One slot per trip of a flat loop.  A slot claimed (word 1) but
+          // not yet published (word 0) is read again on the next trip
+          // rather than waited on inside the trip: the claiming lane may sit
+          // in the same wavefront, and its publishing store then completes
+          // within the current trip.
+          uint64_t n = 0;
+          int jj = 0, spins = 0;
+          while (!done) {
+            SlotW<W>* s = &tab[b + jj];
+            uint64_t k1 = rload(&s->k1);
+            if (k1 == 0) {
               if (!gate) {
                 done = true;
               } else {
                 uint64_t expect = 0;
-                bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D | kMark, __ATOMIC_RELAXED,
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (ok) {
+                if (__hip_atomic_compare_exchange_strong(&s->k1, &expect, D.hi | kOcc | kMark, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                  __hip_atomic_store(&s->k0, D.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                   hit = s;
                   mine = true;
                   done = true;
-                } else if ((expect & kKeyMask) == D) {
-                  hit = s;
-                  done = true;
                 }
+                k1 = expect;
               }
             }
+            if (!done && (k1 & kHiMask) == D.hi) {
+              const uint64_t k0 = rload(&s->k0);
+              if (k0 == kSent) {
+                if (++spins > (1 << 22)) break;
+                continue;
+              }
+              if (k0 == D.lo) {
+                hit = s;
+                marked = (k1 & kMark) != 0;
+                done = true;
+              }
+            }
+            if (!done && ++jj == 4) {
+              jj = 0;
+              b = (b + 4) & mask;
+              n += 4;
+              if (n > mask) break;
+            }
           }
-          b = (b + 4) & mask;
         }
         if (!done) fail = true;
         if (hit) {
@@ -446,37 +508,38 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const u
 // Values after the scatter: per-element |x*xss+z| (0 when the element does
 // not see its determinant), exact partial sums of H*x and dz*x, and the
 // chunk's top-M list sorted by (value, determinant) descending.
-template <bool MIXED>
-__global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uint64_t* pdet,
-                                               const double* dx, const double* Hs, int ldh, Slot* tab,
-                                               uint64_t mask, const uint64_t* vis, int vis_words,
-                                               int M, int nchunks, StageEnt* stage, int* stage_cnt,
-                                               int64_t* part) {
-  __shared__ ColCtx C;
+template <bool MIXED, int W>
+__global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
+                                               const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
+                                               const uint64_t* vis, int vis_words, int M, int nchunks,
+                                               StageEnt* stage, int* stage_cnt, int64_t* part) {
+  __shared__ ColCtxW<W> C;
   __shared__ uint64_t ka[kChunk];
   __shared__ uint64_t kb[kChunk];
+  __shared__ uint64_t kc[W == 2 ? kChunk : 1];
   __shared__ uint32_t pl[kChunk];
   __shared__ i128 red[kWG / 64][2];
   if (st->error) return;
   const int t = blockIdx.y;
   const int chunk = blockIdx.x;
-  setup_ctx(C, H, pdet[t]);
+  setup_ctx<W>(C, H, pdet[t]);
   const double dxt = dx[t];
   const dd s = load_xss(st);
   const double diag = Hs[t * ldh + t];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   i128 nz = 0, xt = 0;
-  uint64_t A[kEPT], B[kEPT];
+  uint64_t A[kEPT], B[kEPT], B1[kEPT];
   uint32_t P[kEPT];
   int nvalid = 0;
 #pragma unroll
   for (int r = 0; r < kEPT; ++r) {
     const int li = r * kWG + threadIdx.x;
     const int e = chunk * kChunk + li;
-    uint64_t keyv = 0, keyd = 0;
+    uint64_t keyv = 0;
+    D2 keyd{0, 0};
     uint32_t slot = 0xffffffffu;
     if (e < H.n_enum) {
-      uint64_t D = 0;
+      D2 D{0, 0};
       double V = 0.0;
       bool valid;
       if (e == 0) {
@@ -484,15 +547,20 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
         V = diag;
         valid = true;
       } else {
-        valid = gen_elem<MIXED>(H, C, e, D, V);
+        valid = gen_elem<MIXED, W>(H, C, e, D, V);
       }
       if (valid) {
         const double dz = V * dxt;
-        long long si = find_slot(tab, mask, D);
+        long long si = find_slot<W>(tab, mask, D);
         double x = 0.0, val = 0.0;
         if (si >= 0) {
-          Slot* sl = &tab[si];
-          if (sl->key & kMark) sl->key = D;  // clear the inserted-this-iteration mark
+          SlotW<W>* sl = &tab[si];
+          // clear the inserted-this-iteration mark
+          if constexpr (W == 1) {
+            if (sl->key & kMark) sl->key = D.lo;
+          } else {
+            if (sl->k1 & kMark) sl->k1 = D.hi | kOcc;
+          }
           x = sl->x;
           const uint64_t w = vis[(size_t)t * vis_words + (e >> 6)];
           if ((w >> (e & 63)) & 1) {
@@ -509,7 +577,8 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
       }
     }
     A[r] = keyv;
-    B[r] = keyd;
+    B[r] = keyd.lo;
+    B1[r] = keyd.hi;
     P[r] = slot;
   }
   // Exact chunk sums.
@@ -521,7 +590,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
   }
   // Valid entries have a nonzero determinant and sort before the (0, 0)
   // fillers.
-  block_sort_desc<kEPT>(A, B, P, SortScratch{ka, kb, pl});
+  block_sort_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   __shared__ int wtot[kWG / 64 + 1];
   int total_valid;
   (void)block_excl_scan(nvalid, wtot, total_valid);
@@ -546,6 +615,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
     if (i < keep) {
       out[i].val = A[r];
       out[i].det = B[r];
+      out[i].det1 = B1[r];
       out[i].slot = P[r];
       out[i].pad = 0;
     }
@@ -563,14 +633,15 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const uin
 // a power of two) holds the running top Mp in its first half; each further
 // chunk list is loaded reversed into the second half and one bitonic merge
 // level keeps the top Mp (chunk lists are sorted descending by pass 2).
-template <int E, int T, bool STREAM>
-__global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet, const uint32_t* pslot,
-                                               Slot* tab, int M, int nchunks, const StageEnt* stage,
+template <int E, int T, bool STREAM, int W>
+__global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot, SlotW<W>* tab, int M,
+                                               int nchunks, const StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
                                                const double* dx, int count_iteration) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
+  __shared__ uint64_t kc[W == 2 ? T * E : 1];
   __shared__ uint32_t pl[T * E];
   __shared__ int offs[1025];
   __shared__ int total;
@@ -599,8 +670,9 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
   }
   __syncthreads();
   const int n = total;
-  uint64_t A[E], B[E];
+  uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
+  const SortScratch scr{ka, kb, pl, kc};
   if constexpr (!STREAM) {
     if (n > T * E) {
       if (threadIdx.x == 0) st->error = ERR_CAP;
@@ -611,6 +683,7 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
       const int g = threadIdx.x * E + r;
       A[r] = 0;
       B[r] = 0;
+      B1[r] = 0;
       P[r] = 0xffffffffu;
       if (g < n) {
         int lo = 0, hi = nchunks;
@@ -622,37 +695,40 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
         const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
         A[r] = e.val;
         B[r] = e.det;
+        B1[r] = e.det1;
         P[r] = e.slot;
       }
     }
-    block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
+    block_sort_desc3<E, W == 2>(A, B, B1, P, scr);
   } else {
     const int half = T * E / 2;  // >= mw
-    auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint32_t& p) {
+    auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint64_t& b1, uint32_t& p) {
       a = 0;
       b = 0;
+      b1 = 0;
       p = 0xffffffffu;
       if (chunk < nchunks && j < offs[chunk + 1] - offs[chunk]) {
         const StageEnt& e = stage[((size_t)t * nchunks + chunk) * mw + j];
         a = e.val;
         b = e.det;
+        b1 = e.det1;
         p = e.slot;
       }
     };
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const int i = threadIdx.x * E + r;
-      if (i < half) fetch(0, i, A[r], B[r], P[r]);
-      else fetch(1, 2 * half - 1 - i, A[r], B[r], P[r]);
+      if (i < half) fetch(0, i, A[r], B[r], B1[r], P[r]);
+      else fetch(1, 2 * half - 1 - i, A[r], B[r], B1[r], P[r]);
     }
-    block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
+    block_sort_desc3<E, W == 2>(A, B, B1, P, scr, T * E);
     for (int chunk = 2; chunk < nchunks; ++chunk) {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
         const int i = threadIdx.x * E + r;
-        if (i >= half) fetch(chunk, 2 * half - 1 - i, A[r], B[r], P[r]);
+        if (i >= half) fetch(chunk, 2 * half - 1 - i, A[r], B[r], B1[r], P[r]);
       }
-      block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
+      block_sort_desc3<E, W == 2>(A, B, B1, P, scr, T * E);
     }
   }
   // First M entries; keep those present in the table (order preserved).
@@ -668,6 +744,7 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
     for (int r = 0; r < E; ++r) {
       if (threadIdx.x * E + r < m && P[r] != 0xffffffffu) {
         cand[(size_t)t * M + pos].det = B[r];
+        cand[(size_t)t * M + pos].det1 = B1[r];
         cand[(size_t)t * M + pos].slot = P[r];
         cand[(size_t)t * M + pos].pad = 0;
         ++pos;
@@ -697,7 +774,7 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
       nz += sums[w][0];
       xt += sums[w][1];
     }
-    Slot* sp = &tab[pslot[t]];
+    SlotW<W>* sp = &tab[pslot[t]];
     sp->za = (int64_t)(nz >> 32);
     sp->zb = (int64_t)(uint64_t)(nz & 0xffffffffULL);
     col_xz[2 * t] = (int64_t)(xt >> 64);
@@ -729,12 +806,13 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint64_t* pdet,
 // ------------------------------------------------------------------ k_pick
 // gcd_grad pick (src/solver.h:118-224): refetch every candidate's (x, z),
 // sort by (|x*s+z| desc, determinant desc), keep the first k distinct.
-template <int E>
+template <int E, int W>
 __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const CandEnt* cand,
-                                               const int* cand_cnt, const Slot* tab, uint64_t* pdet,
+                                               const int* cand_cnt, const SlotW<W>* tab, D2* pdet,
                                                uint32_t* pslot, double* px, double* pz) {
   __shared__ uint64_t ka[1024 * E];
   __shared__ uint64_t kb[1024 * E];
+  __shared__ uint64_t kc[W == 2 ? 1024 * E : 1];
   __shared__ uint32_t pl[1024 * E];
   __shared__ int offs[kMaxK + 2];
   if (st->error) return;
@@ -769,13 +847,14 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
   }
   const dd s = load_xss(st);
   // Refetch: candidate g belongs to the column c with offs[c] <= g < offs[c+1].
-  uint64_t A[E], B[E];
+  uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int g = threadIdx.x * E + r;
     A[r] = 0;
     B[r] = 0;
+    B1[r] = 0;
     P[r] = 0xffffffffu;
     if (g < n) {
       int lo = 0, hi = kp;
@@ -785,17 +864,21 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
         else hi = mid;
       }
       const CandEnt ce = cand[(size_t)lo * M + (g - offs[lo])];
-      const Slot& sl = tab[ce.slot];
+      const SlotW<W>& sl = tab[ce.slot];
       double z = fix_to_d(fix_join(sl.za, sl.zb));
       A[r] = bitsd(grad_key(s, sl.x, z));
       B[r] = ce.det;
+      B1[r] = ce.det1;
       P[r] = ce.slot;
     }
   }
-  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
+  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < E; ++r) kb[threadIdx.x * E + r] = B[r];
+  for (int r = 0; r < E; ++r) {
+    kb[threadIdx.x * E + r] = B[r];
+    if constexpr (W == 2) kc[threadIdx.x * E + r] = B1[r];
+  }
   __syncthreads();
   // Adjacent duplicates (equal determinants sort together) count once.
   int cnt = 0;
@@ -803,7 +886,9 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
-    first[r] = i < n && (i == 0 || kb[i - 1] != B[r]);
+    bool dup = i > 0 && kb[i - 1] == B[r];
+    if constexpr (W == 2) dup = dup && kc[i - 1] == B1[r];
+    first[r] = i < n && !dup;
     cnt += first[r] ? 1 : 0;
   }
   int distinct;
@@ -812,9 +897,9 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     if (!first[r] || pos >= k) continue;
-    pdet[pos] = B[r];
+    pdet[pos] = D2{B[r], B1[r]};
     pslot[pos] = P[r];
-    const Slot& sl = tab[P[r]];
+    const SlotW<W>& sl = tab[P[r]];
     px[pos] = sl.x;
     pz[pos] = fix_to_d(fix_join(sl.za, sl.zb));
     ++pos;
@@ -828,49 +913,53 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
 // distinct determinants above it), then lists are merged pairwise, adjacent
 // duplicates dropped and the result cut to L distinct entries, until one
 // list remains.  Same result as sorting all candidates at once.
-template <int E, int T>
+template <int E, int T, int W>
 __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, const CandEnt* cand,
-                                                 const int* cand_cnt, const Slot* tab, PickEnt* out,
+                                                 const int* cand_cnt, const SlotW<W>* tab, PickEnt* out,
                                                  int* out_cnt) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
+  __shared__ uint64_t kc[W == 2 ? T * E : 1];
   __shared__ uint32_t pl[T * E];
   if (st->error) return;
   const int t = blockIdx.x;
   const int n = t < st->k_prev ? min(cand_cnt[t], M) : 0;
   const dd s = load_xss(st);
-  uint64_t A[E], B[E];
+  uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int g = threadIdx.x * E + r;
     A[r] = 0;
     B[r] = 0;
+    B1[r] = 0;
     P[r] = 0xffffffffu;
     if (g < n) {
       const CandEnt ce = cand[(size_t)t * M + g];
-      const Slot& sl = tab[ce.slot];
+      const SlotW<W>& sl = tab[ce.slot];
       const double z = fix_to_d(fix_join(sl.za, sl.zb));
       A[r] = bitsd(grad_key(s, sl.x, z));
       B[r] = ce.det;
+      B1[r] = ce.det1;
       P[r] = ce.slot;
     }
   }
-  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl});
+  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   const int keep = n < L ? n : L;
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
-    if (i < keep) out[(size_t)t * L + i] = PickEnt{A[r], B[r], P[r], 0};
+    if (i < keep) out[(size_t)t * L + i] = PickEnt{A[r], B[r], B1[r], P[r], 0};
   }
   if (threadIdx.x == 0) out_cnt[t] = keep;
 }
 
-template <int E, int T>
+template <int E, int T, int W>
 __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, const PickEnt* in,
                                                   const int* in_cnt, int nin, PickEnt* out, int* out_cnt) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
+  __shared__ uint64_t kc[W == 2 ? T * E : 1];
   __shared__ uint32_t pl[T * E];
   __shared__ int wtot[T / 64 + 1];
   if (st->error) return;
@@ -878,38 +967,45 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, con
   const int la = 2 * j, lb = 2 * j + 1;
   const int na = in_cnt[la], nb = lb < nin ? in_cnt[lb] : 0;
   const int half = T * E / 2;  // >= L
-  uint64_t A[E], B[E];
+  uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
     A[r] = 0;
     B[r] = 0;
+    B1[r] = 0;
     P[r] = 0xffffffffu;
     if (i < half) {
       if (i < na) {
         const PickEnt e = in[(size_t)la * L + i];
-        A[r] = e.key, B[r] = e.det, P[r] = e.slot;
+        A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
       }
     } else {
       const int q = 2 * half - 1 - i;
       if (q < nb) {
         const PickEnt e = in[(size_t)lb * L + q];
-        A[r] = e.key, B[r] = e.det, P[r] = e.slot;
+        A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
       }
     }
   }
-  block_sort_desc<E>(A, B, P, SortScratch{ka, kb, pl}, T * E);
+  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc}, T * E);
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < E; ++r) kb[threadIdx.x * E + r] = B[r];
+  for (int r = 0; r < E; ++r) {
+    kb[threadIdx.x * E + r] = B[r];
+    if constexpr (W == 2) kc[threadIdx.x * E + r] = B1[r];
+  }
   __syncthreads();
   bool first[E];
   int cnt = 0;
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
-    first[r] = B[r] != 0 && (i == 0 || kb[i - 1] != B[r]);
+    // Fillers are (0, 0, 0); a real determinant is never zero.
+    bool dup = i > 0 && kb[i - 1] == B[r];
+    if constexpr (W == 2) dup = dup && kc[i - 1] == B1[r];
+    first[r] = (B[r] | B1[r]) != 0 && !dup;
     cnt += first[r] ? 1 : 0;
   }
   int distinct;
@@ -917,14 +1013,15 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, con
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     if (!first[r]) continue;
-    if (pos < L) out[(size_t)j * L + pos] = PickEnt{A[r], B[r], P[r], 0};
+    if (pos < L) out[(size_t)j * L + pos] = PickEnt{A[r], B[r], B1[r], P[r], 0};
     ++pos;
   }
   if (threadIdx.x == 0) out_cnt[j] = distinct < L ? distinct : L;
 }
 
-__global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* in_cnt, const Slot* tab,
-                            uint64_t* pdet, uint32_t* pslot, double* px, double* pz) {
+template <int W>
+__global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* in_cnt, const SlotW<W>* tab,
+                            D2* pdet, uint32_t* pslot, double* px, double* pz) {
   if (st->error) return;
   if (in_cnt[0] < k) {
     if (threadIdx.x == 0) st->error = ERR_CANDIDATES;
@@ -932,9 +1029,9 @@ __global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* i
   }
   for (int i = threadIdx.x; i < k; i += blockDim.x) {
     const PickEnt e = in[i];
-    pdet[i] = e.det;
+    pdet[i] = D2{e.det, e.det1};
     pslot[i] = e.slot;
-    const Slot& sl = tab[e.slot];
+    const SlotW<W>& sl = tab[e.slot];
     px[i] = sl.x;
     pz[i] = fix_to_d(fix_join(sl.za, sl.zb));
   }
@@ -942,15 +1039,27 @@ __global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* i
 
 // ------------------------------------------------------------------ init
 // update_x of the reference determinant with dx = 1 (src/solver.h:864-872).
-__global__ void k_init_ref(DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
-                           uint32_t* pslot, double* px, double* pz, double* dx) {
+template <int W>
+__global__ void k_init_ref(DevState* st, D2 det, SlotW<W>* tab, uint64_t mask, D2* pdet, uint32_t* pslot,
+                           double* px, double* pz, double* dx) {
   if (threadIdx.x != 0) return;
-  uint64_t b = mix64(det) & mask & ~3ULL;
+  uint64_t b = dhash<W>(det) & mask & ~3ULL;
   long long si = -1;
+  bool fresh = false;
   for (uint64_t n = 0; n <= mask && si < 0; n += 4) {
     for (int j = 0; j < 4; ++j) {
-      if (tab[b + j].key == det || tab[b + j].key == 0) {
+      const SlotW<W>& s = tab[b + j];
+      bool empty, match;
+      if constexpr (W == 1) {
+        empty = s.key == 0;
+        match = s.key == det.lo;
+      } else {
+        empty = s.k1 == 0;
+        match = !empty && (s.k1 & kHiMask) == det.hi && s.k0 == det.lo;
+      }
+      if (match || empty) {
         si = (long long)(b + j);
+        fresh = empty;
         break;
       }
     }
@@ -960,9 +1069,13 @@ __global__ void k_init_ref(DevState* st, uint64_t det, Slot* tab, uint64_t mask,
     st->error = ERR_PROBE;
     return;
   }
-  Slot* s = &tab[si];
-  bool fresh = s->key == 0;
-  s->key = det;
+  SlotW<W>* s = &tab[si];
+  if constexpr (W == 1) {
+    s->key = det.lo;
+  } else {
+    s->k0 = det.lo;
+    s->k1 = det.hi | kOcc;
+  }
   if (fresh) {
     s->x = 1.0;
     s->za = 0;
@@ -989,17 +1102,39 @@ __global__ void k_init_ref(DevState* st, uint64_t det, Slot* tab, uint64_t mask,
   dx[0] = 1.0;
 }
 
+// Empty table.  W = 1: all zero.  W = 2: word 0 = kSent (word 1 = 0 marks
+// the slot empty).  With 2^32 slots the last slot index is the "absent"
+// marker of the staging lists, so that slot is filled with a key no
+// determinant of the sector has: all 63 bits set for one word (at most 62
+// spin-orbitals), the empty determinant for two (at least one electron).
+template <int W>
+__global__ void k_table_init(SlotW<W>* tab, uint64_t nslots) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  SlotW<W> s;
+  const bool reserved = i == 0xffffffffULL;
+  if constexpr (W == 1) {
+    s.key = reserved ? kKeyMask : 0;
+  } else {
+    s.k0 = reserved ? 0 : kSent;
+    s.k1 = reserved ? kOcc : 0;
+  }
+  s.x = 0.0;
+  s.za = 0;
+  s.zb = 0;
+  tab[i] = s;
+}
+
 // ------------------------------------------------------------------ test hooks
-template <bool MIXED>
-__global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, uint64_t det, double diag, uint64_t* odet,
-                                                     double* oval, unsigned long long* count,
-                                                     long long cap) {
-  __shared__ ColCtx C;
-  setup_ctx(C, H, det);
+template <bool MIXED, int W>
+__global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, D2 det, double diag, uint64_t* odet,
+                                                     double* oval, unsigned long long* count, long long cap) {
+  __shared__ ColCtxW<W> C;
+  setup_ctx<W>(C, H, det);
   for (int r = 0; r < kEPT; ++r) {
     int e = blockIdx.x * kChunk + r * kWG + threadIdx.x;
     if (e >= H.n_enum) continue;
-    uint64_t D;
+    D2 D;
     double V;
     bool valid;
     if (e == 0) {
@@ -1007,26 +1142,36 @@ __global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, uint64_t det, dou
       V = diag;
       valid = true;
     } else {
-      valid = gen_elem<MIXED>(H, C, e, D, V);
+      valid = gen_elem<MIXED, W>(H, C, e, D, V);
     }
     if (!valid) continue;
     unsigned long long pos = atomicAdd(count, 1ULL);
     if ((long long)pos < cap) {
-      odet[pos] = D;
+      odet[2 * pos] = D.lo;
+      odet[2 * pos + 1] = D.hi;
       oval[pos] = V;
     }
   }
 }
 
-__global__ void k_table_compact(const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox, int64_t* oz,
+template <int W>
+__global__ void k_table_compact(const SlotW<W>* tab, uint64_t nslots, uint64_t* odet, double* ox, int64_t* oz,
                                 unsigned long long* count, long long cap) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  Slot s = tab[i];
-  if (s.key == 0) return;
+  if (i >= nslots || i == 0xffffffffULL) return;
+  SlotW<W> s = tab[i];
+  D2 d;
+  if constexpr (W == 1) {
+    if (s.key == 0) return;
+    d = D2{s.key & kKeyMask, 0};
+  } else {
+    if (s.k1 == 0) return;
+    d = D2{s.k0, s.k1 & kHiMask};
+  }
   unsigned long long pos = atomicAdd(count, 1ULL);
   if ((long long)pos < cap) {
-    odet[pos] = s.key & kKeyMask;
+    odet[2 * pos] = d.lo;
+    odet[2 * pos + 1] = d.hi;
     ox[pos] = s.x;
     oz[2 * pos] = s.za;
     oz[2 * pos + 1] = s.zb;
@@ -1034,153 +1179,188 @@ __global__ void k_table_compact(const Slot* tab, uint64_t nslots, uint64_t* odet
 }
 
 // ------------------------------------------------------------------ launch wrappers
-void launch_submatrix(hipStream_t s, const DevHam& H, const DevState* st, const uint64_t* dets, int k,
+#define MC_W(W_, ...)      \
+  do {                     \
+    if ((W_) == 1) {       \
+      constexpr int W = 1; \
+      __VA_ARGS__;         \
+    } else {               \
+      constexpr int W = 2; \
+      __VA_ARGS__;         \
+    }                      \
+  } while (0)
+
+void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st, const D2* dets, int k,
                       double* Hs, int ldh) {
-  hipLaunchKernelGGL(k_submatrix, dim3(k), dim3(kWG), 0, s, H, st, dets, k, Hs, ldh);
+  MC_W(nw, hipLaunchKernelGGL(k_submatrix<W>, dim3(k), dim3(kWG), 0, s, H, st, dets, k, Hs, ldh));
 }
 
-void launch_scatter(hipStream_t s, const DevHam& H, DevState* st, const uint64_t* pdet, const double* dx,
-                    const double* Hs, int ldh, Slot* tab, uint64_t mask, uint64_t* vis, int vis_words,
-                    int k, int nchunks) {
-  if (H.mixed)
-    hipLaunchKernelGGL(k_scatter<true>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words);
-  else
-    hipLaunchKernelGGL(k_scatter<false>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words);
+void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
+                    const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
+                    int nchunks) {
+  MC_W(nw, {
+    if (H.mixed)
+      hipLaunchKernelGGL((k_scatter<true, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
+                         (SlotW<W>*)tab, mask, vis, vis_words);
+    else
+      hipLaunchKernelGGL((k_scatter<false, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
+                         (SlotW<W>*)tab, mask, vis, vis_words);
+  });
 }
 
-void launch_pass2(hipStream_t s, const DevHam& H, DevState* st, const uint64_t* pdet, const double* dx,
-                  const double* Hs, int ldh, Slot* tab, uint64_t mask, const uint64_t* vis, int vis_words,
-                  int M, int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
-  if (H.mixed)
-    hipLaunchKernelGGL(k_pass2<true>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab, mask,
-                       vis, vis_words, M, nchunks, stage, stage_cnt, part);
-  else
-    hipLaunchKernelGGL(k_pass2<false>, dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
+void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
+                  const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
+                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
+  MC_W(nw, {
+    if (H.mixed)
+      hipLaunchKernelGGL((k_pass2<true, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
+                         (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
+    else
+      hipLaunchKernelGGL((k_pass2<false, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
+                         (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
+  });
 }
 
-void launch_merge(hipStream_t s, DevState* st, const uint64_t* pdet, const uint32_t* pslot, Slot* tab,
-                  int M, int k, int nchunks, const StageEnt* stage, const int* stage_cnt,
-                  const int64_t* part, int64_t* col_xz, CandEnt* cand, int* cand_cnt, const double* dx,
-                  int count_iteration) {
+void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
+                  int nchunks, const StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
+                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration) {
   // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
   // entries, then four entries per thread).  Beyond: the streaming merge
   // with a window of 2 * Mp entries.
   const int mw = M < kChunk ? M : kChunk;
-#define MC_MERGE(E_, T_, S_)                                                                              \
-  hipLaunchKernelGGL((k_merge<E_, T_, S_>), dim3(k), dim3(T_), 0, s, st, pdet, pslot, tab, M, nchunks,    \
+#define MC_MERGE(E_, T_, S_)                                                                                 \
+  hipLaunchKernelGGL((k_merge<E_, T_, S_, W>), dim3(k), dim3(T_), 0, s, st, pslot, (SlotW<W>*)tab, M, nchunks, \
                      stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration)
-  if ((long long)nchunks * mw <= kSortCap) {
-    int w = kWG;
-    while (w < nchunks * mw) w <<= 1;
-    switch (w) {
-      case 256: MC_MERGE(1, 256, false); break;
-      case 512: MC_MERGE(2, 256, false); break;
-      case 1024: MC_MERGE(4, 256, false); break;
-      case 2048: MC_MERGE(4, 512, false); break;
-      default: MC_MERGE(4, 1024, false); break;
+  MC_W(nw, {
+    if ((long long)nchunks * mw <= kSortCap) {
+      int w = kWG;
+      while (w < nchunks * mw) w <<= 1;
+      switch (w) {
+        case 256: MC_MERGE(1, 256, false); break;
+        case 512: MC_MERGE(2, 256, false); break;
+        case 1024: MC_MERGE(4, 256, false); break;
+        case 2048: MC_MERGE(4, 512, false); break;
+        default: MC_MERGE(4, 1024, false); break;
+      }
+    } else {
+      int w = kWG;
+      while (w < 2 * mw) w <<= 1;
+      switch (w) {
+        case 256: MC_MERGE(1, 256, true); break;
+        case 512: MC_MERGE(2, 256, true); break;
+        case 1024: MC_MERGE(4, 256, true); break;
+        default: MC_MERGE(8, 256, true); break;
+      }
     }
-  } else {
-    int w = kWG;
-    while (w < 2 * mw) w <<= 1;
-    switch (w) {
-      case 256: MC_MERGE(1, 256, true); break;
-      case 512: MC_MERGE(2, 256, true); break;
-      case 1024: MC_MERGE(4, 256, true); break;
-      default: MC_MERGE(8, 256, true); break;
-    }
-  }
+  });
 #undef MC_MERGE
 }
 
-void launch_pick(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
-                 const Slot* tab, uint64_t* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0,
-                 PickEnt* pk1, int* pc0, int* pc1) {
-  if ((long long)k * M <= kSortCap) {
-    // One workgroup sorts all candidates: the smallest power of two of at
-    // least k * M entries (at least one wave); four entries per thread from
-    // 256 entries up.
-    int w = 64;
-    while (w < k * M) w <<= 1;
-    const int E = w >= 256 ? 4 : (w >= 128 ? 2 : 1);
-    const int threads = w / E;
-    switch (E) {
-      case 1:
-        hipLaunchKernelGGL(k_pick<1>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
-                           pslot, px, pz);
-        break;
-      case 2:
-        hipLaunchKernelGGL(k_pick<2>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
-                           pslot, px, pz);
-        break;
-      default:
-        hipLaunchKernelGGL(k_pick<4>, dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tab, pdet,
-                           pslot, px, pz);
-        break;
-    }
-    return;
-  }
-  const int L = M < k ? M : k;
-  // Column sorts: width next power of two >= M (<= 1024).
-  int wc = 64;
-  while (wc < M) wc <<= 1;
-#define MC_COLS(E_, T_) \
-  hipLaunchKernelGGL((k_pick_cols<E_, T_>), dim3(k), dim3(T_), 0, s, st, M, L, cand, cand_cnt, tab, pk0, pc0)
-  switch (wc) {
-    case 64: MC_COLS(1, 64); break;
-    case 128: MC_COLS(1, 128); break;
-    case 256: MC_COLS(1, 256); break;
-    case 512: MC_COLS(2, 256); break;
-    default: MC_COLS(4, 256); break;
-  }
+template <int W>
+void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
+                   const SlotW<W>* tb, D2* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0,
+                   PickEnt* pk1, int* pc0, int* pc1) {
+  {
+    if ((long long)k * M <= kSortCap) {
+      // One workgroup sorts all candidates: the smallest power of two of at
+      // least k * M entries (at least one wave); four entries per thread from
+      // 256 entries up.
+      int w = 64;
+      while (w < k * M) w <<= 1;
+      const int E = w >= 256 ? 4 : (w >= 128 ? 2 : 1);
+      const int threads = w / E;
+      switch (E) {
+        case 1:
+          hipLaunchKernelGGL((k_pick<1, W>), dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tb, pdet,
+                             pslot, px, pz);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_pick<2, W>), dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tb, pdet,
+                             pslot, px, pz);
+          break;
+        default:
+          hipLaunchKernelGGL((k_pick<4, W>), dim3(1), dim3(threads), 0, s, st, k, M, cand, cand_cnt, tb, pdet,
+                             pslot, px, pz);
+          break;
+      }
+    } else {
+      const int L = M < k ? M : k;
+      // Column sorts: width next power of two >= M (<= 1024).
+      int wc = 64;
+      while (wc < M) wc <<= 1;
+#define MC_COLS(E_, T_)                                                                                       \
+  hipLaunchKernelGGL((k_pick_cols<E_, T_, W>), dim3(k), dim3(T_), 0, s, st, M, L, cand, cand_cnt, tb, pk0, pc0)
+      switch (wc) {
+        case 64: MC_COLS(1, 64); break;
+        case 128: MC_COLS(1, 128); break;
+        case 256: MC_COLS(1, 256); break;
+        case 512: MC_COLS(2, 256); break;
+        default: MC_COLS(4, 256); break;
+      }
 #undef MC_COLS
-  // Merge levels: window 2 * next power of two >= L.
-  int wm = 64;
-  while (wm < 2 * L) wm <<= 1;
-  PickEnt *in = pk0, *out = pk1;
-  int *ic = pc0, *oc = pc1;
-  int nin = k;
-  while (nin > 1) {
-    const int nout = (nin + 1) / 2;
+      // Merge levels: window 2 * next power of two >= L.
+      int wm = 64;
+      while (wm < 2 * L) wm <<= 1;
+      PickEnt *in = pk0, *out = pk1;
+      int *ic = pc0, *oc = pc1;
+      int nin = k;
+      while (nin > 1) {
+        const int nout = (nin + 1) / 2;
 #define MC_PM(E_, T_) \
-  hipLaunchKernelGGL((k_pick_merge<E_, T_>), dim3(nout), dim3(T_), 0, s, st, L, in, ic, nin, out, oc)
-    switch (wm) {
-      case 64: MC_PM(1, 64); break;
-      case 128: MC_PM(1, 128); break;
-      case 256: MC_PM(1, 256); break;
-      case 512: MC_PM(2, 256); break;
-      default: MC_PM(4, 256); break;
-    }
+  hipLaunchKernelGGL((k_pick_merge<E_, T_, W>), dim3(nout), dim3(T_), 0, s, st, L, in, ic, nin, out, oc)
+        switch (wm) {
+          case 64: MC_PM(1, 64); break;
+          case 128: MC_PM(1, 128); break;
+          case 256: MC_PM(1, 256); break;
+          case 512: MC_PM(2, 256); break;
+          default: MC_PM(4, 256); break;
+        }
 #undef MC_PM
-    std::swap(in, out);
-    std::swap(ic, oc);
-    nin = nout;
+        std::swap(in, out);
+        std::swap(ic, oc);
+        nin = nout;
+      }
+      hipLaunchKernelGGL(k_pick_emit<W>, dim3(1), dim3(256), 0, s, st, k, in, ic, tb, pdet, pslot, px, pz);
+    }
   }
-  hipLaunchKernelGGL(k_pick_emit, dim3(1), dim3(256), 0, s, st, k, in, ic, tab, pdet, pslot, px, pz);
 }
 
-void launch_init_ref(hipStream_t s, DevState* st, uint64_t det, Slot* tab, uint64_t mask, uint64_t* pdet,
+void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
+                 const void* tab, D2* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0, PickEnt* pk1,
+                 int* pc0, int* pc1) {
+  MC_W(nw, launch_pick_w<W>(s, st, k, M, cand, cand_cnt, (const SlotW<W>*)tab, pdet, pslot, px, pz, pk0, pk1, pc0,
+                            pc1));
+}
+
+void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uint64_t mask, D2* pdet,
                      uint32_t* pslot, double* px, double* pz, double* dx) {
-  hipLaunchKernelGGL(k_init_ref, dim3(1), dim3(64), 0, s, st, det, tab, mask, pdet, pslot, px, pz, dx);
+  MC_W(nw, hipLaunchKernelGGL(k_init_ref<W>, dim3(1), dim3(64), 0, s, st, det, (SlotW<W>*)tab, mask, pdet, pslot,
+                              px, pz, dx));
 }
 
-void launch_column_dump(hipStream_t s, const DevHam& H, uint64_t det, double diag, uint64_t* odet,
-                        double* oval, unsigned long long* count, long long cap) {
-  int nch = (H.n_enum + kChunk - 1) / kChunk;
-  if (H.mixed)
-    hipLaunchKernelGGL(k_column_dump<true>, dim3(nch), dim3(kWG), 0, s, H, det, diag, odet, oval, count, cap);
-  else
-    hipLaunchKernelGGL(k_column_dump<false>, dim3(nch), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
-                       cap);
+void launch_table_init(hipStream_t s, int nw, void* tab, uint64_t nslots) {
+  const uint64_t blocks = (nslots + 255) / 256;
+  MC_W(nw, hipLaunchKernelGGL(k_table_init<W>, dim3((unsigned)blocks), dim3(256), 0, s, (SlotW<W>*)tab, nslots));
 }
 
-void launch_table_compact(hipStream_t s, const Slot* tab, uint64_t nslots, uint64_t* odet, double* ox,
+void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double diag, uint64_t* odet, double* oval,
+                        unsigned long long* count, long long cap) {
+  const int nchunks = (H.n_enum + kChunk - 1) / kChunk;
+  MC_W(nw, {
+    if (H.mixed)
+      hipLaunchKernelGGL((k_column_dump<true, W>), dim3(nchunks), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
+                         cap);
+    else
+      hipLaunchKernelGGL((k_column_dump<false, W>), dim3(nchunks), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
+                         cap);
+  });
+}
+
+void launch_table_compact(hipStream_t s, int nw, const void* tab, uint64_t nslots, uint64_t* odet, double* ox,
                           int64_t* oz, unsigned long long* count, long long cap) {
-  uint64_t blocks = (nslots + 255) / 256;
-  hipLaunchKernelGGL(k_table_compact, dim3((unsigned)blocks), dim3(256), 0, s, tab, nslots, odet, ox, oz,
-                     count, cap);
+  const uint64_t blocks = (nslots + 255) / 256;
+  MC_W(nw, hipLaunchKernelGGL(k_table_compact<W>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<W>*)tab,
+                              nslots, odet, ox, oz, count, cap));
 }
 
 }  // namespace mc

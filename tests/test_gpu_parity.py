@@ -26,10 +26,11 @@ def _path(name, workloads, h2o_fcidump):
 
 
 def _dict(dets, vals):
-    return dict(zip(dets[:, 0].tolist(), vals.tolist()))
+    return dict(zip(map(tuple, dets.tolist()), vals.tolist()))
 
 
-@pytest.mark.parametrize("name", ["c1", "c3_synth", "c3_hubbard", "h2o", "c2", "c4"])
+# c5 (42 orbitals, 84 spin-orbitals) runs on two-word determinants.
+@pytest.mark.parametrize("name", ["c1", "c3_synth", "c3_hubbard", "h2o", "c2", "c4", "c5"])
 def test_columns_bit_exact(name, workloads, h2o_fcidump):
     path = _path(name, workloads, h2o_fcidump)
     print("case", name, flush=True)
@@ -47,17 +48,17 @@ def test_columns_bit_exact(name, workloads, h2o_fcidump):
         assert len(gd) == len(od2)
         assert A.keys() == B.keys()
         assert all(A[key] == B[key] for key in A)
-        assert gd[0, 0] == d[0]
+        assert np.array_equal(gd[0], np.asarray(d).reshape(-1))
 
 
-@pytest.mark.parametrize("name", ["c1", "c3_synth", "h2o"])
+@pytest.mark.parametrize("name", ["c1", "c3_synth", "h2o", "c5"])
 def test_submatrix_bit_exact(name, workloads, h2o_fcidump):
     path = _path(name, workloads, h2o_fcidump)
     o = Oracle(path)
     g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.01, threshold=0.0)
     od, _ = o.column(o.hf())
     rng = np.random.default_rng(4)
-    dets = od[rng.choice(len(od), min(40, len(od)), replace=False), 0]
+    dets = od[rng.choice(len(od), min(40, len(od)), replace=False)]
     H = g.submatrix(dets)
     for i in range(len(dets)):
         for j in range(i, len(dets)):
@@ -106,12 +107,12 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
             assert rg[key] == ro[key], (key, done, rg[key], ro[key])
     gd, gx, gz = g.table()
     od, ox, oz, _ = o.table()
-    assert np.array_equal(gd[:, 0], od[:, 0])
+    assert np.array_equal(gd, od)
     assert np.array_equal(gx, ox)
     assert np.array_equal(gz, oz)
     cd, cx, cz = g.candidates()
     kd, kx, kz = o.candidates()
-    assert sorted(cd[:, 0].tolist()) == sorted(kd[:, 0].tolist())
+    assert sorted(map(tuple, cd.tolist())) == sorted(map(tuple, kd.tolist()))
     return g, o
 
 
@@ -130,11 +131,14 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
     ("c2", 16, 1024, 1e-7, 12),
     # the norb-24 molecule at k = M = 256
     ("c4", 256, 256, 1e-6, 4),
+    # 42 orbitals: two-word determinants
+    ("c5", 8, 16, 1e-5, 12),
+    ("c5", 64, 64, 1e-5, 4),
 ])
 def test_trajectory_bit_exact(name, k, M, z_thr, iters, workloads, h2o_fcidump):
     print("case", name, k, M, z_thr, iters, flush=True)
     path = _path(name, workloads, h2o_fcidump)
-    mem = 8.0 if name in ("c2", "c4") else 0.05
+    mem = 8.0 if name in ("c2", "c4", "c5") else 0.05
     _run_pair(path, k, M, z_thr, iters, max_memory=mem)
 
 
