@@ -8,14 +8,17 @@ iteration (pick, H_sub, subspace update, column scatter, candidate update).
 
 Phases:
   1. throughput: the mixed+batched form (k = 64 coordinates per iteration,
-     FP32 integrals and H elements, FP64 z / energy / eigensolve), W warmup
-     iterations, then exactly K timed iterations; value = k*K / seconds.
+     FP32 integrals and H elements, FP64 z / energy / eigensolve): launch
+     forms autotuned on real iterations, iterations replayed from HIP graphs
+     of G captured iterations, W warmup iterations, then exactly K timed
+     iterations; value = k*K / seconds.
   2. roofline: a short segment with HIP events around every kernel on the
      solver stream gives the average k_scatter duration; algorithmic bytes =
      H-elements per launch * 32 B (key 8 + x 8 + z read 8 + z write 8).
   3. time to |E - E_ref| < 1e-5 Ha: plain FP64 (k = 8) and mixed+batched
-     each run for a bounded wall time from the same start; E_ref is the
-     plain run's final energy.
+     each run for a bounded wall time from the same start, both with the
+     same launch machinery (autotune inside the timed run, graph replay);
+     E_ref is the plain run's final energy.
   4. cpu_baseline: the CPU oracle (a restatement of the reference's OpenMP
      path) on a bounded sample of the same workload on this host.
 """
@@ -45,14 +48,19 @@ def make_workload(name, outdir):
     return synth.generate(name, outdir)
 
 
-def run_for(ctx, seconds, report_every, max_iters=10 ** 9):
-    """Run until `seconds` elapse or the wavefunction table fills.
+def run_for(ctx, seconds, report_every, tune_rounds=0, graph=0, max_iters=10 ** 9):
+    """Run until `seconds` elapse or the wavefunction table fills; the
+    autotuning iterations run inside the timed span and count.
 
     Returns ([(t, iterations, energy)], stop reason)."""
     from mcdfci_amd import McdfciError
     traj = []
     t0 = time.perf_counter()
     it = 0
+    if tune_rounds:
+        it += ctx.autotune(tune_rounds)["iterations"]
+    if graph:
+        ctx.set_graph(graph)
     while True:
         ctx.step(report_every)
         try:
@@ -125,6 +133,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-ttt", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=20, help="iterations per captured HIP graph (0: off)")
+    ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -162,6 +172,9 @@ def main():
     mb = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem, threshold=0.0,
                precision=PREC_MIXED)
     mb.init()
+    choice = mb.autotune(args.tune_rounds) if args.tune_rounds else None
+    log(f"autotune: {choice}")
+    mb.set_graph(args.graph)
     mb.step(args.warmup)
     r0 = mb.report()
     t0 = time.perf_counter()
@@ -195,12 +208,12 @@ def main():
         plain = Cdfci(fcid, num_of_coordinate=args.plain_k, estimate=args.plain_k, z_threshold=z_thr,
                       max_memory=max_mem, threshold=0.0, precision=PREC_FP64)
         plain.init()
-        tp, stop_p = run_for(plain, args.ttt_seconds, 200)
+        tp, stop_p = run_for(plain, args.ttt_seconds, 200, args.tune_rounds, args.graph)
         plain.close()
         mb2 = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem,
                     threshold=0.0, precision=PREC_MIXED)
         mb2.init()
-        tm, stop_m = run_for(mb2, args.ttt_seconds, 20)
+        tm, stop_m = run_for(mb2, args.ttt_seconds, 20, args.tune_rounds, args.graph)
         mb2.close()
         e_ref = tp[-1][2]
         t_plain, i_plain = first_within(tp, e_ref, 1e-5)
@@ -256,7 +269,10 @@ def main():
         "config": {"workload": f"{args.workload}: synthetic norb={spec.norb} nelec={spec.nelec} "
                                f"(ERI rank {spec.rank}, seed {spec.seed})",
                    "num_of_coordinate": k, "estimate": k, "z_threshold": z_thr, "max_memory_gb": max_mem,
-                   "precision": "mixed", "parallelism": "single GPU"},
+                   "precision": "mixed", "parallelism": "single GPU",
+                   "graph_iterations": args.graph,
+                   "scatter_elements_per_thread": choice["elements_per_thread"] if choice else 4,
+                   "tree_pick": choice["tree_pick"] if choice else None},
         "h_elements_per_s": h_elem_rate,
         "energy_after_steps": mb_energy_steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -121,6 +121,24 @@ int mcdfci_step(mcdfci_ctx* ctx, int64_t n_iter);
 /* The report row (src/solver.h:761-798). Synchronises the device. */
 int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
 
+/* Batching additions around mcdfci_step; no counterpart in the reference,
+ * whose inner loop (src/solver.h:701-759) they execute.  Neither changes a
+ * result bit.
+ *
+ * mcdfci_set_graph: iterations_per_graph > 0 makes mcdfci_step replay a HIP
+ * graph holding that many captured iterations (captured on first use and
+ * after mcdfci_init or mcdfci_autotune; the remainder of a step runs
+ * uncaptured); 0 launches every kernel directly (the default).  Steps with
+ * timing enabled are never captured.
+ *
+ * mcdfci_autotune: runs `rounds` rounds of real iterations (they advance the
+ * solver like mcdfci_step), timing the scatter at 1, 2, 4 and 8 column
+ * elements per thread and the pick with one-workgroup and merge-tree sorts,
+ * and keeps the fastest of each (median kernel time).  choice (may be NULL)
+ * receives {elements per thread, 1 if merge-tree pick, iterations run}. */
+int mcdfci_set_graph(mcdfci_ctx* ctx, int iterations_per_graph);
+int mcdfci_autotune(mcdfci_ctx* ctx, int rounds, int64_t* choice);
+
 /* Phase times in seconds (pick, submatrix, columns, eig, update_x, update_z),
  * the step_timer of src/solver.h:683,704-754; device phases are measured
  * with HIP events when timing is enabled (mcdfci_set_timing). */

@@ -89,6 +89,8 @@ SIGNATURES = {
     "mcdfci_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mcdfci_timers": (C.c_int, [C.c_void_p, _f64p]),
     "mcdfci_kernel_times": (C.c_int, [C.c_void_p, _f64p, C.POINTER(C.c_int64)]),
+    "mcdfci_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mcdfci_autotune": (C.c_int, [C.c_void_p, C.c_int, _i64p]),
     "mcdfci_column": (C.c_int, [C.c_void_p, _u64p, _u64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_submatrix": (C.c_int, [C.c_void_p, C.c_int, _u64p, _f64p]),
     "mcdfci_eig_update": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
@@ -200,6 +202,17 @@ class Cdfci:
         n = C.c_int64()
         self._check(self.lib.mcdfci_kernel_times(self.h, t, C.byref(n)))
         return dict(zip(["pick", "submatrix", "eig", "scatter", "pass2", "merge"], t.tolist())), n.value
+
+    def set_graph(self, iterations_per_graph):
+        """Replay steps from a HIP graph of this many captured iterations (0: off)."""
+        self._check(self.lib.mcdfci_set_graph(self.h, int(iterations_per_graph)))
+
+    def autotune(self, rounds=3):
+        """Time the scatter and pick launch forms on real iterations and keep the
+        fastest; returns {"elements_per_thread", "tree_pick", "iterations"}."""
+        ch = np.zeros(3, np.int64)
+        self._check(self.lib.mcdfci_autotune(self.h, int(rounds), ch))
+        return {"elements_per_thread": int(ch[0]), "tree_pick": bool(ch[1]), "iterations": int(ch[2])}
 
     def column(self, det):
         d = np.zeros(2, np.uint64)

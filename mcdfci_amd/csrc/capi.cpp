@@ -78,6 +78,12 @@ struct mcdfci_ctx {
   int* pc0 = nullptr;
   int* pc1 = nullptr;
   size_t iter_bytes = 0;
+  // launch choices (result-neutral; set by mcdfci_autotune)
+  int scatter_ept = 4;
+  bool pick_tree = false;
+  // captured iterations (mcdfci_set_graph)
+  int graph_iters = 0;
+  hipGraphExec_t graph = nullptr;
   bool ready = false;
   bool timing = false;
   double timers[6] = {0, 0, 0, 0, 0, 0};
@@ -338,7 +344,7 @@ int slot_doubles(int nw) { return (int)(slot_bytes(nw) / sizeof(double)); }
 void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
   hipStream_t s = c->stream;
   launch_scatter(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
-                 k, c->nchunks);
+                 k, c->scatter_ept);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[4], s);
   launch_pass2(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
                c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part);
@@ -355,7 +361,7 @@ void launch_iteration(mcdfci_ctx* c) {
   const int k = c->k;
   if (c->timing) (void)hipEventRecord(c->ev[0], s);
   launch_pick(s, c->nw, c->st, k, c->M, c->cand, c->cand_cnt, c->tab, c->pdet, c->pslot, c->px, c->pz, c->pk0,
-              c->pk1, c->pc0, c->pc1);
+              c->pk1, c->pc0, c->pc1, c->pick_tree);
   if (c->timing) (void)hipEventRecord(c->ev[1], s);
   launch_submatrix(s, c->nw, c->dh, c->st, c->pdet, k, c->Hs, k);
   if (c->timing) (void)hipEventRecord(c->ev[2], s);
@@ -376,6 +382,32 @@ void launch_iteration(mcdfci_ctx* c) {
     c->timers[3] += ms[2] * 1e-3;
     c->timers[5] += (ms[3] + ms[4] + ms[5]) * 1e-3;
   }
+}
+
+void drop_graph(mcdfci_ctx* c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+}
+
+// Captures graph_iters iterations of the inner loop on the solver stream.
+int capture_graph(mcdfci_ctx* c) {
+  drop_graph(c);
+  HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < c->graph_iters; ++i) launch_iteration(c);
+  hipGraph_t g = nullptr;
+  HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph = nullptr;
+    return fail(c, MCDFCI_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return MCDFCI_OK;
+}
+
+double median(std::vector<double> v) {
+  std::sort(v.begin(), v.end());
+  return v.empty() ? 0.0 : v[v.size() / 2];
 }
 
 }  // namespace
@@ -475,6 +507,7 @@ int mcdfci_destroy(mcdfci_ctx* c) {
   if (!c) return MCDFCI_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (auto& e : c->ev)
@@ -524,6 +557,7 @@ int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
                 "Reference determinant is invalid. The number of valid orbitals does not match the number of "
                 "electrons.");
   set_sector(c->dh, na, nb);
+  drop_graph(c);  // the iteration buffers are reallocated
   rc = alloc_iteration(c);
   if (rc) return rc;
   launch_table_init(c->stream, c->nw, c->tab, c->nslots);
@@ -554,9 +588,75 @@ int mcdfci_step(mcdfci_ctx* c, int64_t n) {
   if (!c) return fail(nullptr, MCDFCI_E_INPUT, "null context");
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
   HIPCHECK(c, hipSetDevice(c->device));
+  if (c->graph_iters > 0 && !c->timing && n >= c->graph_iters) {
+    if (!c->graph) {
+      int rc = capture_graph(c);
+      if (rc) return rc;
+    }
+    for (; n >= c->graph_iters; n -= c->graph_iters) HIPCHECK(c, hipGraphLaunch(c->graph, c->stream));
+  }
   for (int64_t i = 0; i < n; ++i) launch_iteration(c);
   HIPCHECK(c, hipGetLastError());
   return MCDFCI_OK;
+}
+
+int mcdfci_set_graph(mcdfci_ctx* c, int iterations_per_graph) {
+  if (!c || iterations_per_graph < 0) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (iterations_per_graph != c->graph_iters) drop_graph(c);
+  c->graph_iters = iterations_per_graph;
+  return MCDFCI_OK;
+}
+
+// Online tuning on real iterations: each round runs one iteration per
+// (elements per thread, pick form) pair, timing the scatter and the pick
+// with HIP events; the per-kernel medians pick the configuration.
+int mcdfci_autotune(mcdfci_ctx* c, int rounds, int64_t* choice) {
+  if (!c || rounds < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const int epts[4] = {1, 2, 4, 8};
+  const bool can_single = (long long)c->k * c->M <= kSortCap;
+  const int npick = can_single ? 2 : 1;
+  std::vector<double> ts[4], tp[2];
+  const bool timing = c->timing;
+  double kt[6];
+  std::copy(c->ktimes, c->ktimes + 6, kt);
+  const long long kn = c->ktimed;
+  c->timing = true;
+  int64_t iters = 0;
+  for (int r = 0; r < rounds; ++r)
+    for (int e = 0; e < 4; ++e)
+      for (int p = 0; p < npick; ++p) {
+        c->scatter_ept = epts[e];
+        c->pick_tree = !can_single || p == 1;
+        const double s0 = c->ktimes[3], p0 = c->ktimes[0];
+        launch_iteration(c);  // synchronises (timing mode)
+        ts[e].push_back(c->ktimes[3] - s0);
+        tp[c->pick_tree ? 1 : 0].push_back(c->ktimes[0] - p0);
+        ++iters;
+      }
+  c->timing = timing;
+  // the tuning iterations count in the step timers but not in the
+  // per-kernel times of the chosen configuration
+  std::copy(kt, kt + 6, c->ktimes);
+  c->ktimed = kn;
+  int best = 2;
+  for (int e = 0; e < 4; ++e)
+    if (median(ts[e]) < median(ts[best])) best = e;
+  c->scatter_ept = epts[best];
+  c->pick_tree = !can_single || median(tp[1]) < median(tp[0]);
+  drop_graph(c);
+  HIPCHECK(c, hipGetLastError());
+  if (choice) {
+    choice[0] = c->scatter_ept;
+    choice[1] = c->pick_tree ? 1 : 0;
+    choice[2] = iters;
+  }
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  return check_state_error(c, s);
 }
 
 int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {

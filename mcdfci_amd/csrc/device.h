@@ -27,6 +27,10 @@ constexpr int kWG = 256;               // threads per workgroup
 constexpr int kEPT = 4;                // column elements per thread
 constexpr int kChunk = kWG * kEPT;     // column elements per workgroup
 constexpr int kSortCap = 4096;         // entries sorted by one workgroup
+// Workgroups of the whole-table sweeps (init, dump, compaction), which loop
+// over the slots: a dispatch's grid is limited to 2^32 work-items, the table
+// may hold 2^32 slots.
+constexpr uint64_t kSlotBlocks = 1 << 20;
 constexpr int kMaxK = 256;             // coordinates per iteration
 constexpr int kMaxOrb = 31;            // spatial orbitals of one-word determinants (bit 63 spare)
 constexpr int kMaxOrb2 = 63;           // spatial orbitals of two-word determinants (bits 126, 127 spare)

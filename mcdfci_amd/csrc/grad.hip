@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "column.h"
 #include "device.h"
 #include "grad.h"
@@ -54,22 +56,24 @@ __device__ __forceinline__ void acc_add(int64_t* acc, size_t idx, int64_t a, int
 template <int W>
 __global__ void k_compact_x(const SlotW<W>* tab, uint64_t nslots, D2* odet, double* ox, unsigned long long* count,
                             long long cap) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots || i == 0xffffffffULL) return;
-  const SlotW<W> s = tab[i];
-  D2 d;
-  if constexpr (W == 1) {
-    if (s.key == 0) return;
-    d = D2{s.key & kKeyMask, 0};
-  } else {
-    if (s.k1 == 0) return;
-    d = D2{s.k0, s.k1 & kHiMask};
-  }
-  if (!(__builtin_fabs(s.x) != 0.0)) return;
-  const unsigned long long pos = atomicAdd(count, 1ULL);
-  if ((long long)pos < cap) {
-    odet[pos] = d;
-    ox[pos] = s.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    if (i == 0xffffffffULL) continue;
+    const SlotW<W> s = tab[i];
+    D2 d;
+    if constexpr (W == 1) {
+      if (s.key == 0) continue;
+      d = D2{s.key & kKeyMask, 0};
+    } else {
+      if (s.k1 == 0) continue;
+      d = D2{s.k0, s.k1 & kHiMask};
+    }
+    if (!(__builtin_fabs(s.x) != 0.0)) continue;
+    const unsigned long long pos = atomicAdd(count, 1ULL);
+    if ((long long)pos < cap) {
+      odet[pos] = d;
+      ox[pos] = s.x;
+    }
   }
 }
 
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(kWG) void k_energy_grad(DevHam H, const SlotW<W>* t
 
 void launch_compact_x(hipStream_t s, int nw, const void* tab, uint64_t nslots, D2* odet, double* ox,
                       unsigned long long* count, long long cap) {
-  const uint64_t blocks = (nslots + 255) / 256;
+  const uint64_t blocks = std::min<uint64_t>((nslots + 255) / 256, kSlotBlocks);
   if (nw == 1)
     hipLaunchKernelGGL(k_compact_x<1>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<1>*)tab, nslots, odet,
                        ox, count, cap);

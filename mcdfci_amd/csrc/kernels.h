@@ -12,18 +12,22 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st
                       double* Hs, int ldh);
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode);
+// ept: column elements per thread of the scatter (1, 2, 4 or 8); any value
+// gives the same result.
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                    int nchunks);
+                    int ept);
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
                   int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part);
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, const StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration);
+// tree: per-column sorts and a merge tree even where one workgroup could
+// sort all k * M candidates (same result either way).
 void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
                  const void* tab, D2* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0, PickEnt* pk1,
-                 int* pc0, int* pc1);
+                 int* pc0, int* pc1, bool tree);
 void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uint64_t mask, D2* pdet,
                      uint32_t* pslot, double* px, double* pz, double* dx);
 void launch_table_init(hipStream_t s, int nw, void* tab, uint64_t nslots);

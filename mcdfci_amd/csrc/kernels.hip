@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <utility>
 
 #include "device.h"
@@ -362,8 +363,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int& total) {
 
 // ------------------------------------------------------------------ k_scatter
 // Column generation (src/hamiltonian.h:336-418) fused with the scatter-add
-// z += H dx and the insert gate (src/wavefunction.h:499-528).
-template <bool MIXED, int W>
+// z += H dx and the insert gate (src/wavefunction.h:499-528).  Each
+// workgroup takes EPT * kWG consecutive column elements; the visibility
+// bitmap is indexed by element, so EPT is free of the pass-2 chunking.
+template <bool MIXED, int W, int EPT>
 __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                  const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                  uint64_t* vis, int vis_words) {
@@ -379,8 +382,8 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   long long n_new = 0, n_upd = 0, n_valid = 0;
   bool fail = false;
-  for (int r = 0; r < kEPT; ++r) {
-    const int e = chunk * kChunk + r * kWG + threadIdx.x;
+  for (int r = 0; r < EPT; ++r) {
+    const int e = chunk * (kWG * EPT) + r * kWG + threadIdx.x;
     bool visible = false;
     if (e < H.n_enum) {
       D2 D{0, 0};
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
       }
     }
     unsigned long long ball = __ballot(visible);
-    const int word = chunk * (kChunk / 64) + r * (kWG / 64) + wave;
+    const int word = chunk * (kWG * EPT / 64) + r * (kWG / 64) + wave;
     if (lane == 0 && word < vis_words) vis[(size_t)t * vis_words + word] = ball;
   }
   n_new = wave_sum_ll(n_new);
@@ -1109,20 +1112,21 @@ __global__ void k_init_ref(DevState* st, D2 det, SlotW<W>* tab, uint64_t mask, D
 // spin-orbitals), the empty determinant for two (at least one electron).
 template <int W>
 __global__ void k_table_init(SlotW<W>* tab, uint64_t nslots) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots) return;
-  SlotW<W> s;
-  const bool reserved = i == 0xffffffffULL;
-  if constexpr (W == 1) {
-    s.key = reserved ? kKeyMask : 0;
-  } else {
-    s.k0 = reserved ? 0 : kSent;
-    s.k1 = reserved ? kOcc : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    SlotW<W> s;
+    const bool reserved = i == 0xffffffffULL;
+    if constexpr (W == 1) {
+      s.key = reserved ? kKeyMask : 0;
+    } else {
+      s.k0 = reserved ? 0 : kSent;
+      s.k1 = reserved ? kOcc : 0;
+    }
+    s.x = 0.0;
+    s.za = 0;
+    s.zb = 0;
+    tab[i] = s;
   }
-  s.x = 0.0;
-  s.za = 0;
-  s.zb = 0;
-  tab[i] = s;
 }
 
 // ------------------------------------------------------------------ test hooks
@@ -1157,24 +1161,26 @@ __global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, D2 det, double di
 template <int W>
 __global__ void k_table_compact(const SlotW<W>* tab, uint64_t nslots, uint64_t* odet, double* ox, int64_t* oz,
                                 unsigned long long* count, long long cap) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nslots || i == 0xffffffffULL) return;
-  SlotW<W> s = tab[i];
-  D2 d;
-  if constexpr (W == 1) {
-    if (s.key == 0) return;
-    d = D2{s.key & kKeyMask, 0};
-  } else {
-    if (s.k1 == 0) return;
-    d = D2{s.k0, s.k1 & kHiMask};
-  }
-  unsigned long long pos = atomicAdd(count, 1ULL);
-  if ((long long)pos < cap) {
-    odet[2 * pos] = d.lo;
-    odet[2 * pos + 1] = d.hi;
-    ox[pos] = s.x;
-    oz[2 * pos] = s.za;
-    oz[2 * pos + 1] = s.zb;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    if (i == 0xffffffffULL) continue;
+    const SlotW<W> s = tab[i];
+    D2 d;
+    if constexpr (W == 1) {
+      if (s.key == 0) continue;
+      d = D2{s.key & kKeyMask, 0};
+    } else {
+      if (s.k1 == 0) continue;
+      d = D2{s.k0, s.k1 & kHiMask};
+    }
+    unsigned long long pos = atomicAdd(count, 1ULL);
+    if ((long long)pos < cap) {
+      odet[2 * pos] = d.lo;
+      odet[2 * pos + 1] = d.hi;
+      ox[pos] = s.x;
+      oz[2 * pos] = s.za;
+      oz[2 * pos + 1] = s.zb;
+    }
   }
 }
 
@@ -1195,16 +1201,29 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st
   MC_W(nw, hipLaunchKernelGGL(k_submatrix<W>, dim3(k), dim3(kWG), 0, s, H, st, dets, k, Hs, ldh));
 }
 
+template <int W, int EPT>
+void launch_scatter_e(hipStream_t s, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
+                      const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask, uint64_t* vis, int vis_words, int k) {
+  const int nch = (H.n_enum + kWG * EPT - 1) / (kWG * EPT);
+  if (H.mixed)
+    hipLaunchKernelGGL((k_scatter<true, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
+                       mask, vis, vis_words);
+  else
+    hipLaunchKernelGGL((k_scatter<false, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
+                       mask, vis, vis_words);
+}
+
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                    int nchunks) {
+                    int ept) {
   MC_W(nw, {
-    if (H.mixed)
-      hipLaunchKernelGGL((k_scatter<true, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
-                         (SlotW<W>*)tab, mask, vis, vis_words);
-    else
-      hipLaunchKernelGGL((k_scatter<false, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
-                         (SlotW<W>*)tab, mask, vis, vis_words);
+    SlotW<W>* tb = (SlotW<W>*)tab;
+    switch (ept) {
+      case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+      case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+      case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+      default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+    }
   });
 }
 
@@ -1259,9 +1278,9 @@ void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, vo
 template <int W>
 void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
                    const SlotW<W>* tb, D2* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0,
-                   PickEnt* pk1, int* pc0, int* pc1) {
+                   PickEnt* pk1, int* pc0, int* pc1, bool tree) {
   {
-    if ((long long)k * M <= kSortCap) {
+    if (!tree && (long long)k * M <= kSortCap) {
       // One workgroup sorts all candidates: the smallest power of two of at
       // least k * M entries (at least one wave); four entries per thread from
       // 256 entries up.
@@ -1327,9 +1346,9 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
 
 void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,
                  const void* tab, D2* pdet, uint32_t* pslot, double* px, double* pz, PickEnt* pk0, PickEnt* pk1,
-                 int* pc0, int* pc1) {
+                 int* pc0, int* pc1, bool tree) {
   MC_W(nw, launch_pick_w<W>(s, st, k, M, cand, cand_cnt, (const SlotW<W>*)tab, pdet, pslot, px, pz, pk0, pk1, pc0,
-                            pc1));
+                            pc1, tree));
 }
 
 void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uint64_t mask, D2* pdet,
@@ -1339,7 +1358,7 @@ void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uin
 }
 
 void launch_table_init(hipStream_t s, int nw, void* tab, uint64_t nslots) {
-  const uint64_t blocks = (nslots + 255) / 256;
+  const uint64_t blocks = std::min<uint64_t>((nslots + 255) / 256, kSlotBlocks);
   MC_W(nw, hipLaunchKernelGGL(k_table_init<W>, dim3((unsigned)blocks), dim3(256), 0, s, (SlotW<W>*)tab, nslots));
 }
 
@@ -1358,7 +1377,7 @@ void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double d
 
 void launch_table_compact(hipStream_t s, int nw, const void* tab, uint64_t nslots, uint64_t* odet, double* ox,
                           int64_t* oz, unsigned long long* count, long long cap) {
-  const uint64_t blocks = (nslots + 255) / 256;
+  const uint64_t blocks = std::min<uint64_t>((nslots + 255) / 256, kSlotBlocks);
   MC_W(nw, hipLaunchKernelGGL(k_table_compact<W>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<W>*)tab,
                               nslots, odet, ox, oz, count, cap));
 }

@@ -1,0 +1,87 @@
+"""Batching additions: HIP-graph replay of captured iterations and the
+online autotuner.  Both are result-neutral, so every check is exact: graph
+replays against direct launches, and autotuned runs (which pass through
+every scatter and pick launch form) against the oracle.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from mcdfci_amd import Cdfci
+from oracle_ctypes import Oracle
+
+
+def _same_state(a, b):
+    ra, rb = a.report(), b.report()
+    for key in ("energy", "x_size", "z_size", "cols_size", "updates", "scale", "xx", "dx_max", "iterations"):
+        assert ra[key] == rb[key], (key, ra[key], rb[key])
+    da, xa, za = a.table()
+    db, xb, zb = b.table()
+    assert np.array_equal(da, db)
+    assert np.array_equal(xa, xb)
+    assert np.array_equal(za, zb)
+
+
+@pytest.mark.parametrize("name,k,M,z_thr,mem,G", [
+    ("c3_synth", 8, 8, 0.0, 0.05, 5),
+    ("c4", 64, 64, 1e-6, 8.0, 4),
+])
+def test_graph_replay_matches_direct_launches(name, k, M, z_thr, mem, G, workloads):
+    print("case", name, k, M, G, flush=True)
+    path = workloads(name)["fcidump"]
+    kw = dict(num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=mem, threshold=0.0)
+    a = Cdfci(path, **kw)
+    b = Cdfci(path, **kw)
+    a.init()
+    b.init()
+    b.set_graph(G)
+    # several replays with host synchronisation and direct launches between
+    # them (the remainder of each step runs uncaptured)
+    for n in (1, 2 * G + 3, G, 3 * G - 1):
+        a.step(n)
+        b.step(n)
+        _same_state(a, b)
+    # a re-initialisation reallocates the iteration buffers: the graph is
+    # captured again
+    a.init()
+    b.init()
+    a.step(2 * G)
+    b.step(2 * G)
+    _same_state(a, b)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name,k,M,z_thr,mem", [
+    ("c3_synth", 8, 8, 0.0, 0.05),      # one-workgroup and merge-tree picks both eligible
+    ("c4", 64, 64, 1e-6, 8.0),
+    ("c5", 16, 16, 1e-5, 8.0),          # two-word determinants
+])
+def test_autotune_keeps_the_trajectory(name, k, M, z_thr, mem, workloads):
+    print("case", name, k, M, flush=True)
+    path = workloads(name)["fcidump"]
+    g = Cdfci(path, num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=mem, threshold=0.0)
+    g.init()
+    choice = g.autotune(2)
+    print("choice", choice, flush=True)
+    assert choice["elements_per_thread"] in (1, 2, 4, 8)
+    assert choice["iterations"] == 2 * 4 * (2 if k * M <= 4096 else 1)
+    g.set_graph(3)
+    g.step(7)
+    n = choice["iterations"] + 7
+    o = Oracle(path)
+    o.solver_init(k, M, z_thr, g.report()["max_size"], nthreads=8)
+    o.step(n)
+    rg, ro = g.report(), o.report()
+    assert rg["iterations"] == n
+    assert rg["energy"] == ro["energy"]
+    for key in ("x_size", "z_size", "cols_size", "updates", "scale", "xx", "dx_max"):
+        assert rg[key] == ro[key], (key, rg[key], ro[key])
+    gd, gx, gz = g.table()
+    od, ox, oz, _ = o.table()
+    assert np.array_equal(gd, od)
+    assert np.array_equal(gx, ox)
+    assert np.array_equal(gz, oz)
+    g.close()

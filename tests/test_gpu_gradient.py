@@ -27,11 +27,15 @@ def _path(name, workloads, h2o_fcidump):
 
 
 @pytest.mark.parametrize("name,k,iters", [("c1", 8, 600), ("c3_synth", 8, 200), ("c3_hubbard", 4, 200),
-                                          ("h2o", 8, 300)])
+                                          ("h2o", 8, 300),
+                                          # 42 orbitals: two-word determinants
+                                          ("c5", 16, 20)])
 def test_gradient_bit_exact(name, k, iters, workloads, h2o_fcidump):
     print("case", name, k, iters, flush=True)
     path = _path(name, workloads, h2o_fcidump)
-    g = Cdfci(path, num_of_coordinate=k, estimate=8, max_memory=0.05, threshold=0.0)
+    big = name == "c5"
+    g = Cdfci(path, num_of_coordinate=k, estimate=k if big else 8, max_memory=8.0 if big else 0.05, threshold=0.0,
+              z_threshold=1e-5 if big else 0.0)
     g.init()
     g.step(iters)
     e, gh, ge = g.energy_gradient()
