@@ -132,7 +132,7 @@ int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
  * timing enabled are never captured.
  *
  * mcdfci_autotune: runs `rounds` rounds of real iterations (they advance the
- * solver like mcdfci_step), timing the scatter at 1, 2, 4 and 8 column
+ * solver like mcdfci_step), timing the scatter at 1, 2, 4, 8 and 16 column
  * elements per thread and the pick with one-workgroup and merge-tree sorts,
  * and keeps the fastest of each (median kernel time).  choice (may be NULL)
  * receives {elements per thread, 1 if merge-tree pick, iterations run}. */

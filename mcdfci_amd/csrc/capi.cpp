@@ -616,10 +616,11 @@ int mcdfci_autotune(mcdfci_ctx* c, int rounds, int64_t* choice) {
   if (!c || rounds < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
   HIPCHECK(c, hipSetDevice(c->device));
-  const int epts[4] = {1, 2, 4, 8};
+  constexpr int kNE = 5;
+  const int epts[kNE] = {1, 2, 4, 8, 16};
   const bool can_single = (long long)c->k * c->M <= kSortCap;
   const int npick = can_single ? 2 : 1;
-  std::vector<double> ts[4], tp[2];
+  std::vector<double> ts[kNE], tp[2];
   const bool timing = c->timing;
   double kt[6];
   std::copy(c->ktimes, c->ktimes + 6, kt);
@@ -627,7 +628,7 @@ int mcdfci_autotune(mcdfci_ctx* c, int rounds, int64_t* choice) {
   c->timing = true;
   int64_t iters = 0;
   for (int r = 0; r < rounds; ++r)
-    for (int e = 0; e < 4; ++e)
+    for (int e = 0; e < kNE; ++e)
       for (int p = 0; p < npick; ++p) {
         c->scatter_ept = epts[e];
         c->pick_tree = !can_single || p == 1;
@@ -643,7 +644,7 @@ int mcdfci_autotune(mcdfci_ctx* c, int rounds, int64_t* choice) {
   std::copy(kt, kt + 6, c->ktimes);
   c->ktimed = kn;
   int best = 2;
-  for (int e = 0; e < 4; ++e)
+  for (int e = 0; e < kNE; ++e)
     if (median(ts[e]) < median(ts[best])) best = e;
   c->scatter_ept = epts[best];
   c->pick_tree = !can_single || median(tp[1]) < median(tp[0]);

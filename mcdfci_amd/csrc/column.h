@@ -51,37 +51,49 @@ struct ColCtxW {
 };
 using ColCtx = ColCtxW<1>;
 
-__device__ inline void fill_pairs(uint16_t* out, int n, int tid, int nthreads) {
-  for (int x = tid; x < n; x += nthreads) {
-    int base = x * n - x * (x + 1) / 2;
-    for (int y = x + 1; y < n; ++y) out[base + (y - x - 1)] = (uint16_t)(x | (y << 8));
-  }
+// Pairs x | y << 8 (x < y < n) of list positions, in the order x-major,
+// y ascending; entry of pair (x, y) at x * n - x * (x + 1) / 2 + y - x - 1.
+__device__ __forceinline__ void fill_pairs_row(uint16_t* out, int n, int x) {
+  if (x >= n) return;
+  const int base = x * n - x * (x + 1) / 2;
+  for (int y = x + 1; y < n; ++y) out[base + (y - x - 1)] = (uint16_t)(x | (y << 8));
 }
 
+// Decodes det into the context.  Every spin-orbital s is placed by one
+// thread at its rank: occ[#occupied below s], and in its spin's occupied or
+// virtual list at #(same-spin occupied below s) or #(same-spin virtual
+// below s); then the four pair lists are filled side by side, one row x
+// per thread (a 256-thread workgroup covers all four lists at once).
 template <int W>
 __device__ inline void setup_ctx(ColCtxW<W>& C, const DevHam& H, const D2& det) {
-  if (threadIdx.x == 0) {
-    C.det = det;
-    int ne = 0, na = 0, nb = 0, nva = 0, nvb = 0;
-    for (int s = 0; s < H.ns; ++s) {
-      bool o = dtest<W>(det, s);
-      int p = s >> 1;
-      if (o) C.occ[ne++] = (uint8_t)s;
-      if (s & 1) {
-        if (o) C.ob[nb++] = (uint8_t)p;
-        else C.vb[nvb++] = (uint8_t)p;
-      } else {
-        if (o) C.oa[na++] = (uint8_t)p;
-        else C.va[nva++] = (uint8_t)p;
-      }
+  const int tid = threadIdx.x;
+  const D2 alpha = dand(det, D2{0x5555555555555555ULL, 0x5555555555555555ULL});
+  const D2 beta = dand(det, D2{0xAAAAAAAAAAAAAAAAULL, 0xAAAAAAAAAAAAAAAAULL});
+  for (int s = tid; s < H.ns; s += blockDim.x) {
+    const bool o = dtest<W>(det, s);
+    const int p = s >> 1;
+    if (o) C.occ[dbelow<W>(det, s)] = (uint8_t)s;
+    const int same = dbelow<W>((s & 1) ? beta : alpha, s);  // same-spin occupied below s
+    if (s & 1) {
+      if (o) C.ob[same] = (uint8_t)p;
+      else C.vb[p - same] = (uint8_t)p;
+    } else {
+      if (o) C.oa[same] = (uint8_t)p;
+      else C.va[p - same] = (uint8_t)p;
     }
-    C.ne = ne;
+  }
+  if (tid == 0) {
+    C.det = det;
+    C.ne = dpopc<W>(det);
   }
   __syncthreads();
-  fill_pairs(C.poa, H.na, threadIdx.x, blockDim.x);
-  fill_pairs(C.pob, H.nb, threadIdx.x, blockDim.x);
-  fill_pairs(C.pva, H.nva, threadIdx.x, blockDim.x);
-  fill_pairs(C.pvb, H.nvb, threadIdx.x, blockDim.x);
+  for (int r = tid; r < 4 * 64; r += blockDim.x) {
+    const int list = r >> 6, x = r & 63;
+    if (list == 0) fill_pairs_row(C.poa, H.na, x);
+    else if (list == 1) fill_pairs_row(C.pob, H.nb, x);
+    else if (list == 2) fill_pairs_row(C.pva, H.nva, x);
+    else fill_pairs_row(C.pvb, H.nvb, x);
+  }
   __syncthreads();
 }
 

@@ -1222,6 +1222,7 @@ void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const 
       case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
       case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
       case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+      case 16: launch_scatter_e<W, 16>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
       default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
     }
   });

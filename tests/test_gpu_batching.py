@@ -66,8 +66,8 @@ def test_autotune_keeps_the_trajectory(name, k, M, z_thr, mem, workloads):
     g.init()
     choice = g.autotune(2)
     print("choice", choice, flush=True)
-    assert choice["elements_per_thread"] in (1, 2, 4, 8)
-    assert choice["iterations"] == 2 * 4 * (2 if k * M <= 4096 else 1)
+    assert choice["elements_per_thread"] in (1, 2, 4, 8, 16)
+    assert choice["iterations"] == 2 * 5 * (2 if k * M <= 4096 else 1)
     g.set_graph(3)
     g.step(7)
     n = choice["iterations"] + 7
