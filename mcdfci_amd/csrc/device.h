@@ -27,6 +27,11 @@ constexpr int kWG = 256;               // threads per workgroup
 constexpr int kEPT = 4;                // column elements per thread
 constexpr int kChunk = kWG * kEPT;     // column elements per workgroup
 constexpr int kSortCap = 4096;         // entries sorted by one workgroup
+// Iteration counters (new keys, updates, column elements) are accumulated in
+// kCtrStripes stripes, one 128-B line apart, so that the scatter's per-
+// workgroup atomics spread over L2 channels instead of queueing on one line.
+constexpr int kCtrStripes = 64;
+constexpr int kCtrStride = 16;  // int64 per stripe
 // Workgroups of the whole-table sweeps (init, dump, compaction), which loop
 // over the slots: a dispatch's grid is limited to 2^32 work-items, the table
 // may hold 2^32 slots.
@@ -87,7 +92,6 @@ struct DevState {
   int64_t xz_hi;   // x.z as a fixed-point 128-bit value
   uint64_t xz_lo;
   long long x_size, z_size, updates, cols_size, iterations;
-  long long it_new, it_upd, it_cols;  // counters of the running iteration
   int k_cur;       // columns of the running update
   int k_prev;      // columns whose candidates feed the next pick
   int error;

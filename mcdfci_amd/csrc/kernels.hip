@@ -369,7 +369,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int& total) {
 template <bool MIXED, int W, int EPT>
 __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                  const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
-                                                 uint64_t* vis, int vis_words) {
+                                                 uint64_t* vis, int vis_words, long long* ctr) {
   __shared__ ColCtxW<W> C;
   if (st->error) return;
   const int t = blockIdx.y;
@@ -499,10 +499,18 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
   n_new = wave_sum_ll(n_new);
   n_upd = wave_sum_ll(n_upd);
   n_valid = wave_sum_ll(n_valid);
+  __shared__ long long wsum[kWG / 64][3];
   if (lane == 0) {
-    if (n_new) atomicAdd((unsigned long long*)&st->it_new, (unsigned long long)n_new);
-    if (n_upd) atomicAdd((unsigned long long*)&st->it_upd, (unsigned long long)n_upd);
-    if (n_valid) atomicAdd((unsigned long long*)&st->it_cols, (unsigned long long)n_valid);
+    wsum[wave][0] = n_new;
+    wsum[wave][1] = n_upd;
+    wsum[wave][2] = n_valid;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    long long v = 0;
+    for (int w = 0; w < kWG / 64; ++w) v += wsum[w][threadIdx.x];
+    const int stripe = (int)((blockIdx.y * gridDim.x + blockIdx.x) % kCtrStripes);
+    if (v) atomicAdd((unsigned long long*)&ctr[stripe * kCtrStride + threadIdx.x], (unsigned long long)v);
   }
   if (fail) st->error = ERR_PROBE;
 }
@@ -641,7 +649,7 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
                                                int nchunks, const StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
-                                               const double* dx, int count_iteration) {
+                                               const double* dx, int count_iteration, long long* ctr) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
   __shared__ uint64_t kc[W == 2 ? T * E : 1];
@@ -787,16 +795,30 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
     last = (tk == (unsigned int)(k - 1));
   }
   __syncthreads();
-  if (!last || threadIdx.x != 0) return;
+  if (!last) return;
   __threadfence();
+  // Iteration counters: wave 0 sums the stripes and clears them.
+  long long c_new = 0, c_upd = 0, c_cols = 0;
+  if (threadIdx.x < 64) {
+    long long* cs = ctr + threadIdx.x * kCtrStride;
+    c_new = cs[0];
+    c_upd = cs[1];
+    c_cols = cs[2];
+    cs[0] = 0;
+    cs[1] = 0;
+    cs[2] = 0;
+    c_new = wave_sum_ll(c_new);
+    c_upd = wave_sum_ll(c_upd);
+    c_cols = wave_sum_ll(c_cols);
+  }
+  if (threadIdx.x != 0) return;
   i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
   for (int c = 0; c < k; ++c) xz += (((i128)col_xz[2 * c]) << 64) | (i128)(uint64_t)col_xz[2 * c + 1];
   st->xz_hi = (int64_t)(xz >> 64);
   st->xz_lo = (uint64_t)xz;
-  st->z_size += st->it_new;
-  st->updates = st->it_upd;
-  st->cols_size = st->it_cols;
-  st->it_new = st->it_upd = st->it_cols = 0;
+  st->z_size += c_new;
+  st->updates = c_upd;
+  st->cols_size = c_cols;
   st->ticket = 0;
   st->k_prev = k;
   double dm = 0.0;
@@ -1203,27 +1225,28 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st
 
 template <int W, int EPT>
 void launch_scatter_e(hipStream_t s, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
-                      const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask, uint64_t* vis, int vis_words, int k) {
+                      const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
+                      long long* ctr) {
   const int nch = (H.n_enum + kWG * EPT - 1) / (kWG * EPT);
   if (H.mixed)
     hipLaunchKernelGGL((k_scatter<true, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words);
+                       mask, vis, vis_words, ctr);
   else
     hipLaunchKernelGGL((k_scatter<false, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words);
+                       mask, vis, vis_words, ctr);
 }
 
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                    int ept) {
+                    int ept, long long* ctr) {
   MC_W(nw, {
     SlotW<W>* tb = (SlotW<W>*)tab;
     switch (ept) {
-      case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
-      case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
-      case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
-      case 16: launch_scatter_e<W, 16>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
-      default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k); break;
+      case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
+      case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
+      case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
+      case 16: launch_scatter_e<W, 16>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
+      default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
     }
   });
 }
@@ -1243,14 +1266,14 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
 
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, const StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
-                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration) {
+                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr) {
   // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
   // entries, then four entries per thread).  Beyond: the streaming merge
   // with a window of 2 * Mp entries.
   const int mw = M < kChunk ? M : kChunk;
 #define MC_MERGE(E_, T_, S_)                                                                                 \
   hipLaunchKernelGGL((k_merge<E_, T_, S_, W>), dim3(k), dim3(T_), 0, s, st, pslot, (SlotW<W>*)tab, M, nchunks, \
-                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration)
+                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration, ctr)
   MC_W(nw, {
     if ((long long)nchunks * mw <= kSortCap) {
       int w = kWG;
