@@ -134,6 +134,10 @@ def main():
     ap.add_argument("--skip-ttt", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=20, help="iterations per captured HIP graph (0: off)")
+    ap.add_argument("--sweep-k", default="8,256",
+                    help="further num_of_coordinate values for the throughput sweep (empty: none)")
+    ap.add_argument("--ttt-extra-k", default="256",
+                    help="further batched num_of_coordinate values timed to the threshold (empty: none)")
     ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (byte-counter calibration)")
@@ -171,6 +175,11 @@ def main():
     # 1. throughput of the mixed + batched form
     mb = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem, threshold=0.0,
                precision=PREC_MIXED)
+    # 0. random-slot access ceiling on the same table (before init clears it)
+    n_ops = 1 << 25
+    ceiling = {name: n_ops / mb.access_ceiling(mode, n_ops)
+               for mode, name in ((0, "insert_ops_per_s"), (1, "update_ops_per_s"), (2, "read_ops_per_s"))}
+    log(f"access ceiling on the {max_mem:g} GB table: " + json.dumps(ceiling))
     mb.init()
     choice = mb.autotune(args.tune_rounds) if args.tune_rounds else None
     log(f"autotune: {choice}")
@@ -202,6 +211,30 @@ def main():
     mb.close()
     torch.cuda.synchronize()
 
+    # 2b. throughput sweep over num_of_coordinate (same machinery, 100 steps)
+    sweep = [{"k": k, "precision": "mixed", "coord_updates_per_s": value, "ms_per_iteration": ms_per_step,
+              "launch": choice}]
+    for ks in [int(v) for v in args.sweep_k.split(",") if v.strip()]:
+        for prec, pname in ((PREC_MIXED, "mixed"), (PREC_FP64, "fp64")):
+            if pname == "fp64" and ks != args.plain_k:
+                continue
+            c = Cdfci(fcid, num_of_coordinate=ks, estimate=ks, z_threshold=z_thr, max_memory=max_mem,
+                      threshold=0.0, precision=prec)
+            c.init()
+            ch = c.autotune(args.tune_rounds) if args.tune_rounds else None
+            c.set_graph(args.graph)
+            c.step(args.warmup)
+            c.report()
+            n = 100
+            t0 = time.perf_counter()
+            c.step(n)
+            c.report()
+            d = time.perf_counter() - t0
+            c.close()
+            sweep.append({"k": ks, "precision": pname, "coord_updates_per_s": ks * n / d,
+                          "ms_per_iteration": 1e3 * d / n, "launch": ch})
+            log(f"sweep: {sweep[-1]}")
+
     # 3. time to |dE| < 1e-5 Ha
     ttt = None
     if not args.skip_ttt:
@@ -218,6 +251,18 @@ def main():
         e_ref = tp[-1][2]
         t_plain, i_plain = first_within(tp, e_ref, 1e-5)
         t_mb, i_mb = first_within(tm, e_ref, 1e-5)
+        extra = []
+        for kx in [int(v) for v in args.ttt_extra_k.split(",") if v.strip()]:
+            cx = Cdfci(fcid, num_of_coordinate=kx, estimate=kx, z_threshold=z_thr, max_memory=max_mem,
+                       threshold=0.0, precision=PREC_MIXED)
+            cx.init()
+            tx, stop_x = run_for(cx, args.ttt_seconds, max(1, 1280 // kx), args.tune_rounds, args.graph)
+            cx.close()
+            t_x, i_x = first_within(tx, e_ref, 1e-5)
+            extra.append({"batched_k": kx, "mixed_batched_s": t_x, "mixed_batched_iterations": i_x,
+                          "speedup": (t_plain / t_x) if (t_plain and t_x) else None,
+                          "mixed_final_energy": tx[-1][2], "mixed_stop": stop_x, "mixed_last": tx[-1]})
+            log(f"time-to-threshold k={kx}: {extra[-1]}")
         # energy of each form at equal wall time (the mixed run's last report)
         t_last = tm[-1][0]
         e_plain_at = min(tp, key=lambda r: abs(r[0] - t_last))[2]
@@ -239,6 +284,7 @@ def main():
             "plain_last": tp[-1],
             "mixed_stop": stop_m,
             "mixed_last": tm[-1],
+            "other_batched_k": extra,
         }
         log("time-to-threshold: " + json.dumps(ttt))
 
@@ -282,6 +328,12 @@ def main():
                      "kernel": "k_scatter", "elements_per_launch": elems,
                      "bytes_per_element": BYTES_PER_ELEMENT, "avg_launch_us": 1e6 * scatter_s},
         "kernel_ms_per_iteration": {a: 1e3 * b / max(nt, 1) for a, b in kt.items()},
+        "access_ceiling": dict(ceiling, **{
+            "scatter_elements_per_s": elems / scatter_s,
+            "scatter_frac_of_update": elems / scatter_s / ceiling["update_ops_per_s"],
+            "pass2_elements_per_s": elems / (kt["pass2"] / max(nt, 1)),
+            "pass2_frac_of_read": elems / (kt["pass2"] / max(nt, 1)) / ceiling["read_ops_per_s"]}),
+        "sweep": sweep,
         "time_to_threshold": ttt,
         "cpu_baseline": cpu,
     }

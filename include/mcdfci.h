@@ -139,6 +139,25 @@ int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
 int mcdfci_set_graph(mcdfci_ctx* ctx, int iterations_per_graph);
 int mcdfci_autotune(mcdfci_ctx* ctx, int rounds, int64_t* choice);
 
+/* Checkpoint and resume (no counterpart in the reference, which restarts
+ * from the Hartree-Fock determinant only, src/solver.h:853-873): save the
+ * wavefunction table ({D: x_D, z_D} with z exact), the scalars, counters and
+ * candidate lists to a file; load restores them into a context created for
+ * the same Hamiltonian and num_of_coordinate / estimate / precision /
+ * threshold options (MCDFCI_E_INPUT otherwise), in place of mcdfci_init.
+ * Stepping a loaded context continues the saved run bit for bit.  The
+ * loaded run keeps its own table size (max_memory). */
+int mcdfci_checkpoint_save(mcdfci_ctx* ctx, const char* path);
+int mcdfci_checkpoint_load(mcdfci_ctx* ctx, const char* path);
+
+/* Measurement (no counterpart in the reference): the device time of n_ops
+ * random-slot operations on this context's own table -- mode 0 a new-key
+ * insert (key load, CAS, two 64-bit atomic adds), 1 an update (key load,
+ * two atomic adds), 2 a read of key, x and z -- the access ceiling of the
+ * column scatter and pass 2 on the same footprint.  Overwrites the table:
+ * mcdfci_init must follow before mcdfci_step. */
+int mcdfci_access_ceiling(mcdfci_ctx* ctx, int mode, int64_t n_ops, double* seconds);
+
 /* Phase times in seconds (pick, submatrix, columns, eig, update_x, update_z),
  * the step_timer of src/solver.h:683,704-754; device phases are measured
  * with HIP events when timing is enabled (mcdfci_set_timing). */

@@ -91,6 +91,9 @@ SIGNATURES = {
     "mcdfci_kernel_times": (C.c_int, [C.c_void_p, _f64p, C.POINTER(C.c_int64)]),
     "mcdfci_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mcdfci_autotune": (C.c_int, [C.c_void_p, C.c_int, _i64p]),
+    "mcdfci_access_ceiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double)]),
+    "mcdfci_checkpoint_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "mcdfci_checkpoint_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mcdfci_column": (C.c_int, [C.c_void_p, _u64p, _u64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_submatrix": (C.c_int, [C.c_void_p, C.c_int, _u64p, _f64p]),
     "mcdfci_eig_update": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
@@ -213,6 +216,21 @@ class Cdfci:
         ch = np.zeros(3, np.int64)
         self._check(self.lib.mcdfci_autotune(self.h, int(rounds), ch))
         return {"elements_per_thread": int(ch[0]), "tree_pick": bool(ch[1]), "iterations": int(ch[2])}
+
+    def save(self, path):
+        """Write a checkpoint (table, scalars, counters, candidates) to path."""
+        self._check(self.lib.mcdfci_checkpoint_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        """Resume from a checkpoint written by save() (in place of init())."""
+        self._check(self.lib.mcdfci_checkpoint_load(self.h, os.fsencode(path)))
+
+    def access_ceiling(self, mode, n_ops=1 << 25):
+        """Device seconds for n_ops random-slot operations on this context's table
+        (mode 0 insert, 1 update, 2 read).  Overwrites the table: call init() after."""
+        sec = C.c_double()
+        self._check(self.lib.mcdfci_access_ceiling(self.h, int(mode), int(n_ops), C.byref(sec)))
+        return sec.value
 
     def column(self, det):
         d = np.zeros(2, np.uint64)

@@ -603,6 +603,247 @@ int mcdfci_step(mcdfci_ctx* c, int64_t n) {
   return MCDFCI_OK;
 }
 
+int mcdfci_access_ceiling(mcdfci_ctx* c, int mode, int64_t n_ops, double* seconds) {
+  if (!c || !seconds || mode < 0 || mode > 2 || n_ops < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  unsigned long long* sink = nullptr;
+  int rc = dalloc(c, &sink, 1);
+  if (rc) return rc;
+  c->ready = false;  // the table no longer holds a wavefunction
+  drop_graph(c);
+  // one untimed pass brings the clocks up; the second is timed
+  for (int pass = 0; pass < 2; ++pass) {
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    launch_access_probe(c->stream, c->nw, c->tab, c->nslots - 1, (long long)n_ops, mode,
+                        0x9e3779b97f4a7c15ULL * (uint64_t)(pass + 1), sink);
+    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+  }
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0.0f;
+  HIPCHECK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  dfree(c, sink);
+  *seconds = ms * 1e-3;
+  return MCDFCI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Checkpoint file: a fixed header, the raw device state, the candidate
+// lists, then the table's entries in chunks of at most kCkChunk (per chunk:
+// the count, two words per determinant, x, and the (zA, zB) pairs).
+constexpr char kCkMagic[8] = {'M', 'C', 'D', 'F', 'C', 'I', 'C', 'K'};
+constexpr int kCkVersion = 1;
+constexpr long long kCkChunk = 1LL << 22;
+
+struct CkHeader {
+  char magic[8];
+  int32_t version, state_bytes, nio, nw;
+  int32_t norb, nelec, ms2, na, nb, k, M, precision;
+  double threshold, core;
+  uint64_t ham_hash;
+  int64_t entries;
+};
+
+uint64_t ham_hash(const SpatialHam& S) {
+  uint64_t h = 0xcbf29ce484222325ULL;
+  auto feed = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ULL;
+  };
+  feed(S.h.data(), S.h.size() * sizeof(double));
+  feed(S.eri.data(), S.eri.size() * sizeof(double));
+  feed(&S.core, sizeof(S.core));
+  return h;
+}
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) fclose(f);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mcdfci_checkpoint_save(mcdfci_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, MCDFCI_E_INPUT, "null argument");
+  if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (s.error) return check_state_error(c, s);
+  // count, then compact the occupied slots
+  unsigned long long* cnt = nullptr;
+  int rc = dalloc(c, &cnt, 1);
+  if (rc) return rc;
+  HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+  launch_table_compact(c->stream, c->nw, c->tab, c->nslots, nullptr, nullptr, nullptr, cnt, 0);
+  unsigned long long m = 0;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
+  uint64_t* od = nullptr;
+  double* ox = nullptr;
+  int64_t* oz = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)cnt, (void*)od, (void*)ox, (void*)oz}) dfree(c, p);
+  };
+  if ((rc = dalloc(c, &od, 2 * m)) || (rc = dalloc(c, &ox, m)) || (rc = dalloc(c, &oz, 2 * m))) {
+    release();
+    return rc;
+  }
+  HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+  launch_table_compact(c->stream, c->nw, c->tab, c->nslots, od, ox, oz, cnt, (long long)m);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> ccnt(c->k);
+  std::vector<CandEnt> cand((size_t)c->k * c->M);
+  HIPCHECK(c, hipMemcpy(ccnt.data(), c->cand_cnt, sizeof(int) * c->k, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(cand.data(), c->cand, sizeof(CandEnt) * cand.size(), hipMemcpyDeviceToHost));
+  FileCloser fc{fopen(path, "wb")};
+  if (!fc.f) {
+    release();
+    return fail(c, MCDFCI_E_INPUT, std::string("cannot open ") + path + " for writing");
+  }
+  CkHeader hd{};
+  std::memcpy(hd.magic, kCkMagic, 8);
+  hd.version = kCkVersion;
+  hd.state_bytes = (int32_t)sizeof(DevState);
+  hd.nio = c->nio;
+  hd.nw = c->nw;
+  hd.norb = c->ham.norb;
+  hd.nelec = c->ham.nelec;
+  hd.ms2 = c->ham.ms2;
+  hd.na = c->dh.na;
+  hd.nb = c->dh.nb;
+  hd.k = c->k;
+  hd.M = c->M;
+  hd.precision = c->opts.precision;
+  hd.threshold = c->opts.threshold;
+  hd.core = c->ham.core;
+  hd.ham_hash = ham_hash(c->ham);
+  hd.entries = (int64_t)m;
+  bool ok = fwrite(&hd, sizeof(hd), 1, fc.f) == 1 && fwrite(&s, sizeof(s), 1, fc.f) == 1 &&
+            fwrite(ccnt.data(), sizeof(int), ccnt.size(), fc.f) == ccnt.size() &&
+            fwrite(cand.data(), sizeof(CandEnt), cand.size(), fc.f) == cand.size();
+  std::vector<uint64_t> hd2;
+  std::vector<double> hx;
+  std::vector<int64_t> hz;
+  for (long long off = 0; ok && off < (long long)m; off += kCkChunk) {
+    const long long n = std::min<long long>(kCkChunk, (long long)m - off);
+    hd2.resize(2 * n);
+    hx.resize(n);
+    hz.resize(2 * n);
+    if (hipMemcpy(hd2.data(), od + 2 * off, 16 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hx.data(), ox + off, 8 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hz.data(), oz + 2 * off, 16 * n, hipMemcpyDeviceToHost) != hipSuccess) {
+      release();
+      return fail(c, MCDFCI_E_HIP, "checkpoint copy failed");
+    }
+    const int64_t nn = n;
+    ok = fwrite(&nn, sizeof(nn), 1, fc.f) == 1 && fwrite(hd2.data(), 8, 2 * n, fc.f) == (size_t)(2 * n) &&
+         fwrite(hx.data(), 8, n, fc.f) == (size_t)n && fwrite(hz.data(), 8, 2 * n, fc.f) == (size_t)(2 * n);
+  }
+  release();
+  if (!ok) return fail(c, MCDFCI_E_INPUT, std::string("write to ") + path + " failed");
+  return MCDFCI_OK;
+}
+
+int mcdfci_checkpoint_load(mcdfci_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  FileCloser fc{fopen(path, "rb")};
+  if (!fc.f) return fail(c, MCDFCI_E_INPUT, std::string("cannot open ") + path);
+  CkHeader hd{};
+  DevState s{};
+  if (fread(&hd, sizeof(hd), 1, fc.f) != 1 || std::memcmp(hd.magic, kCkMagic, 8) != 0 ||
+      hd.version != kCkVersion || hd.state_bytes != (int32_t)sizeof(DevState))
+    return fail(c, MCDFCI_E_INPUT, std::string(path) + " is not a checkpoint of this library version");
+  if (hd.nio != c->nio || hd.nw != c->nw || hd.norb != c->ham.norb || hd.nelec != c->ham.nelec ||
+      hd.ms2 != c->ham.ms2 || hd.k != c->k || hd.M != c->M || hd.precision != c->opts.precision ||
+      hd.threshold != c->opts.threshold || hd.ham_hash != ham_hash(c->ham))
+    return fail(c, MCDFCI_E_INPUT,
+                "the checkpoint was written for another Hamiltonian or other num_of_coordinate / estimate / "
+                "precision / threshold options");
+  if (fread(&s, sizeof(s), 1, fc.f) != 1) return fail(c, MCDFCI_E_INPUT, "truncated checkpoint");
+  std::vector<int> ccnt(c->k);
+  std::vector<CandEnt> cand((size_t)c->k * c->M);
+  if (fread(ccnt.data(), sizeof(int), ccnt.size(), fc.f) != ccnt.size() ||
+      fread(cand.data(), sizeof(CandEnt), cand.size(), fc.f) != cand.size())
+    return fail(c, MCDFCI_E_INPUT, "truncated checkpoint");
+  if ((double)s.z_size > 0.79 * c->max_size || hd.entries > (int64_t)(0.79 * c->max_size) + c->k + 16)
+    return fail(c, MCDFCI_E_TABLE_FULL, "the checkpoint does not fit this context's wavefunction table");
+  c->ready = false;
+  set_sector(c->dh, hd.na, hd.nb);
+  drop_graph(c);
+  int rc = alloc_iteration(c);
+  if (rc) return rc;
+  launch_table_init(c->stream, c->nw, c->tab, c->nslots);
+  uint64_t* dd2 = nullptr;
+  double* dxx = nullptr;
+  int64_t* dz2 = nullptr;
+  int* dfail = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)dd2, (void*)dxx, (void*)dz2, (void*)dfail}) dfree(c, p);
+  };
+  const long long cap = std::min<long long>(kCkChunk, std::max<long long>(hd.entries, 1));
+  if ((rc = dalloc(c, &dd2, 2 * cap)) || (rc = dalloc(c, &dxx, cap)) || (rc = dalloc(c, &dz2, 2 * cap)) ||
+      (rc = dalloc(c, &dfail, 1))) {
+    release();
+    return rc;
+  }
+  HIPCHECK(c, hipMemsetAsync(dfail, 0, sizeof(int), c->stream));
+  std::vector<uint64_t> hd2;
+  std::vector<double> hx;
+  std::vector<int64_t> hz;
+  for (long long off = 0; off < hd.entries;) {
+    int64_t n = 0;
+    if (fread(&n, sizeof(n), 1, fc.f) != 1 || n < 1 || n > cap || off + n > hd.entries) {
+      release();
+      return fail(c, MCDFCI_E_INPUT, "truncated or corrupt checkpoint");
+    }
+    hd2.resize(2 * n);
+    hx.resize(n);
+    hz.resize(2 * n);
+    if (fread(hd2.data(), 8, 2 * n, fc.f) != (size_t)(2 * n) || fread(hx.data(), 8, n, fc.f) != (size_t)n ||
+        fread(hz.data(), 8, 2 * n, fc.f) != (size_t)(2 * n)) {
+      release();
+      return fail(c, MCDFCI_E_INPUT, "truncated checkpoint");
+    }
+    // the copies are synchronous, so the host buffers may be refilled next
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(dd2, hd2.data(), 16 * n, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(dxx, hx.data(), 8 * n, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(dz2, hz.data(), 16 * n, hipMemcpyHostToDevice));
+    launch_table_insert(c->stream, c->nw, c->tab, c->nslots - 1, n, dd2, dxx, dz2, dfail);
+    off += n;
+  }
+  HIPCHECK(c, hipMemcpyAsync(c->cand_cnt, ccnt.data(), sizeof(int) * c->k, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->cand, cand.data(), sizeof(CandEnt) * cand.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  launch_cand_resolve(c->stream, c->nw, c->tab, c->nslots - 1, c->cand, c->cand_cnt, c->k, c->M, dfail);
+  s.max_size = c->max_size;
+  s.ticket = 0;
+  HIPCHECK(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  int f = 0;
+  HIPCHECK(c, hipMemcpy(&f, dfail, sizeof(int), hipMemcpyDeviceToHost));
+  release();
+  if (f) return fail(c, MCDFCI_E_TABLE_FULL, "the checkpoint's entries do not fit the table");
+  c->opts.z_threshold = s.z_thr;
+  for (double& t : c->timers) t = 0.0;
+  for (double& t : c->ktimes) t = 0.0;
+  c->ktimed = 0;
+  c->ready = true;
+  return MCDFCI_OK;
+}
+
 int mcdfci_set_graph(mcdfci_ctx* c, int iterations_per_graph) {
   if (!c || iterations_per_graph < 0) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   HIPCHECK(c, hipSetDevice(c->device));

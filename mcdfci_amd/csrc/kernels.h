@@ -31,6 +31,16 @@ void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEn
 void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uint64_t mask, D2* pdet,
                      uint32_t* pslot, double* px, double* pz, double* dx);
 void launch_table_init(hipStream_t s, int nw, void* tab, uint64_t nslots);
+// Checkpoint reload: insert distinct (det, x, zA, zB) entries into an empty
+// table; re-resolve the candidates' slots.  *fail is set on a full table or
+// a candidate missing from it.
+void launch_table_insert(hipStream_t s, int nw, void* tab, uint64_t mask, long long n, const uint64_t* det2,
+                         const double* x, const int64_t* z2, int* fail);
+void launch_cand_resolve(hipStream_t s, int nw, const void* tab, uint64_t mask, CandEnt* cand, const int* cnt,
+                         int k, int M, int* fail);
+// Random-slot access ceiling (mode 0 insert, 1 update, 2 read); clobbers the table.
+void launch_access_probe(hipStream_t s, int nw, void* tab, uint64_t mask, long long n, int mode, uint64_t seed,
+                         unsigned long long* sink);
 // Column elements of det: two words per determinant in odet.
 void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double diag, uint64_t* odet, double* oval,
                         unsigned long long* count, long long cap);

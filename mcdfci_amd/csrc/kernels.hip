@@ -1151,6 +1151,95 @@ __global__ void k_table_init(SlotW<W>* tab, uint64_t nslots) {
   }
 }
 
+// ------------------------------------------------------------------ checkpoint load
+// Inserts n distinct determinants (two words each in det2) with their x and
+// z (zA, zB pairs) into an empty table.  Keys are unique, so a slot claimed
+// by another thread is never this thread's: no waiting, the probe moves on.
+template <int W>
+__global__ void k_table_insert(SlotW<W>* tab, uint64_t mask, long long n, const uint64_t* det2, const double* x,
+                               const int64_t* z2, int* fail) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const D2 D{det2[2 * i], det2[2 * i + 1]};
+    uint64_t b = dhash<W>(D) & mask & ~3ULL;
+    SlotW<W>* hit = nullptr;
+    for (uint64_t m = 0; m <= mask && !hit; m += 4) {
+      for (int jj = 0; jj < 4 && !hit; ++jj) {
+        SlotW<W>* sl = &tab[b + jj];
+        uint64_t expect = 0;
+        if constexpr (W == 1) {
+          if (__hip_atomic_compare_exchange_strong(&sl->key, &expect, D.lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+            hit = sl;
+        } else {
+          if (__hip_atomic_compare_exchange_strong(&sl->k1, &expect, D.hi | kOcc, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            sl->k0 = D.lo;
+            hit = sl;
+          }
+        }
+      }
+      b = (b + 4) & mask;
+    }
+    if (!hit) {
+      *fail = 1;
+      continue;
+    }
+    hit->x = x[i];
+    hit->za = z2[2 * i];
+    hit->zb = z2[2 * i + 1];
+  }
+}
+
+// Slot of every candidate (cand[t * M + i], i < cnt[t]) after a reload.
+template <int W>
+__global__ void k_cand_resolve(const SlotW<W>* tab, uint64_t mask, CandEnt* cand, const int* cnt, int k, int M,
+                               int* fail) {
+  const int t = blockIdx.x;
+  if (t >= k) return;
+  for (int i = threadIdx.x; i < cnt[t]; i += blockDim.x) {
+    CandEnt& e = cand[(size_t)t * M + i];
+    const long long si = find_slot<W>(tab, mask, D2{e.det, e.det1});
+    if (si < 0) *fail = 1;
+    else e.slot = (uint32_t)si;
+  }
+}
+
+// ------------------------------------------------------------------ access ceiling
+// Random-slot access rates on the table's own footprint, the ceiling the
+// scatter and pass 2 are measured against.  n operations, operation i on
+// slot mix64(i + seed) & mask.  mode 0 (insert): relaxed load of the key
+// word, CAS of an empty key, two 64-bit atomic adds; mode 1 (update): load
+// of the key word and the two atomic adds; mode 2 (read): load of the key,
+// x and both z words.  A per-thread sum of what was read goes to *sink so
+// that no load is dead.
+template <int W>
+__global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t mask, long long n, int mode,
+                                                      uint64_t seed, unsigned long long* sink) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  uint64_t acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    SlotW<W>* sl = &tab[mix64((uint64_t)i + seed) & mask];
+    uint64_t* kw;
+    if constexpr (W == 1) kw = &sl->key;
+    else kw = &sl->k1;
+    if (mode == 2) {
+      acc += rload(kw) + (uint64_t)sl->za + (uint64_t)sl->zb + bitsd(sl->x);
+      continue;
+    }
+    const uint64_t k = rload(kw);
+    acc += k;
+    if (mode == 0 && k == 0) {
+      uint64_t expect = 0;
+      __hip_atomic_compare_exchange_strong(kw, &expect, ((uint64_t)i & kHiMask) | kOcc, __ATOMIC_RELAXED,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_fetch_add(&sl->za, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(&sl->zb, (int64_t)3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (acc == 0x5a5a5a5a5a5a5a5aULL) atomicAdd(sink, 1ULL);
+}
+
 // ------------------------------------------------------------------ test hooks
 template <bool MIXED, int W>
 __global__ __launch_bounds__(kWG) void k_column_dump(DevHam H, D2 det, double diag, uint64_t* odet,
@@ -1379,6 +1468,26 @@ void launch_init_ref(hipStream_t s, int nw, DevState* st, D2 det, void* tab, uin
                      uint32_t* pslot, double* px, double* pz, double* dx) {
   MC_W(nw, hipLaunchKernelGGL(k_init_ref<W>, dim3(1), dim3(64), 0, s, st, det, (SlotW<W>*)tab, mask, pdet, pslot,
                               px, pz, dx));
+}
+
+void launch_access_probe(hipStream_t s, int nw, void* tab, uint64_t mask, long long n, int mode, uint64_t seed,
+                         unsigned long long* sink) {
+  MC_W(nw, hipLaunchKernelGGL(k_access_probe<W>, dim3(4096), dim3(kWG), 0, s, (SlotW<W>*)tab, mask, n, mode, seed,
+                              sink));
+}
+
+void launch_table_insert(hipStream_t s, int nw, void* tab, uint64_t mask, long long n, const uint64_t* det2,
+                         const double* x, const int64_t* z2, int* fail) {
+  const long long blocks = std::min<long long>((n + 255) / 256, 1 << 16);
+  if (blocks < 1) return;
+  MC_W(nw, hipLaunchKernelGGL(k_table_insert<W>, dim3((unsigned)blocks), dim3(256), 0, s, (SlotW<W>*)tab, mask, n,
+                              det2, x, z2, fail));
+}
+
+void launch_cand_resolve(hipStream_t s, int nw, const void* tab, uint64_t mask, CandEnt* cand, const int* cnt,
+                         int k, int M, int* fail) {
+  MC_W(nw, hipLaunchKernelGGL(k_cand_resolve<W>, dim3(k), dim3(256), 0, s, (const SlotW<W>*)tab, mask, cand, cnt,
+                              k, M, fail));
 }
 
 void launch_table_init(hipStream_t s, int nw, void* tab, uint64_t nslots) {

@@ -9,7 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from mcdfci_amd import Cdfci
+from mcdfci_amd import Cdfci, McdfciError
 from oracle_ctypes import Oracle
 
 
@@ -85,3 +85,23 @@ def test_autotune_keeps_the_trajectory(name, k, M, z_thr, mem, workloads):
     assert np.array_equal(gx, ox)
     assert np.array_equal(gz, oz)
     g.close()
+
+
+def test_access_ceiling_then_reinit(workloads):
+    """The access-ceiling probe overwrites the table: stepping needs a fresh
+    init, after which the run matches one that never probed."""
+    path = workloads("c3_synth")["fcidump"]
+    kw = dict(num_of_coordinate=8, estimate=8, max_memory=0.05, threshold=0.0)
+    a = Cdfci(path, **kw)
+    b = Cdfci(path, **kw)
+    for mode in (0, 1, 2):
+        assert a.access_ceiling(mode, 1 << 16) > 0.0
+    with pytest.raises(McdfciError):
+        a.step(1)
+    a.init()
+    b.init()
+    a.step(40)
+    b.step(40)
+    _same_state(a, b)
+    a.close()
+    b.close()
