@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
 // Values after the scatter: per-element |x*xss+z| (0 when the element does
 // not see its determinant), exact partial sums of H*x and dz*x, and the
 // chunk's top-M list sorted by (value, determinant) descending.
-template <bool MIXED, int W>
+template <bool MIXED, int W, bool TOP>
 __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
@@ -600,8 +600,18 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
     red[wave][1] = xt;
   }
   // Valid entries have a nonzero determinant and sort before the (0, 0)
-  // fillers.
-  block_sort_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
+  // fillers.  Up to 64 kept entries: per-wave sorts and a top-L merge
+  // tournament (the top L under a total order, so the same entries as a full
+  // sort's first L); above, one sort of the whole chunk.
+  const int mw = M < kChunk ? M : kChunk;
+  int L = 1;
+  while (L < mw) L <<= 1;
+  uint64_t ta = 0, tb = 0, tc = 0;
+  uint32_t tp = 0;
+  if constexpr (TOP)
+    block_top_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc}, L, ta, tb, tc, tp);
+  else
+    block_sort_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   __shared__ int wtot[kWG / 64 + 1];
   int total_valid;
   (void)block_excl_scan(nvalid, wtot, total_valid);
@@ -617,18 +627,28 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
     o[2] = (int64_t)(b >> 64);
     o[3] = (int64_t)(uint64_t)b;
   }
-  const int mw = M < kChunk ? M : kChunk;
   StageEnt* out = stage + ((size_t)t * nchunks + chunk) * mw;
   const int keep = total_valid < mw ? total_valid : mw;
+  if constexpr (TOP) {
+    if ((int)threadIdx.x < keep) {
+      StageEnt& o = out[threadIdx.x];
+      o.val = ta;
+      o.det = tb;
+      o.det1 = tc;
+      o.slot = tp;
+      o.pad = 0;
+    }
+  } else {
 #pragma unroll
-  for (int r = 0; r < kEPT; ++r) {
-    const int i = threadIdx.x * kEPT + r;
-    if (i < keep) {
-      out[i].val = A[r];
-      out[i].det = B[r];
-      out[i].det1 = B1[r];
-      out[i].slot = P[r];
-      out[i].pad = 0;
+    for (int r = 0; r < kEPT; ++r) {
+      const int i = threadIdx.x * kEPT + r;
+      if (i < keep) {
+        out[i].val = A[r];
+        out[i].det = B[r];
+        out[i].det1 = B1[r];
+        out[i].slot = P[r];
+        out[i].pad = 0;
+      }
     }
   }
   if (threadIdx.x == 0) stage_cnt[t * nchunks + chunk] = keep;
@@ -689,28 +709,78 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
       if (threadIdx.x == 0) st->error = ERR_CAP;
       return;
     }
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      const int g = threadIdx.x * E + r;
-      A[r] = 0;
-      B[r] = 0;
-      B1[r] = 0;
-      P[r] = 0xffffffffu;
-      if (g < n) {
-        int lo = 0, hi = nchunks;
-        while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (offs[mid] <= g) lo = mid;
-          else hi = mid;
+    int L = 1;
+    while (L < mw) L <<= 1;
+    if (L <= 64 && nchunks * L <= T * E) {
+      // Tournament: the chunk lists (sorted, padded to L with fillers) are
+      // reduced pairwise in place, list i absorbing list i + s at stride s,
+      // each pair by one wave keeping the top L.  List 0 ends as the top L of
+      // all entries: the first L entries of a full sort.
+      for (int g = threadIdx.x; g < nchunks * L; g += T) {
+        const int c = g / L, j = g - c * L;
+        uint64_t a = 0, b = 0, b1 = 0;
+        uint32_t p = 0xffffffffu;
+        if (j < offs[c + 1] - offs[c]) {
+          const StageEnt& e = stage[((size_t)t * nchunks + c) * mw + j];
+          a = e.val;
+          b = e.det;
+          b1 = e.det1;
+          p = e.slot;
         }
-        const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
-        A[r] = e.val;
-        B[r] = e.det;
-        B1[r] = e.det1;
-        P[r] = e.slot;
+        ka[g] = a;
+        kb[g] = b;
+        pl[g] = p;
+        if constexpr (W == 2) kc[g] = b1;
       }
+      __syncthreads();
+      const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      for (int s = 1; s < nchunks; s <<= 1) {
+        for (int i = 2 * s * wv; i + s < nchunks; i += 2 * s * (T / 64)) {
+          uint64_t a, b, b1;
+          uint32_t p;
+          wave_merge_top<W == 2>(scr, i * L, (i + s) * L, L, a, b, b1, p);
+          if (lane < L) {
+            ka[i * L + lane] = a;
+            kb[i * L + lane] = b;
+            pl[i * L + lane] = p;
+            if constexpr (W == 2) kc[i * L + lane] = b1;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const int g = threadIdx.x * E + r;
+        const bool in = g < L;
+        A[r] = in ? ka[g] : 0;
+        B[r] = in ? kb[g] : 0;
+        B1[r] = (W == 2 && in) ? kc[g] : 0;
+        P[r] = in ? pl[g] : 0xffffffffu;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const int g = threadIdx.x * E + r;
+        A[r] = 0;
+        B[r] = 0;
+        B1[r] = 0;
+        P[r] = 0xffffffffu;
+        if (g < n) {
+          int lo = 0, hi = nchunks;
+          while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (offs[mid] <= g) lo = mid;
+            else hi = mid;
+          }
+          const StageEnt& e = stage[((size_t)t * nchunks + lo) * mw + (g - offs[lo])];
+          A[r] = e.val;
+          B[r] = e.det;
+          B1[r] = e.det1;
+          P[r] = e.slot;
+        }
+      }
+      block_sort_desc3<E, W == 2>(A, B, B1, P, scr);
     }
-    block_sort_desc3<E, W == 2>(A, B, B1, P, scr);
   } else {
     const int half = T * E / 2;  // >= mw
     auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint64_t& b1, uint32_t& p) {
@@ -1343,14 +1413,21 @@ void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const 
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
                   int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
+  // Up to 64 kept entries per chunk: the top-L tournament; above, a full sort.
+  const bool top = (M < kChunk ? M : kChunk) <= 64;
+#define MC_P2(MX_, TOP_)                                                                                 \
+  hipLaunchKernelGGL((k_pass2<MX_, W, TOP_>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, \
+                     (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part)
   MC_W(nw, {
-    if (H.mixed)
-      hipLaunchKernelGGL((k_pass2<true, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
-                         (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
-    else
-      hipLaunchKernelGGL((k_pass2<false, W>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh,
-                         (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part);
+    if (H.mixed) {
+      if (top) MC_P2(true, true);
+      else MC_P2(true, false);
+    } else {
+      if (top) MC_P2(false, true);
+      else MC_P2(false, false);
+    }
   });
+#undef MC_P2
 }
 
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,

@@ -41,11 +41,11 @@ __device__ __forceinline__ bool key_gt3(uint64_t a0, uint64_t b0, uint64_t c0, u
 // runtime-indexed register array would be placed in scratch memory).
 template <int E, int S, int R, bool C3>
 __device__ __forceinline__ void sort_reg_stage(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
-                                               uint32_t (&p)[E], int size) {
+                                               uint32_t (&p)[E], int size, int base) {
   if constexpr (R < E) {
     if constexpr ((R & S) == 0) {
       constexpr int Q = R | S;
-      const bool desc = ((int)(threadIdx.x * E + R) & size) == 0;
+      const bool desc = ((base + R) & size) == 0;
       const uint64_t ar = a[R], br = b[R], aq = a[Q], bq = b[Q];
       const uint64_t cr = c[R], cq = c[Q];
       const uint32_t pr = p[R], pq = p[Q];
@@ -61,26 +61,27 @@ __device__ __forceinline__ void sort_reg_stage(uint64_t (&a)[E], uint64_t (&b)[E
         c[Q] = sw ? cr : cq;
       }
     }
-    sort_reg_stage<E, S, R + 1, C3>(a, b, c, p, size);
+    sort_reg_stage<E, S, R + 1, C3>(a, b, c, p, size, base);
   }
 }
 
 // The in-thread tail of one bitonic level: strides min(size, E)/2 ... 1 in
 // order.  The stages run in sequence (no alternative branches whose common
-// code the compiler could merge into address-selected accesses).
+// code the compiler could merge into address-selected accesses).  base is
+// the sort index of register 0.
 template <int E, bool C3>
 __device__ __forceinline__ void sort_reg_tail(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
-                                              uint32_t (&p)[E], int size) {
+                                              uint32_t (&p)[E], int size, int base) {
   if constexpr (E >= 16) {
-    if (size >= 16) sort_reg_stage<E, 8, 0, C3>(a, b, c, p, size);
+    if (size >= 16) sort_reg_stage<E, 8, 0, C3>(a, b, c, p, size, base);
   }
   if constexpr (E >= 8) {
-    if (size >= 8) sort_reg_stage<E, 4, 0, C3>(a, b, c, p, size);
+    if (size >= 8) sort_reg_stage<E, 4, 0, C3>(a, b, c, p, size, base);
   }
   if constexpr (E >= 4) {
-    if (size >= 4) sort_reg_stage<E, 2, 0, C3>(a, b, c, p, size);
+    if (size >= 4) sort_reg_stage<E, 2, 0, C3>(a, b, c, p, size, base);
   }
-  if constexpr (E >= 2) sort_reg_stage<E, 1, 0, C3>(a, b, c, p, size);
+  if constexpr (E >= 2) sort_reg_stage<E, 1, 0, C3>(a, b, c, p, size, base);
 }
 
 // Exchange with the entry `stride` away; the lower index keeps the larger
@@ -139,8 +140,97 @@ __device__ __forceinline__ void block_sort_desc3(uint64_t (&a)[E], uint64_t (&b)
         }
       }
     }
-    sort_reg_tail<E, C3>(a, b, c, p, size);
+    sort_reg_tail<E, C3>(a, b, c, p, size, base);
   }
+}
+
+// One wave sorts its own 64*E entries (entry lane*E + r) descending, every
+// stage in registers or lane shuffles: no LDS and no barrier.
+template <int E, bool C3>
+__device__ __forceinline__ void wave_sort_desc3(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
+                                                uint32_t (&p)[E]) {
+  const int lane = threadIdx.x & 63;
+  const int base = lane * E;
+  for (int size = 2; size <= 64 * E; size <<= 1) {
+    for (int stride = size >> 1; stride >= E; stride >>= 1) {
+      const bool desc = (base & size) == 0;
+      const int m = stride / E;
+      const bool lower = (lane & m) == 0;
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const uint64_t pa = __shfl_xor(a[r], m, 64);
+        const uint64_t pb = __shfl_xor(b[r], m, 64);
+        const uint64_t pc = C3 ? __shfl_xor(c[r], m, 64) : 0;
+        const uint32_t pp = __shfl_xor(p[r], m, 64);
+        sort_keep<C3>(a[r], b[r], c[r], p[r], pa, pb, pc, pp, lower, desc);
+      }
+    }
+    sort_reg_tail<E, C3>(a, b, c, p, size, base);
+  }
+}
+
+// Top L (a power of two, at most 64) of two lists X and Y of L entries,
+// each sorted descending, into lane i (i < L) of the calling wave, sorted
+// descending: lane i keeps the larger of X[i] and Y[L-1-i] (together the
+// top L, a bitonic sequence), then a bitonic merge over lanes.  Lanes from L
+// up get the filler key (0, 0, 0).
+template <bool C3>
+__device__ __forceinline__ void wave_merge_top(const SortScratch& x, int xo, int yo, int L, uint64_t& a,
+                                               uint64_t& b, uint64_t& c, uint32_t& p) {
+  const int lane = threadIdx.x & 63;
+  a = 0, b = 0, c = 0, p = 0xffffffffu;
+  if (lane < L) {
+    const int i = xo + lane, j = yo + L - 1 - lane;
+    const uint64_t xc = C3 ? x.c[i] : 0, yc = C3 ? x.c[j] : 0;
+    const bool ty = key_gt3<C3>(x.a[j], x.b[j], yc, x.a[i], x.b[i], xc);
+    a = ty ? x.a[j] : x.a[i];
+    b = ty ? x.b[j] : x.b[i];
+    c = ty ? yc : xc;
+    p = ty ? x.p[j] : x.p[i];
+  }
+  for (int m = L >> 1; m >= 1; m >>= 1) {
+    const uint64_t pa = __shfl_xor(a, m, 64);
+    const uint64_t pb = __shfl_xor(b, m, 64);
+    const uint64_t pc = C3 ? __shfl_xor(c, m, 64) : 0;
+    const uint32_t pp = __shfl_xor(p, m, 64);
+    sort_keep<C3>(a, b, c, p, pa, pb, pc, pp, (lane & m) == 0, true);
+  }
+}
+
+// Top L (power of two, at most 64) of the workgroup's T*E entries (T = 256,
+// four waves), sorted descending, into lane i (i < L) of wave 0: each wave
+// sorts its entries, stores its top L, and two rounds of pairwise top-L
+// merges (waves 0 and 1, then wave 0) reduce the four lists.  The scratch
+// needs 6L entries.  Returns in every thread; only wave 0's outputs are set.
+template <int E, bool C3>
+__device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
+                                                uint32_t (&p)[E], SortScratch x, int L, uint64_t& oa,
+                                                uint64_t& ob, uint64_t& oc, uint32_t& op) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  wave_sort_desc3<E, C3>(a, b, c, p);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = lane * E + r;
+    if (i < L) {
+      x.a[w * L + i] = a[r];
+      x.b[w * L + i] = b[r];
+      x.p[w * L + i] = p[r];
+      if constexpr (C3) x.c[w * L + i] = c[r];
+    }
+  }
+  __syncthreads();
+  if (w < 2) {
+    wave_merge_top<C3>(x, 2 * w * L, (2 * w + 1) * L, L, oa, ob, oc, op);
+    if (lane < L) {
+      const int i = 4 * L + w * L + lane;
+      x.a[i] = oa;
+      x.b[i] = ob;
+      x.p[i] = op;
+      if constexpr (C3) x.c[i] = oc;
+    }
+  }
+  __syncthreads();
+  if (w == 0) wave_merge_top<C3>(x, 4 * L, 5 * L, L, oa, ob, oc, op);
 }
 
 template <int E>
