@@ -8,6 +8,7 @@
 // C ABI of include/mcdfci.h.  Extension key: solver.cdfci.precision
 // ("fp64" default, or "mixed").
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -241,6 +242,8 @@ double run(Value& option) {
     check(c, mcdfci_set_z_threshold(c, z_threshold));
     check(c, mcdfci_init(c, ref));
     check(c, mcdfci_set_timing(c, timing ? 1 : 0));
+    // Report blocks replay graphs of captured iterations (no effect on results).
+    check(c, mcdfci_set_graph(c, (int)std::min<long>(report_interval, 32)));
     std::cout << std::setw(13) << std::left << "Iteration";
     std::cout << std::setw(18) << std::right << "Energy";
     std::cout << std::setw(18) << "|dx|_max";
