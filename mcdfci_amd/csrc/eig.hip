@@ -56,16 +56,61 @@ __device__ __forceinline__ int lz_capacity(int n) {
   return c < n ? c : n;
 }
 
+// The value of lane (lane ^ M) without the LDS crossbar: lane-swap
+// instructions across 32- and 16-lane halves, DPP row rotation for 8,
+// bank-masked row shifts for 4 and quad permutations for 2 and 1.  The
+// caller's whole wave must be active.
+template <int M>
+__device__ __forceinline__ int lane_xor_i32(int x) {
+  if constexpr (M == 32) {
+    const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (threadIdx.x & 32) ? p[0] : p[1];
+  } else if constexpr (M == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (threadIdx.x & 16) ? p[0] : p[1];
+  } else if constexpr (M == 8) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (M == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0x5, false);  // row_shl:4, banks 0, 2
+    return __builtin_amdgcn_update_dpp(t, x, 0x114, 0xF, 0xA, false);         // row_shr:4, banks 1, 3
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  }
+}
+
+template <int M>
+__device__ __forceinline__ double lane_xor(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor_i32<M>((int)b), hi = lane_xor_i32<M>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the wave by the xor butterfly at offsets 32, 16, ..., 1 (every
+// lane ends with the same value).
 __device__ __forceinline__ double bfly_sum(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+  v = v + lane_xor<32>(v);
+  v = v + lane_xor<16>(v);
+  v = v + lane_xor<8>(v);
+  v = v + lane_xor<4>(v);
+  v = v + lane_xor<2>(v);
+  v = v + lane_xor<1>(v);
   return v;
 }
 
+template <int M>
+__device__ __forceinline__ dd bfly_step_dd(dd v) {
+  return dd_add(v, dd{lane_xor<M>(v.hi), lane_xor<M>(v.lo)});
+}
+
 __device__ __forceinline__ dd bfly_sum_dd(dd v) {
-  for (int off = 32; off >= 1; off >>= 1) {
-    dd o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
-    v = dd_add(v, o);
-  }
+  v = bfly_step_dd<32>(v);
+  v = bfly_step_dd<16>(v);
+  v = bfly_step_dd<8>(v);
+  v = bfly_step_dd<4>(v);
+  v = bfly_step_dd<2>(v);
+  v = bfly_step_dd<1>(v);
   return v;
 }
 
@@ -451,38 +496,32 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       if (r < n && w == 0) L.wv[r] = W[t];
     }
     __syncthreads();
-    // Full reorthogonalisation, one classical Gram-Schmidt pass: thread i
-    // forms h_i = q_i.w as four interleaved sums over the rows, then every
-    // wave subtracts sum_i h_i q_i in ascending i.
-    for (int i = tid; i <= j; i += blockDim.x) {
-      const double* qi = Q + (size_t)i * n;
-      double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
-      int r = 0;
-      for (; r + 15 < n; r += 16) {
-        double qa[16], wa[16];
+    // Full reorthogonalisation, one classical Gram-Schmidt pass:
+    // h_i = q_i.w as four interleaved sums S_g over the rows r = g mod 4
+    // (ascending), combined (S0 + S1) + (S2 + S3); then every wave subtracts
+    // sum_i h_i q_i in ascending i.  Thread 4i + g forms S_g and the four
+    // lanes of i combine by two exchanges.
+    for (int i0 = 0; i0 <= j; i0 += (int)blockDim.x >> 2) {
+      const int i = i0 + (tid >> 2), g = tid & 3;
+      double S = 0.0;
+      if (i <= j) {
+        const double* qi = Q + (size_t)i * n;
+        int r = g;
+        for (; r + 12 < n; r += 16) {
+          double qa[4], wa[4];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          qa[u] = qi[r + u];
-          wa[u] = L.wv[r + u];
-        }
+          for (int u = 0; u < 4; ++u) {
+            qa[u] = qi[r + 4 * u];
+            wa[u] = L.wv[r + 4 * u];
+          }
 #pragma unroll
-        for (int u = 0; u < 16; u += 4) {
-          S0 = S0 + qa[u] * wa[u];
-          S1 = S1 + qa[u + 1] * wa[u + 1];
-          S2 = S2 + qa[u + 2] * wa[u + 2];
-          S3 = S3 + qa[u + 3] * wa[u + 3];
+          for (int u = 0; u < 4; ++u) S = S + qa[u] * wa[u];
         }
+        for (; r < n; r += 4) S = S + qi[r] * L.wv[r];
       }
-      for (; r + 3 < n; r += 4) {
-        S0 = S0 + qi[r] * L.wv[r];
-        S1 = S1 + qi[r + 1] * L.wv[r + 1];
-        S2 = S2 + qi[r + 2] * L.wv[r + 2];
-        S3 = S3 + qi[r + 3] * L.wv[r + 3];
-      }
-      if (r < n) S0 = S0 + qi[r] * L.wv[r];
-      if (r + 1 < n) S1 = S1 + qi[r + 1] * L.wv[r + 1];
-      if (r + 2 < n) S2 = S2 + qi[r + 2] * L.wv[r + 2];
-      L.hbuf[i] = (S0 + S1) + (S2 + S3);
+      const double s01 = S + lane_xor<1>(S);
+      const double h = s01 + lane_xor<2>(s01);
+      if (g == 0 && i <= j) L.hbuf[i] = h;
     }
     __syncthreads();
     double pb = 0.0;

@@ -525,10 +525,13 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
                                                StageEnt* stage, int* stage_cnt, int64_t* part) {
   __shared__ ColCtxW<W> C;
-  __shared__ uint64_t ka[kChunk];
-  __shared__ uint64_t kb[kChunk];
-  __shared__ uint64_t kc[W == 2 ? kChunk : 1];
-  __shared__ uint32_t pl[kChunk];
+  // Selection scratch: the top-L tournament needs 6L <= 384 entries, a full
+  // sort the whole chunk (less LDS per workgroup, more workgroups per CU).
+  constexpr int kSel = TOP ? 6 * 64 : kChunk;
+  __shared__ uint64_t ka[kSel];
+  __shared__ uint64_t kb[kSel];
+  __shared__ uint64_t kc[W == 2 ? kSel : 1];
+  __shared__ uint32_t pl[kSel];
   __shared__ i128 red[kWG / 64][2];
   if (st->error) return;
   const int t = blockIdx.y;
