@@ -35,6 +35,7 @@
 
 #include "device.h"
 #include "kernels.h"
+#include "lanes.h"
 
 namespace mc {
 
@@ -54,37 +55,6 @@ __device__ __forceinline__ int lz_capacity(int n) {
   const int avail = kQBudget - (n <= kEigLdsN ? n * n : 0);
   const int c = avail / n;
   return c < n ? c : n;
-}
-
-// The value of lane (lane ^ M) without the LDS crossbar: lane-swap
-// instructions across 32- and 16-lane halves, DPP row rotation for 8,
-// bank-masked row shifts for 4 and quad permutations for 2 and 1.  The
-// caller's whole wave must be active.
-template <int M>
-__device__ __forceinline__ int lane_xor_i32(int x) {
-  if constexpr (M == 32) {
-    const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
-    return (threadIdx.x & 32) ? p[0] : p[1];
-  } else if constexpr (M == 16) {
-    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-    return (threadIdx.x & 16) ? p[0] : p[1];
-  } else if constexpr (M == 8) {
-    return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);  // row_ror:8
-  } else if constexpr (M == 4) {
-    const int t = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0x5, false);  // row_shl:4, banks 0, 2
-    return __builtin_amdgcn_update_dpp(t, x, 0x114, 0xF, 0xA, false);         // row_shr:4, banks 1, 3
-  } else if constexpr (M == 2) {
-    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-  } else {
-    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-  }
-}
-
-template <int M>
-__device__ __forceinline__ double lane_xor(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = lane_xor_i32<M>((int)b), hi = lane_xor_i32<M>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Sum over the wave by the xor butterfly at offsets 32, 16, ..., 1 (every
