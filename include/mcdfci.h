@@ -42,6 +42,12 @@ enum {
 
 enum { MCDFCI_PREC_FP64 = 0, MCDFCI_PREC_MIXED = 1 };
 
+/* coordinate_update (src/run_wrapper.h:145-150): the subspace eigen-update
+ * (src/solver.h:319-610) or the line search on one coordinate
+ * (src/solver.h:254-314; num_of_coordinate must be 1; |x|^2 and scale are
+ * maintained, which the reference omits). */
+enum { MCDFCI_UPDATE_EIG = 0, MCDFCI_UPDATE_LS = 1 };
+
 typedef struct mcdfci_ctx mcdfci_ctx;
 
 /* Options.  Meanings and defaults follow src/run_wrapper.h:64-87. */
@@ -53,6 +59,7 @@ typedef struct {
   double threshold;      /* hamiltonian.threshold: table entries need |v| > threshold */
   int precision;         /* MCDFCI_PREC_FP64 or MCDFCI_PREC_MIXED */
   int device;            /* HIP device ordinal */
+  int coordinate_update; /* MCDFCI_UPDATE_EIG (default) or MCDFCI_UPDATE_LS */
 } mcdfci_opts;
 
 /* Spin-restricted Hamiltonian in spatial orbitals (what an RHF FCIDUMP holds). */

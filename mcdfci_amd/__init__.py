@@ -28,6 +28,8 @@ DRIVER_PATH = os.path.join(HERE, "bin", "cdfci_omp")
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+UPDATE_EIG = 0  # coordinate_update: subspace eigen-update
+UPDATE_LS = 1   # coordinate_update: line search, one coordinate per iteration
 
 _ERRORS = {
     1: "input", 2: "out of device memory", 3: "table full", 4: "HIP runtime",
@@ -44,7 +46,7 @@ class McdfciError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("num_of_coordinate", C.c_int), ("estimate", C.c_int), ("z_threshold", C.c_double),
                 ("max_memory", C.c_double), ("threshold", C.c_double), ("precision", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("coordinate_update", C.c_int)]
 
 
 class Ham(C.Structure):
@@ -123,10 +125,10 @@ class Cdfci:
     """One device solver context (one HIP device, one stream)."""
 
     def __init__(self, fcidump=None, *, ham=None, num_of_coordinate=1, estimate=100, z_threshold=0.0,
-                 max_memory=1.0, threshold=1e-13, precision=PREC_FP64, device=0):
+                 max_memory=1.0, threshold=1e-13, precision=PREC_FP64, device=0, coordinate_update=UPDATE_EIG):
         self.lib = load_library()
         o = Opts(int(num_of_coordinate), int(estimate), float(z_threshold), float(max_memory),
-                 float(threshold), int(precision), int(device))
+                 float(threshold), int(precision), int(device), int(coordinate_update))
         h = C.c_void_p()
         if fcidump is not None:
             rc = self.lib.mcdfci_create_from_fcidump(os.fsencode(fcidump), C.byref(o), C.byref(h))

@@ -368,7 +368,10 @@ void launch_iteration(mcdfci_ctx* c) {
   if (c->timing) (void)hipEventRecord(c->ev[1], s);
   launch_submatrix(s, c->nw, c->dh, c->st, c->pdet, k, c->Hs, k);
   if (c->timing) (void)hipEventRecord(c->ev[2], s);
-  launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot, 0);
+  if (c->opts.coordinate_update == MCDFCI_UPDATE_LS)
+    launch_ls(s, c->st, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot);
+  else
+    launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot, 0);
   if (c->timing) (void)hipEventRecord(c->ev[3], s);
   launch_update_z(c, k, 1);
   if (c->timing) {
@@ -434,6 +437,11 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
                 "solver:cdfci:max_memory is invalid. It is the maximum memory (GB) allowed for the "
                 "wavefunction. Please set a positive number.");
   if (opts->threshold < 0.0) return fail(nullptr, MCDFCI_E_UNSUPPORTED, "hamiltonian:threshold must be >= 0");
+  if (opts->coordinate_update != MCDFCI_UPDATE_EIG && opts->coordinate_update != MCDFCI_UPDATE_LS)
+    return fail(nullptr, MCDFCI_E_INPUT, "coordinate_update must be MCDFCI_UPDATE_EIG or MCDFCI_UPDATE_LS");
+  if (opts->coordinate_update == MCDFCI_UPDATE_LS && opts->num_of_coordinate != 1)
+    return fail(nullptr, MCDFCI_E_UNSUPPORTED,
+                "the line-search coordinate update takes num_of_coordinate = 1 (one coordinate per iteration)");
   mcdfci_ctx* c = new mcdfci_ctx();
   c->opts = *opts;
   c->k = opts->num_of_coordinate;

@@ -143,6 +143,11 @@ double run(Value& option) {
   }
   o.device = 0;
   if (const char* dv = std::getenv("MCDFCI_DEVICE")) o.device = std::atoi(dv);
+  // The update functor is validated below, where the reference does it;
+  // the context is created with the line search when it is named.
+  o.coordinate_update = (cdo["coordinate_update"].kind == Value::String && cdo["coordinate_update"].s == "ls")
+                            ? MCDFCI_UPDATE_LS
+                            : MCDFCI_UPDATE_EIG;
   // Validate the functor choices before touching the device (src/run_wrapper.h:136-156).
   Ctx ctx;
   int rc = mcdfci_create_from_fcidump(path.c_str(), &o, &ctx.c);
@@ -178,7 +183,7 @@ double run(Value& option) {
   std::cout << "The size of the wavefunction is 2^" << (int)std::log2((double)slots) << " = " << slots
             << ", which can store " << (size_t)(0.79 * (double)slots) << " determinants at most." << std::endl;
   std::cout << "The wavefunction uses about " << std::fixed << std::setprecision(3)
-            << (double)slots * 32 / 1073741824 << " GB memory." << std::endl
+            << (double)slots * bytes_per_det / 1073741824 << " GB memory." << std::endl
             << std::endl;
 
   if (cdo["coordinate_pick"].kind != Value::String || cdo["coordinate_pick"].s != "gcd_grad") {
@@ -192,10 +197,6 @@ double run(Value& option) {
               << " is not implemented." << std::endl;
     std::exit(-1);
   }
-  if (cdo["coordinate_update"].s == "ls")
-    throw std::invalid_argument(
-        "solver:cdfci:coordinate_update \"ls\" does not maintain x.x and the scale in the reference "
-        "(src/solver.h:743-744); use \"eig\".");
 
   const int k = o.num_of_coordinate;
   const long num_iter = (long)cdo["num_iterations"].integer();

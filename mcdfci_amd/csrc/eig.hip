@@ -819,6 +819,62 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   EIG_STAMP(9);
 }
 
+// Line-search update of the one picked coordinate (src/solver.h:254-314):
+// the root of g(t) = t^3 + p1 t + q, p1 = xx - x^2 + h, q = z - h x, that
+// the reference's closed form selects -- the outer root on the side
+// opposite to q -- by Newton's iteration from outside every root
+// (|t| <= max(1, |p1| + |q|)), then the reference's Newton polish of
+// dx = t - x.  The state keeps scale = 1 and xx += dx (2x + dx), which the
+// reference omits for this update.  One thread; update_x as in k_eig.
+__global__ void k_ls(DevState* st, const double* Hs, const double* px, const double* pz, double* dx,
+                     double* tabx, int slot_stride, const uint32_t* pslot) {
+  if (threadIdx.x != 0 || st->error) return;
+  const double h = Hs[0], x = px[0], z = pz[0];
+  dd xx{st->xx_hi, st->xx_lo};
+  const dd x2 = two_prod(x, x);
+  const double p1 = dd_add_d(dd_add(xx, dd{-x2.hi, -x2.lo}), h).hi;
+  const double q = z + (-h) * x;
+  const double sd = q < 0.0 ? 1.0 : -1.0;
+  double t = sd * fmax(1.0, __builtin_fabs(p1) + __builtin_fabs(q));
+  for (int it = 0; it < 400; ++it) {
+    const double g = (t * t + p1) * t + q;
+    const double gp = 3.0 * (t * t) + p1;
+    const double tn = t - g / gp;
+    if (!(sd * tn < sd * t)) break;
+    t = tn;
+  }
+  auto newton = [&](double d) {
+    return d - (d * (d * (d + 3 * x) + (3 * x * x + p1)) + p1 * x + q + x * x * x) /
+                   (d * (3 * d + 6 * x) + 3 * x * x + p1);
+  };
+  double d = t - x;
+  double dn = newton(d);
+  for (int iter = 0; __builtin_fabs((dn - d) / d) > 1e-12 && iter < 10; ++iter) {
+    d = dn;
+    dn = newton(d);
+  }
+  xx = dd_add(xx, dd_mul_d(two_sum(2.0 * x, d), d));
+  st->xx_hi = xx.hi;
+  st->xx_lo = xx.lo;
+  st->sc_hi = 1.0;
+  st->sc_lo = 0.0;
+  dx[0] = d;
+  double* xs = tabx + (size_t)pslot[0] * slot_stride;
+  const double xo = *xs;
+  if (!(__builtin_fabs(xo) != 0.0)) st->x_size += 1;
+  *xs = xo + d;
+  i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
+  xz += to_fix(z * d);
+  st->xz_hi = (int64_t)(xz >> 64);
+  st->xz_lo = (uint64_t)xz;
+  st->k_cur = 1;
+}
+
+void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, const double* pz, double* dx,
+               double* tabx, int slot_stride, const uint32_t* pslot) {
+  hipLaunchKernelGGL(k_ls, dim3(1), dim3(64), 0, s, st, Hs, px, pz, dx, tabx, slot_stride, pslot);
+}
+
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
   // The four-way partial sums assume exactly four waves.

@@ -14,6 +14,9 @@ void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, c
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode);
 // ept: column elements per thread of the scatter (1, 2, 4, 8 or 16); any value
 // gives the same result.
+// Line-search update of the single picked coordinate (k = 1).
+void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, const double* pz, double* dx,
+               double* tabx, int slot_stride, const uint32_t* pslot);
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
                     int ept, long long* ctr);

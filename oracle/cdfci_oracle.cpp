@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "oracle_eig.h"
+#include "oracle_ls.h"
 #include "oracle_num.h"
 
 namespace oc {
@@ -454,6 +455,7 @@ struct Cand {
 struct Solver {
   const Ham* h = nullptr;
   int k = 1, M = 100;
+  int update = 0;  // 0: eig (solver.h:333-609), 1: line search (solver.h:254-314, k = 1)
   double z_thr = 0.0;
   double max_size = 0.0;  // the reference's max_wavefunction_size
   int nthreads = 1;
@@ -670,7 +672,9 @@ struct Solver {
     last_xx = xx;
     last_sc = sc;
     last_xz = xz;
-    int rc = eig_update(k, Hs.data(), px.data(), pz.data(), xx, sc, fixed_to_dd(xz), dx.data(), st);
+    int rc = EIG_OK;
+    if (update == 1) dx[0] = ls_update(Hs[0], px[0], pz[0], xx, sc);
+    else rc = eig_update(k, Hs.data(), px.data(), pz.data(), xx, sc, fixed_to_dd(xz), dx.data(), st);
     if (rc != EIG_OK) {
       error = 2;
       errmsg = "The subspace eigenvalue is not negative; sqrt(-lambda) is undefined. The Hamiltonian "
@@ -882,6 +886,26 @@ int oc_solver_init(void* p, int k, int M, double z_thr, double max_size, int nth
   else ref = c->h.hartree_fock();
   s.init(ref);
   return s.error;
+}
+
+// Coordinate update of the following steps: 0 eig, 1 line search (k = 1).
+int oc_solver_set_update(void* p, int mode) {
+  Ctx* c = (Ctx*)p;
+  if (mode == 1 && c->s.k != 1) {
+    g_err = "the line-search update takes one coordinate per iteration";
+    return 1;
+  }
+  c->s.update = mode;
+  return 0;
+}
+
+// The restated line-search step on given inputs: out[0] = dx, out[1..2] =
+// the new xx (hi, lo).
+void oc_ls_update(double h, double x, double z, double xx_hi, double xx_lo, double* out) {
+  dd xx{xx_hi, xx_lo}, sc = dd_make(1.0);
+  out[0] = ls_update(h, x, z, xx, sc);
+  out[1] = xx.hi;
+  out[2] = xx.lo;
 }
 
 int oc_solver_step(void* p, long n) {

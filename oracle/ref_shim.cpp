@@ -9,6 +9,7 @@
 //   * CoordinatePickGcdGrad<N>              (src/solver.h:106-225)
 //   * WaveFunctionCuckoo<N>::update_x / update_z_and_get_sub / fetch_val
 //                                           (src/wavefunction.h:398-668)
+//   * CoordinateUpdateLS<N>                 (src/solver.h:254-314)
 // CoordinateUpdateEig (src/solver.h:319-610) needs LAPACK/BLAS, which this
 // image does not provide, so it is never instantiated here.
 //
@@ -286,3 +287,14 @@ extern "C" int ref_wf_fetch(void* p, const uint64_t* det, double* x, double* z) 
   return 0;
 }
 
+// CoordinateUpdateLS::operator() on one picked coordinate (x, z) with
+// diagonal h and |x|^2 = xx held by the wavefunction.
+extern "C" double ref_ls_update(double h, double x, double z, double xx) {
+  WfN<1> w(64, 1);
+  w.wf.set_xx(xx);
+  WaveFunctionVector<1> sub;
+  sub.data.push_back({Determinant<1>(), {x, z}});
+  double H[1] = {h};
+  CoordinateUpdateLS<1> ls;
+  return ls(H, w.wf, sub)[0];
+}

@@ -47,6 +47,8 @@ def oracle_lib():
         _sig(lib, "oc_solver_init", C.c_int,
              [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p])
         _sig(lib, "oc_solver_step", C.c_int, [C.c_void_p, C.c_long])
+        _sig(lib, "oc_solver_set_update", C.c_int, [C.c_void_p, C.c_int])
+        _sig(lib, "oc_ls_update", None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _f64p])
         _sig(lib, "oc_solver_report", None, [C.c_void_p, _f64p])
         _sig(lib, "oc_solver_timers", None, [C.c_void_p, _f64p])
         _sig(lib, "oc_table_dump", C.c_long, [C.c_void_p, _u64p, _f64p, _f64p, C.c_void_p, C.c_long])
@@ -82,6 +84,7 @@ def ref_lib():
         _sig(lib, "ref_off_diagonal", C.c_double, [C.c_void_p, _u64p, _u64p])
         _sig(lib, "ref_column", C.c_long, [C.c_void_p, _u64p, _u64p, _f64p, C.c_long])
         _sig(lib, "ref_submatrix", None, [C.c_void_p, C.c_int, _u64p, _f64p])
+        _sig(lib, "ref_ls_update", C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double])
         _sig(lib, "ref_pick", None, [C.c_int, C.c_long, _u64p, _f64p, _f64p, C.c_double,
                                      C.c_double, C.c_int, _u64p, _f64p, _f64p])
         _sig(lib, "ref_wf_create", C.c_void_p, [C.c_int, C.c_long, C.c_int])
@@ -150,6 +153,16 @@ class Oracle:
 
     def step(self, n):
         return self.lib.oc_solver_step(self.h, n)
+
+    def set_update(self, mode):
+        """0: eigen update, 1: line search (one coordinate per iteration)."""
+        if self.lib.oc_solver_set_update(self.h, mode):
+            raise RuntimeError(self.lib.oc_last_error().decode())
+
+    def ls_update(self, h, x, z, xx_hi, xx_lo=0.0):
+        out = np.zeros(3)
+        self.lib.oc_ls_update(h, x, z, xx_hi, xx_lo, out)
+        return out[0], (out[1], out[2])
 
     def report(self):
         out = np.zeros(16)

@@ -17,7 +17,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 import mcdfci_amd
-from mcdfci_amd import Cdfci, McdfciError, PREC_MIXED
+from mcdfci_amd import Cdfci, McdfciError, PREC_MIXED, UPDATE_LS
 from oracle_ctypes import Oracle
 
 
@@ -88,11 +88,13 @@ def test_eig_update_bit_exact(k, workloads):
         assert xxg == (s[0], s[1]) and scg == (s[2], s[3])
 
 
-def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10)):
-    g = Cdfci(path, num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=max_memory, threshold=0.0)
+def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10), update=0):
+    g = Cdfci(path, num_of_coordinate=k, estimate=M, z_threshold=z_thr, max_memory=max_memory, threshold=0.0,
+              coordinate_update=update)
     g.init()
     o = Oracle(path)
     o.solver_init(k, M, z_thr, g.report()["max_size"], nthreads=8)
+    o.set_update(update)
     done = 0
     for n in [c for c in checkpoints if c < iters] + [iters]:
         step = n - done
@@ -140,6 +142,28 @@ def test_trajectory_bit_exact(name, k, M, z_thr, iters, workloads, h2o_fcidump):
     path = _path(name, workloads, h2o_fcidump)
     mem = 8.0 if name in ("c2", "c4", "c5") else 0.05
     _run_pair(path, k, M, z_thr, iters, max_memory=mem)
+
+
+@pytest.mark.parametrize("name,M,z_thr,iters", [
+    ("c1", 8, 0.0, 600),
+    ("c3_synth", 8, 0.0, 300),
+    ("h2o", 8, 0.0, 300),
+    ("c2", 8, 1e-7, 100),
+    ("c5", 16, 1e-5, 12),   # two-word determinants
+])
+def test_line_search_trajectory_bit_exact(name, M, z_thr, iters, workloads, h2o_fcidump):
+    """The line-search coordinate update (one coordinate per iteration)."""
+    print("case ls", name, M, z_thr, iters, flush=True)
+    path = _path(name, workloads, h2o_fcidump)
+    mem = 8.0 if name in ("c2", "c5") else 0.05
+    g, o = _run_pair(path, 1, M, z_thr, iters, max_memory=mem, update=UPDATE_LS)
+    assert g.report()["scale"] == 1.0
+
+
+def test_line_search_needs_one_coordinate(workloads):
+    with pytest.raises(McdfciError):
+        Cdfci(workloads("c1")["fcidump"], num_of_coordinate=2, estimate=8, max_memory=0.01,
+              coordinate_update=UPDATE_LS)
 
 
 def test_c1_reaches_exact_ground_state(workloads):
@@ -239,3 +263,24 @@ def test_driver_end_to_end(workloads, tmp_path):
     inp.write_text(json.dumps(doc))
     r = subprocess.run([mcdfci_amd.DRIVER_PATH, str(inp)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 255 and "is not implemented" in r.stdout
+
+
+def test_driver_line_search(workloads, tmp_path):
+    """coordinate_update "ls" with num_of_coordinate 1 through the driver
+    (the sCDFCI inputs of the reference's experiments)."""
+    info = workloads("c1")
+    import json
+    doc = json.load(open(info["input"]))
+    doc["solver"]["cdfci"].update(num_iterations=3000, report_interval=1000, coordinate_update="ls",
+                                  num_of_coordinate=1, estimate=8)
+    inp = tmp_path / "input.json"
+    inp.write_text(json.dumps(doc))
+    r = subprocess.run([mcdfci_amd.DRIVER_PATH, str(inp)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [l.split() for l in r.stdout.splitlines() if l[:1].isdigit() and len(l.split()) == 9]
+    assert [int(x[0]) for x in rows] == [1000, 2000, 3000]
+    o = Oracle(info["fcidump"])
+    o.solver_init(1, 8, 0.0, 103546.0, nthreads=1)
+    o.set_update(1)
+    o.step(3000)
+    assert rows[-1][1] == "%.10f" % o.report()["energy"]

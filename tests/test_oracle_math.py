@@ -308,3 +308,121 @@ def test_fixed_point_round_trip_exact():
         assert abs(Fraction(v) - exact) <= Fraction(1, 2)
         back = lib.oc_fixed_to_double(int(o[0]), int(o[1]))
         assert abs(_fx(back) - Fraction(v, 2 ** 64)) <= abs(Fraction(v, 2 ** 64)) * Fraction(1, 2 ** 51)
+
+
+# ---------------------------------------------------------------- line search
+def _ls_closed_form(h, x, z, xx):
+    """CoordinateUpdateLS::operator() (src/solver.h:254-314) in numpy float64:
+    Cardano / trigonometric root, then the Newton polish."""
+    dA = -h
+    p1 = xx - x * x - dA
+    q = z + dA * x
+    p3, q2 = p1 / 3, q / 2
+    d = p3 * p3 * p3 + q2 * q2
+    if d >= 0:
+        r = np.sqrt(d)
+        rt = np.cbrt(-q2 + r) + np.cbrt(-q2 - r)
+    else:
+        r = np.sqrt(-d)
+        if q2 >= 0:
+            rt = 2 * np.sqrt(-p3) * np.cos((np.arctan2(-r, -q2) - 2 * np.pi) / 3.0)
+        else:
+            rt = 2 * np.sqrt(-p3) * np.cos(np.arctan2(r, -q2) / 3.0)
+    dx = rt - x
+
+    def newton(dx):
+        return dx - (dx * (dx * (dx + 3 * x) + (3 * x * x + p1)) + p1 * x + q + x * x * x) / \
+            (dx * (3 * dx + 6 * x) + 3 * x * x + p1)
+
+    dxn = newton(dx)
+    it = 0
+    while abs((dxn - dx) / dx) > 1e-12 and it < 10:
+        dx, dxn = dxn, newton(dxn)
+        it += 1
+    return dx
+
+
+def _ls_cases():
+    rng = np.random.default_rng(11)
+    cases = []
+    for _ in range(400):
+        xx = float(rng.uniform(0.2, 40.0))
+        x = float(rng.normal() * np.sqrt(xx) * 0.5)
+        h = float(rng.uniform(-30.0, 5.0))
+        z = float(rng.normal() * 10.0)
+        cases.append((h, x, z, xx))
+    # three real roots of either sign of q, one real root, q = 0, x = 0
+    cases += [(-20.0, 0.3, 1.5, 1.0), (-20.0, 0.3, -1.5, 1.0), (5.0, 0.1, 0.2, 1.0),
+              (-10.0, 0.0, 0.0, 2.0), (-9.0, 1.0, -9.0, 1.0), (-3.0, 0.0, 0.7, 1.0)]
+    return cases
+
+
+def test_line_search_root_matches_closed_form():
+    """The restated root (Newton from outside all roots, IEEE operations only)
+    is the root the reference's closed form selects, to 1e-12 relative."""
+    o = oracle_lib()
+    for h, x, z, xx in _ls_cases():
+        out = np.zeros(3)
+        o.oc_ls_update(h, x, z, xx, 0.0, out)
+        ref = _ls_closed_form(h, x, z, xx)
+        assert abs(out[0] - ref) <= 1e-12 * max(1.0, abs(ref)), (h, x, z, xx, out[0], ref)
+        t = x + out[0]
+        p1, q = xx - x * x + h, z - h * x
+        # a root, and the lowest of f along the coordinate (f' = t^3 + p1 t + q)
+        assert abs(t ** 3 + p1 * t + q) <= 1e-10 * max(1.0, abs(t) ** 3 + abs(p1 * t) + abs(q))
+        roots = np.roots([1.0, 0.0, p1, q])
+        real = roots[np.abs(roots.imag) < 1e-9].real
+        f = lambda s: s ** 4 / 4 + p1 * s * s / 2 + q * s
+        assert f(t) <= min(f(r) for r in real) + 1e-9 * max(1.0, abs(f(t)))
+        # xx bookkeeping: |x|^2 - x^2 + (x + dx)^2
+        assert abs(out[1] - (xx - x * x + t * t)) <= 1e-13 * max(1.0, xx)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                                    "libref.so")), reason="oracle/_ref not built")
+def test_line_search_matches_reference_code():
+    from oracle_ctypes import ref_lib
+    r = ref_lib()
+    o = oracle_lib()
+    for h, x, z, xx in _ls_cases():
+        out = np.zeros(3)
+        o.oc_ls_update(h, x, z, xx, 0.0, out)
+        ref = r.ref_ls_update(h, x, z, xx)
+        assert abs(out[0] - ref) <= 1e-12 * max(1.0, abs(ref)), (h, x, z, xx, out[0], ref)
+
+
+def test_line_search_solver_converges_and_keeps_invariants(workloads):
+    """k = 1 line-search iterations with |x|^2 and scale = 1 maintained:
+    z = H x, xx = sum x^2, and the energy decreases to the ED ground state."""
+    path = workloads("c1")["fcidump"]
+    dets, H = _ed_matrix(path)
+    e0 = np.linalg.eigvalsh(H)[0]
+    idx = {int(d): i for i, d in enumerate(dets)}
+    o = Oracle(path)
+    o.solver_init(1, 8, 0.0, 103546.0, nthreads=1)
+    o.set_update(1)
+    last = 0.0
+    for n in range(6):
+        o.step(500)
+        r = o.report()
+        assert r["error"] == 0
+        assert r["scale"] == 1.0
+        assert r["energy"] <= last + 1e-12 or n == 0
+        last = r["energy"]
+    td, tx, tz, _ = o.table()
+    x = np.zeros(len(dets))
+    z = np.zeros(len(dets))
+    for d, a, b in zip(td[:, 0], tx, tz):
+        x[idx[int(d)]] = a
+        z[idx[int(d)]] = b
+    assert np.allclose(z, H @ x, atol=1e-11)
+    assert abs(r["xx"] - x @ x) < 1e-10 * (x @ x)
+    o.step(20000)
+    assert abs(o.report()["energy"] - e0) < 1e-8
+
+
+def test_line_search_takes_one_coordinate(workloads):
+    o = Oracle(workloads("c1")["fcidump"])
+    o.solver_init(4, 8, 0.0, 103546.0, nthreads=1)
+    with pytest.raises(RuntimeError):
+        o.set_update(1)
