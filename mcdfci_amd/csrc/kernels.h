@@ -24,7 +24,7 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
                   int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part);
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
-                  int nchunks, const StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
+                  int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr);
 // tree: per-column sorts and a merge tree even where one workgroup could
 // sort all k * M candidates (same result either way).

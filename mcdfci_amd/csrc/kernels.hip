@@ -316,6 +316,11 @@ __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st,
   }
 }
 
+template <int N>
+struct LaneCount {
+  static constexpr int value = N;
+};
+
 // ------------------------------------------------------------------ wave / block reductions
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -657,6 +662,87 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   if (threadIdx.x == 0) stage_cnt[t * nchunks + chunk] = keep;
 }
 
+// Tournament in the staging buffer (chunk lists of at most 256 entries):
+// list i absorbs list i + s at stride s, each pair merged by one wave into
+// its top L (L = the power of two >= mw) and written back truncated to mw
+// entries (an entry below position mw of a merged list is below the
+// column's top M).  List 0 ends as the first mw entries of a full sort;
+// they are returned in blocked registers (entry threadIdx.x * E + r).
+template <int W, int E, int T>
+__device__ void column_tournament(StageEnt* lists, int nchunks, int mw, const int* offs, int* lcnt,
+                                uint64_t (&A)[E], uint64_t (&B)[E], uint64_t (&B1)[E], uint32_t (&P)[E]) {
+  for (int c = threadIdx.x; c < nchunks; c += T) lcnt[c] = offs[c + 1] - offs[c];
+  __syncthreads();
+  int L = 1;
+  while (L < mw) L <<= 1;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  auto merge_pair = [&](auto e2tag, int i, int j) {
+    constexpr int E2 = decltype(e2tag)::value;
+    uint64_t a[E2], b[E2], c[E2];
+    uint32_t pp[E2];
+    const int ni = lcnt[i], nj = lcnt[j];
+    const StageEnt* X = lists + (size_t)i * mw;
+    const StageEnt* Y = lists + (size_t)j * mw;
+#pragma unroll
+    for (int r = 0; r < E2; ++r) {
+      const int q = lane * E2 + r;
+      uint64_t xa = 0, xb = 0, xc = 0, ya = 0, yb = 0, yc = 0;
+      uint32_t xp = 0xffffffffu, yp = 0xffffffffu;
+      if (q < L && q < ni) {
+        xa = X[q].val, xb = X[q].det, xc = X[q].det1, xp = X[q].slot;
+      }
+      if (q < L && L - 1 - q < nj) {
+        const StageEnt& e = Y[L - 1 - q];
+        ya = e.val, yb = e.det, yc = e.det1, yp = e.slot;
+      }
+      const bool ty = key_gt3<W == 2>(ya, yb, yc, xa, xb, xc);
+      a[r] = ty ? ya : xa;
+      b[r] = ty ? yb : xb;
+      c[r] = ty ? yc : xc;
+      pp[r] = ty ? yp : xp;
+    }
+    wave_bitonic_merge<E2, W == 2>(a, b, c, pp, L);
+    const int keep = ni + nj < mw ? ni + nj : mw;
+#pragma unroll
+    for (int r = 0; r < E2; ++r) {
+      const int q = lane * E2 + r;
+      if (q < keep) {
+        StageEnt& o = lists[(size_t)i * mw + q];
+        o.val = a[r];
+        o.det = b[r];
+        o.det1 = c[r];
+        o.slot = pp[r];
+        o.pad = 0;
+      }
+    }
+    if (lane == 0) lcnt[i] = keep;
+  };
+  for (int sd = 1; sd < nchunks; sd <<= 1) {
+    for (int i = 2 * sd * wv; i + sd < nchunks; i += 2 * sd * (T / 64)) {
+      if (L <= 64) merge_pair(LaneCount<1>{}, i, i + sd);
+      else if (L == 128) merge_pair(LaneCount<2>{}, i, i + sd);
+      else merge_pair(LaneCount<4>{}, i, i + sd);
+    }
+    __syncthreads();
+  }
+  const int n0 = lcnt[0];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int g = threadIdx.x * E + r;
+    A[r] = 0;
+    B[r] = 0;
+    B1[r] = 0;
+    P[r] = 0xffffffffu;
+    if (g < n0) {
+      const StageEnt& e = lists[g];
+      A[r] = e.val;
+      B[r] = e.det;
+      B1[r] = e.det1;
+      P[r] = e.slot;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ k_merge
 // One workgroup per column: merge chunk lists into the column's top-M, keep
 // the determinants present in the table (the refetch of
@@ -669,7 +755,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
 // level keeps the top Mp (chunk lists are sorted descending by pass 2).
 template <int E, int T, bool STREAM, int W>
 __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot, SlotW<W>* tab, int M,
-                                               int nchunks, const StageEnt* stage,
+                                               int nchunks, StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
                                                const double* dx, int count_iteration, long long* ctr) {
@@ -760,6 +846,9 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
         B1[r] = (W == 2 && in) ? kc[g] : 0;
         P[r] = in ? pl[g] : 0xffffffffu;
       }
+    } else if (mw <= 256) {
+      __shared__ int lcnt[1024];
+      column_tournament<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, A, B, B1, P);
     } else {
 #pragma unroll
       for (int r = 0; r < E; ++r) {
@@ -784,6 +873,9 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
       }
       block_sort_desc3<E, W == 2>(A, B, B1, P, scr);
     }
+  } else if (mw <= 256) {
+    __shared__ int lcnt[1024];
+    column_tournament<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, A, B, B1, P);
   } else {
     const int half = T * E / 2;  // >= mw
     auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint64_t& b1, uint32_t& p) {
@@ -1434,7 +1526,7 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
 }
 
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
-                  int nchunks, const StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
+                  int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr) {
   // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
   // entries, then four entries per thread).  Beyond: the streaming merge

@@ -97,6 +97,32 @@ __device__ __forceinline__ void sort_keep(uint64_t& a, uint64_t& b, uint64_t& c,
   if constexpr (C3) c = take ? pc : c;
 }
 
+// One cross-lane stage over blocked register entries: entry r exchanges
+// with entry r of lane (lane ^ m) and keeps the larger (lower lane, desc) key.
+template <int E, bool C3>
+__device__ __forceinline__ void sort_xlane_stage(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
+                                                 uint32_t (&p)[E], int m, bool desc) {
+  const int lane = threadIdx.x & 63;
+  const bool lower = (lane & m) == 0;
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const uint64_t pa = __shfl_xor(a[r], m, 64);
+    const uint64_t pb = __shfl_xor(b[r], m, 64);
+    const uint64_t pc = C3 ? __shfl_xor(c[r], m, 64) : 0;
+    const uint32_t pp = __shfl_xor(p[r], m, 64);
+    sort_keep<C3>(a[r], b[r], c[r], p[r], pa, pb, pc, pp, lower, desc);
+  }
+}
+
+// Bitonic merge (descending) of the L = 64*E entries a wave holds in
+// blocked order (entry lane*E + r), given a bitonic input.
+template <int E, bool C3>
+__device__ __forceinline__ void wave_bitonic_merge(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
+                                                   uint32_t (&p)[E], int L) {
+  for (int stride = L >> 1; stride >= E; stride >>= 1) sort_xlane_stage<E, C3>(a, b, c, p, stride / E, true);
+  sort_reg_tail<E, C3>(a, b, c, p, L, (int)(threadIdx.x & 63) * E);
+}
+
 // first_size = 2 sorts any input; first_size = n runs only the last level,
 // which sorts a bitonic input (the first half descending, the second
 // ascending) -- the merge of two sorted lists.
