@@ -14,7 +14,8 @@ Phases:
      iterations; value = k*K / seconds.
   2. roofline: a short segment with HIP events around every kernel on the
      solver stream gives the average k_scatter duration; algorithmic bytes =
-     H-elements per launch * 32 B (key 8 + x 8 + z read 8 + z write 8).
+     H-elements per launch * (8N + 24) B (key 8 per word + x 8 + z read 8 +
+     z write 8): 32 B for one-word determinants, 40 B for two.
   3. time to |E - E_ref| < 1e-5 Ha: plain FP64 (k = 8) and mixed+batched
      each run for a bounded wall time from the same start, both with the
      same launch machinery (autotune inside the timed run, graph replay);
@@ -36,7 +37,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_ELEMENT = 32
+
+
+def bytes_per_element(nwords):
+    """Algorithmic bytes of one H-element of the scatter: the key (8 B per
+    determinant word), x (8 B), z read (8 B) and z written (8 B)."""
+    return 8 * nwords + 24
 
 
 def log(*a):
@@ -203,7 +209,8 @@ def main():
     r2 = mb.report()
     elems = r2["cols_size"]
     scatter_s = kt["scatter"] / max(nt, 1)
-    achieved = elems * BYTES_PER_ELEMENT / scatter_s / 1e9
+    bpe = bytes_per_element(mb.nwords)
+    achieved = elems * bpe / scatter_s / 1e9
     log("kernel ms/iter: " + ", ".join(f"{a}={1e3 * b / max(nt, 1):.3f}" for a, b in kt.items()))
     log(f"scatter: {elems} H-elements/launch, {1e6 * scatter_s:.1f} us -> {achieved:.1f} GB/s algorithmic")
     h_elem_rate = elems * args.steps / dt
@@ -324,9 +331,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, k, "mixed"),
-                     "algorithmic_bytes_per_launch": elems * BYTES_PER_ELEMENT,
+                     "algorithmic_bytes_per_launch": elems * bpe,
                      "kernel": "k_scatter", "elements_per_launch": elems,
-                     "bytes_per_element": BYTES_PER_ELEMENT, "avg_launch_us": 1e6 * scatter_s},
+                     "bytes_per_element": bpe, "avg_launch_us": 1e6 * scatter_s},
         "kernel_ms_per_iteration": {a: 1e3 * b / max(nt, 1) for a, b in kt.items()},
         "access_ceiling": dict(ceiling, **{
             "scatter_elements_per_s": elems / scatter_s,
