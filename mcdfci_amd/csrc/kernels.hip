@@ -524,7 +524,9 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
 // Values after the scatter: per-element |x*xss+z| (0 when the element does
 // not see its determinant), exact partial sums of H*x and dz*x, and the
 // chunk's top-M list sorted by (value, determinant) descending.
-template <bool MIXED, int W, bool TOP>
+// SEL 0: full sort of the chunk; 1: top L <= 64 by the per-wave tournament;
+// 2: top 256 (kept entries 65..256).
+template <bool MIXED, int W, int SEL>
 __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   __shared__ ColCtxW<W> C;
   // Selection scratch: the top-L tournament needs 6L <= 384 entries, a full
   // sort the whole chunk (less LDS per workgroup, more workgroups per CU).
-  constexpr int kSel = TOP ? 6 * 64 : kChunk;
+  constexpr int kSel = SEL == 1 ? 6 * 64 : (SEL == 2 ? 512 : kChunk);
   __shared__ uint64_t ka[kSel];
   __shared__ uint64_t kb[kSel];
   __shared__ uint64_t kc[W == 2 ? kSel : 1];
@@ -616,8 +618,10 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   while (L < mw) L <<= 1;
   uint64_t ta = 0, tb = 0, tc = 0;
   uint32_t tp = 0;
-  if constexpr (TOP)
+  if constexpr (SEL == 1)
     block_top_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc}, L, ta, tb, tc, tp);
+  else if constexpr (SEL == 2)
+    block_top256_desc3<W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   else
     block_sort_desc3<kEPT, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
   __shared__ int wtot[kWG / 64 + 1];
@@ -637,7 +641,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   }
   StageEnt* out = stage + ((size_t)t * nchunks + chunk) * mw;
   const int keep = total_valid < mw ? total_valid : mw;
-  if constexpr (TOP) {
+  if constexpr (SEL == 1) {
     if ((int)threadIdx.x < keep) {
       StageEnt& o = out[threadIdx.x];
       o.val = ta;
@@ -1508,18 +1512,22 @@ void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const 
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
                   int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
-  // Up to 64 kept entries per chunk: the top-L tournament; above, a full sort.
-  const bool top = (M < kChunk ? M : kChunk) <= 64;
-#define MC_P2(MX_, TOP_)                                                                                 \
-  hipLaunchKernelGGL((k_pass2<MX_, W, TOP_>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, \
+  // Kept entries per chunk: up to 64 the top-L tournament, up to 256 the
+  // top-256 merge, above a full sort.
+  const int mw = M < kChunk ? M : kChunk;
+  const int sel = mw <= 64 ? 1 : (mw <= 256 ? 2 : 0);
+#define MC_P2(MX_, SEL_)                                                                                 \
+  hipLaunchKernelGGL((k_pass2<MX_, W, SEL_>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, \
                      (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part)
   MC_W(nw, {
     if (H.mixed) {
-      if (top) MC_P2(true, true);
-      else MC_P2(true, false);
+      if (sel == 1) MC_P2(true, 1);
+      else if (sel == 2) MC_P2(true, 2);
+      else MC_P2(true, 0);
     } else {
-      if (top) MC_P2(false, true);
-      else MC_P2(false, false);
+      if (sel == 1) MC_P2(false, 1);
+      else if (sel == 2) MC_P2(false, 2);
+      else MC_P2(false, 0);
     }
   });
 #undef MC_P2

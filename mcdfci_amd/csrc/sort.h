@@ -259,6 +259,51 @@ __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[
   if (w == 0) wave_merge_top<C3>(x, 4 * L, 5 * L, L, oa, ob, oc, op);
 }
 
+// Top 256 of the workgroup's 1024 entries (T = 256, E = 4), sorted
+// descending, into wave 0's registers (blocked, entry lane*4 + r): every
+// wave sorts its 256 entries, then waves 1 and 3 hand their lists to waves 0
+// and 2 through LDS, each of which keeps the top 256 of the pair (the
+// larger of X[i] and Y[255-i], then a bitonic merge), and wave 2 hands its
+// result to wave 0.  The scratch needs 512 entries.
+template <bool C3>
+__device__ __forceinline__ void block_top256_desc3(uint64_t (&a)[4], uint64_t (&b)[4], uint64_t (&c)[4],
+                                                   uint32_t (&p)[4], SortScratch x) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  wave_sort_desc3<4, C3>(a, b, c, p);
+  auto publish = [&](int region) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = region * 256 + lane * 4 + r;
+      x.a[i] = a[r];
+      x.b[i] = b[r];
+      x.p[i] = p[r];
+      if constexpr (C3) x.c[i] = c[r];
+    }
+  };
+  auto absorb = [&](int region) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = region * 256 + 255 - (lane * 4 + r);
+      const uint64_t ya = x.a[j], yb = x.b[j], yc = C3 ? x.c[j] : 0;
+      const uint32_t yp = x.p[j];
+      if (key_gt3<C3>(ya, yb, yc, a[r], b[r], c[r])) {
+        a[r] = ya;
+        b[r] = yb;
+        c[r] = yc;
+        p[r] = yp;
+      }
+    }
+    wave_bitonic_merge<4, C3>(a, b, c, p, 256);
+  };
+  if (w & 1) publish(w >> 1);
+  __syncthreads();
+  if (!(w & 1)) absorb(w >> 1);
+  __syncthreads();
+  if (w == 2) publish(0);
+  __syncthreads();
+  if (w == 0) absorb(0);
+}
+
 template <int E>
 __device__ __forceinline__ void block_sort_desc(uint64_t (&a)[E], uint64_t (&b)[E], uint32_t (&p)[E], SortScratch x,
                                                 int first_size = 2) {
