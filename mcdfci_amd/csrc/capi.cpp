@@ -78,6 +78,7 @@ struct mcdfci_ctx {
   int* pc0 = nullptr;
   int* pc1 = nullptr;
   long long* ctr = nullptr;  // striped iteration counters
+  uint32_t* eslot = nullptr;  // per column element: the slot where the scatter's probe ended
   size_t iter_bytes = 0;
   // launch choices (result-neutral; set by mcdfci_autotune)
   int scatter_ept = 4;
@@ -280,7 +281,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   for (void* p : {(void*)c->pdet, (void*)c->pslot, (void*)c->px, (void*)c->pz, (void*)c->dx, (void*)c->Hs,
                   (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
                   (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt, (void*)c->pk0, (void*)c->pk1,
-                  (void*)c->pc0, (void*)c->pc1, (void*)c->ctr})
+                  (void*)c->pc0, (void*)c->pc1, (void*)c->ctr, (void*)c->eslot})
     dfree(c, p);
   const int k = c->k, M = c->M;
   const DevHam& H = c->dh;
@@ -309,6 +310,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->pc0, k))) return rc;
   if ((rc = dalloc(c, &c->pc1, k))) return rc;
   if ((rc = dalloc(c, &c->ctr, (size_t)kCtrStripes * kCtrStride))) return rc;
+  if ((rc = dalloc(c, &c->eslot, (size_t)k * c->vis_words * 64))) return rc;
   HIPCHECK(c, hipMemsetAsync(c->cand_cnt, 0, sizeof(int) * k, c->stream));
   HIPCHECK(c, hipMemsetAsync(c->ctr, 0, sizeof(long long) * kCtrStripes * kCtrStride, c->stream));
   return MCDFCI_OK;
@@ -347,10 +349,10 @@ int slot_doubles(int nw) { return (int)(slot_bytes(nw) / sizeof(double)); }
 void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
   hipStream_t s = c->stream;
   launch_scatter(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
-                 k, c->scatter_ept, c->ctr);
+                 k, c->scatter_ept, c->ctr, c->eslot);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[4], s);
   launch_pass2(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
-               c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part);
+               c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part, c->eslot);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[5], s);
   launch_merge(s, c->nw, c->st, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
                c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration, c->ctr);

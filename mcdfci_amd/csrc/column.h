@@ -38,6 +38,29 @@ __device__ __forceinline__ long long find_slot(const SlotW<W>* tab, uint64_t mas
   return find_slot<W>(tab, mask, det, dhash<W>(det) & mask & ~3ULL);
 }
 
+// The same lookup resumed at slot r0, a slot of det's probe sequence up to
+// which every slot held another determinant (so det, if present, sits at r0
+// or later: slots are never freed).  The probe sequence is linear in slot
+// order, bucket after bucket.  r0 = 0xffffffff: from the start.
+template <int W>
+__device__ __forceinline__ long long find_slot_at(const SlotW<W>* tab, uint64_t mask, const D2& det, uint32_t r0) {
+  if (r0 == 0xffffffffu) return find_slot<W>(tab, mask, det);
+  uint64_t s = r0;
+  for (uint64_t n = 0; n <= mask; ++n) {
+    if constexpr (W == 1) {
+      const uint64_t k = tab[s].key & kKeyMask;
+      if (k == det.lo) return (long long)s;
+      if (k == 0) return -1;
+    } else {
+      const uint64_t k1 = tab[s].k1;
+      if (k1 == 0) return -1;
+      if ((k1 & kHiMask) == det.hi && tab[s].k0 == det.lo) return (long long)s;
+    }
+    s = (s + 1) & mask;
+  }
+  return -1;
+}
+
 // ------------------------------------------------------------------ column context
 template <int W>
 struct ColCtxW {

@@ -13,16 +13,19 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode);
 // ept: column elements per thread of the scatter (1, 2, 4, 8 or 16); any value
-// gives the same result.
+// gives the same result.  eslot (k columns of vis_words * 64 entries) gets,
+// per column element, the slot where the element's probe ended -- its
+// determinant's slot, or the empty slot that ended a miss -- from which
+// pass 2 resumes the lookup (0xffffffff: probe from the start).
 // Line-search update of the single picked coordinate (k = 1).
 void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, const double* pz, double* dx,
                double* tabx, int slot_stride, const uint32_t* pslot);
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                    int ept, long long* ctr);
+                    int ept, long long* ctr, uint32_t* eslot);
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
-                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part);
+                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot);
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr);

@@ -374,7 +374,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wtot, int& total) {
 template <bool MIXED, int W, int EPT>
 __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                  const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
-                                                 uint64_t* vis, int vis_words, long long* ctr) {
+                                                 uint64_t* vis, int vis_words, long long* ctr, uint32_t* eslot,
+                                                 int estride) {
   __shared__ ColCtxW<W> C;
   if (st->error) return;
   const int t = blockIdx.y;
@@ -409,6 +410,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
         bool mine = false, marked = false;
         uint64_t b = dhash<W>(D) & mask & ~3ULL;
         bool done = false;
+        uint64_t stop = ~0ULL;  // slot where the probe ended (hit, or the empty slot)
         if constexpr (W == 1) {
           for (uint64_t n = 0; n <= mask && !done; n += 4) {
             for (int jj = 0; jj < 4 && !done; ++jj) {
@@ -421,6 +423,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
               } else if (k == 0) {
                 if (!gate) {
                   done = true;
+                  stop = b + jj;
                 } else {
                   uint64_t expect = 0;
                   bool ok = __hip_atomic_compare_exchange_strong(&s->key, &expect, D.lo | kMark, __ATOMIC_RELAXED,
@@ -452,6 +455,7 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
             if (k1 == 0) {
               if (!gate) {
                 done = true;
+                stop = b + jj;
               } else {
                 uint64_t expect = 0;
                 if (__hip_atomic_compare_exchange_strong(&s->k1, &expect, D.hi | kOcc | kMark, __ATOMIC_RELAXED,
@@ -485,6 +489,8 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
           }
         }
         if (!done) fail = true;
+        if (hit) stop = (uint64_t)(hit - tab);
+        eslot[(size_t)t * estride + e] = (uint32_t)(stop & 0xffffffffULL);
         if (hit) {
           // Gate-passing elements always add; the others only into
           // determinants that were present before this iteration.
@@ -530,7 +536,8 @@ template <bool MIXED, int W, int SEL>
 __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
-                                               StageEnt* stage, int* stage_cnt, int64_t* part) {
+                                               StageEnt* stage, int* stage_cnt, int64_t* part,
+                                               const uint32_t* eslot) {
   __shared__ ColCtxW<W> C;
   // Selection scratch: the top-L tournament needs 6L <= 384 entries, a full
   // sort the whole chunk (less LDS per workgroup, more workgroups per CU).
@@ -572,7 +579,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
       }
       if (valid) {
         const double dz = V * dxt;
-        long long si = find_slot<W>(tab, mask, D);
+        long long si = find_slot_at<W>(tab, mask, D, eslot[(size_t)t * nchunks * kChunk + e]);
         double x = 0.0, val = 0.0;
         if (si >= 0) {
           SlotW<W>* sl = &tab[si];
@@ -1484,41 +1491,42 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st
 template <int W, int EPT>
 void launch_scatter_e(hipStream_t s, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                       const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                      long long* ctr) {
+                      long long* ctr, uint32_t* eslot) {
   const int nch = (H.n_enum + kWG * EPT - 1) / (kWG * EPT);
+  const int estride = vis_words * 64;  // column elements rounded up to whole chunks
   if (H.mixed)
     hipLaunchKernelGGL((k_scatter<true, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words, ctr);
+                       mask, vis, vis_words, ctr, eslot, estride);
   else
     hipLaunchKernelGGL((k_scatter<false, W, EPT>), dim3(nch, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, tab,
-                       mask, vis, vis_words, ctr);
+                       mask, vis, vis_words, ctr, eslot, estride);
 }
 
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
-                    int ept, long long* ctr) {
+                    int ept, long long* ctr, uint32_t* eslot) {
   MC_W(nw, {
     SlotW<W>* tb = (SlotW<W>*)tab;
     switch (ept) {
-      case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
-      case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
-      case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
-      case 16: launch_scatter_e<W, 16>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
-      default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr); break;
+      case 1: launch_scatter_e<W, 1>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr, eslot); break;
+      case 2: launch_scatter_e<W, 2>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr, eslot); break;
+      case 8: launch_scatter_e<W, 8>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr, eslot); break;
+      case 16: launch_scatter_e<W, 16>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr, eslot); break;
+      default: launch_scatter_e<W, 4>(s, H, st, pdet, dx, Hs, ldh, tb, mask, vis, vis_words, k, ctr, eslot); break;
     }
   });
 }
 
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
-                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part) {
+                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot) {
   // Kept entries per chunk: up to 64 the top-L tournament, up to 256 the
   // top-256 merge, above a full sort.
   const int mw = M < kChunk ? M : kChunk;
   const int sel = mw <= 64 ? 1 : (mw <= 256 ? 2 : 0);
 #define MC_P2(MX_, SEL_)                                                                                 \
   hipLaunchKernelGGL((k_pass2<MX_, W, SEL_>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, \
-                     (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part)
+                     (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part, eslot)
   MC_W(nw, {
     if (H.mixed) {
       if (sel == 1) MC_P2(true, 1);
