@@ -559,50 +559,96 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   uint64_t A[kEPT], B[kEPT], B1[kEPT];
   uint32_t P[kEPT];
   int nvalid = 0;
+  // Generate the thread's elements, then read every element's slot at the
+  // point where the scatter's probe ended before resolving any of them, so
+  // that the thread's table reads are in flight together.
+  D2 Dv[kEPT];
+  double Vv[kEPT];
+  bool ok[kEPT];
+  uint32_t r0[kEPT];
+  uint64_t vw[kEPT];
 #pragma unroll
   for (int r = 0; r < kEPT; ++r) {
-    const int li = r * kWG + threadIdx.x;
-    const int e = chunk * kChunk + li;
+    const int e = chunk * kChunk + r * kWG + threadIdx.x;
+    Dv[r] = D2{0, 0};
+    Vv[r] = 0.0;
+    ok[r] = false;
+    r0[r] = 0xffffffffu;
+    vw[r] = 0;
+    if (e < H.n_enum) {
+      if (e == 0) {
+        Dv[r] = C.det;
+        Vv[r] = diag;
+        ok[r] = true;
+      } else {
+        ok[r] = gen_elem<MIXED, W>(H, C, e, Dv[r], Vv[r]);
+      }
+      if (ok[r]) {
+        r0[r] = eslot[(size_t)t * nchunks * kChunk + e];
+        vw[r] = vis[(size_t)t * vis_words + (e >> 6)];
+      }
+    }
+  }
+  SlotW<W> sv[kEPT];
+#pragma unroll
+  for (int r = 0; r < kEPT; ++r) {
+    if (r0[r] != 0xffffffffu) {
+      sv[r] = tab[r0[r]];
+    } else {
+      if constexpr (W == 1) sv[r].key = 0;
+      else sv[r].k0 = kSent, sv[r].k1 = 0;
+      sv[r].x = 0.0;
+      sv[r].za = 0;
+      sv[r].zb = 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kEPT; ++r) {
+    const int e = chunk * kChunk + r * kWG + threadIdx.x;
     uint64_t keyv = 0;
     D2 keyd{0, 0};
     uint32_t slot = 0xffffffffu;
-    if (e < H.n_enum) {
-      D2 D{0, 0};
-      double V = 0.0;
-      bool valid;
-      if (e == 0) {
-        D = C.det;
-        V = diag;
-        valid = true;
+    if (ok[r]) {
+      const D2 D = Dv[r];
+      const double V = Vv[r];
+      const double dz = V * dxt;
+      // the slot read above: D's slot, empty (D absent), or another
+      // determinant (then the probe continues after it)
+      long long si;
+      bool here, empty;
+      if constexpr (W == 1) {
+        here = (sv[r].key & kKeyMask) == D.lo;
+        empty = sv[r].key == 0;
       } else {
-        valid = gen_elem<MIXED, W>(H, C, e, D, V);
+        here = sv[r].k1 != 0 && (sv[r].k1 & kHiMask) == D.hi && sv[r].k0 == D.lo;
+        empty = sv[r].k1 == 0;
       }
-      if (valid) {
-        const double dz = V * dxt;
-        long long si = find_slot_at<W>(tab, mask, D, eslot[(size_t)t * nchunks * kChunk + e]);
-        double x = 0.0, val = 0.0;
-        if (si >= 0) {
-          SlotW<W>* sl = &tab[si];
-          // clear the inserted-this-iteration mark
-          if constexpr (W == 1) {
-            if (sl->key & kMark) sl->key = D.lo;
-          } else {
-            if (sl->k1 & kMark) sl->k1 = D.hi | kOcc;
-          }
-          x = sl->x;
-          const uint64_t w = vis[(size_t)t * vis_words + (e >> 6)];
-          if ((w >> (e & 63)) & 1) {
-            double z = fix_to_d(fix_join(sl->za, sl->zb));
-            val = grad_key(s, x, z);
-          }
-          slot = (uint32_t)si;
+      if (r0[r] == 0xffffffffu) si = find_slot<W>(tab, mask, D);
+      else if (here) si = r0[r];
+      else if (empty) si = -1;
+      else si = find_slot_at<W>(tab, mask, D, (uint32_t)((r0[r] + 1ULL) & mask));
+      double x = 0.0, val = 0.0;
+      if (si >= 0) {
+        SlotW<W>* sl = &tab[si];
+        SlotW<W> cur = (si == (long long)r0[r] && r0[r] != 0xffffffffu) ? sv[r] : *sl;
+        // clear the inserted-this-iteration mark
+        if constexpr (W == 1) {
+          if (cur.key & kMark) sl->key = D.lo;
+        } else {
+          if (cur.k1 & kMark) sl->k1 = D.hi | kOcc;
         }
-        nz += to_fix(V * x);
-        xt += to_fix(dz * x);
-        keyv = bitsd(val);
-        keyd = D;
-        ++nvalid;
+        x = cur.x;
+        if ((vw[r] >> (e & 63)) & 1) {
+          double z = fix_to_d(fix_join(cur.za, cur.zb));
+          val = grad_key(s, x, z);
+        }
+        slot = (uint32_t)si;
       }
+      nz += to_fix(V * x);
+      xt += to_fix(dz * x);
+      keyv = bitsd(val);
+      keyd = D;
+      ++nvalid;
     }
     A[r] = keyv;
     B[r] = keyd.lo;
