@@ -131,6 +131,7 @@ __device__ __forceinline__ double dot4_combine(double (*part)[kNMax], int r) {
 
 // Subspace matrix entries A(r, c).
 struct MatLds {  // symmetric, column-major in LDS
+  static constexpr int kBatch = 6;  // columns whose loads mat_partials issues together
   const double* a;
   int n;
   __device__ __forceinline__ double operator()(int r, int c) const { return a[c * n + r]; }
@@ -141,9 +142,17 @@ struct MatLds {  // symmetric, column-major in LDS
 // and stores S_w(r) in part[w][r].  vw(i) returns v[4i + w].
 template <class Mat, class V>
 __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (*part)[kNMax]) {
-  constexpr int CB = 6;  // columns whose loads are issued together
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int CB = Mat::kBatch;  // columns whose loads are issued together
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
+  // Loads are unconditional (indices clamped into A); values outside the
+  // matrix are never accumulated.
+  int rcl[kRows];
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    const int r = lane + 64 * t;
+    rcl[t] = r < n ? r : n - 1;
+  }
   double acc[kRows];
 #pragma unroll
   for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
@@ -153,11 +162,9 @@ __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (
     for (int b = 0; b < CB; ++b) {
       const int i = i0 + b, c = w + 4 * i;
       v[b] = i < nc ? vw(i) : 0.0;
+      const int cc = c < n ? c : n - 1;
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) {
-        const int r = lane + 64 * t;
-        a[b][t] = (i < nc && r < n) ? A(r, c) : 0.0;
-      }
+      for (int t = 0; t < kRows; ++t) a[b][t] = 64 * t < n ? A(rcl[t], cc) : 0.0;
     }
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
@@ -190,6 +197,75 @@ struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
   }
 };
 
+// mat_partials for A read in place from global memory: the same sums in
+// the same order, with every global load unconditional (row and column
+// indices clamped into H; out-of-range values are never accumulated) and
+// the border entries (row 0, column 0; LDS) patched in afterwards, so the
+// loads stay plain global loads without per-load masking.
+template <class V>
+__device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, double (*part)[kNMax]) {
+  constexpr int CB = 8;  // columns whose loads are issued together
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
+  const int o = A.o, kh = n - o;    // H is kh x kh
+  int roff[kRows];
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    int rr = lane + 64 * t - o;
+    rr = rr < 0 ? 0 : (rr > kh - 1 ? kh - 1 : rr);
+    roff[t] = rr;
+  }
+  double acc[kRows];
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
+  for (int i0 = 0; i0 < nc; i0 += CB) {
+    double a[CB][kRows], v[CB];
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      const int i = i0 + b, c = w + 4 * i;
+      v[b] = i < nc ? vw(i) : 0.0;
+      int cc = c - o;
+      cc = cc < 0 ? 0 : (cc > kh - 1 ? kh - 1 : cc);
+      const double* col = A.hs + (size_t)cc * A.ldh;
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) a[b][t] = col[roff[t]];
+    }
+    if (o) {
+#pragma unroll
+      for (int b = 0; b < CB; ++b) {
+        const int c = w + 4 * (i0 + b);
+        if (c == 0) {
+#pragma unroll
+          for (int t = 0; t < kRows; ++t) {
+            const int r = lane + 64 * t;
+            if (r < n) a[b][t] = r == 0 ? A.a00 : A.b0[r - 1];
+          }
+        } else if (lane == 0 && c < n) {
+          a[b][0] = A.b0[c - 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      if (i0 + b < nc) {
+#pragma unroll
+        for (int t = 0; t < kRows; ++t)
+          if (lane + 64 * t < n) acc[t] = acc[t] + a[b][t] * v[b];
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kRows; ++t) {
+    const int r = lane + 64 * t;
+    if (r < n) part[w][r] = acc[t];
+  }
+}
+
+// A condition of the one-thread tridiagonal code as a wave-uniform value
+// (lane 0 is the only active lane there), so that it branches on the
+// scalar unit instead of masking.
+__device__ __forceinline__ bool uni(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
+
 // LU factorisation with partial pivoting of T - shift*I (dgttrf
 // conventions), by one thread; factors land in dg/du/du2/mul/sw.  The
 // elimination chain is carried in registers.
@@ -208,7 +284,7 @@ __device__ __forceinline__ void tri_factor(int n, const double* d, const double*
     e_i = e_n;
     if (i + 2 < n) d_n = d[i + 2];
     if (i + 3 < n) e_n = e[i + 2];
-    if (__builtin_fabs(cur_dg) >= __builtin_fabs(dl)) {
+    if (uni(__builtin_fabs(cur_dg) >= __builtin_fabs(dl))) {
       if (cur_dg == 0.0) cur_dg = tiny;
       const double m = dl / cur_dg;
       mul[i] = m;
@@ -250,7 +326,7 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const doub
       mul_n = mul[i + 1];
       sw_n = swp[i + 1];
     }
-    if (!si) {
+    if (!uni(si != 0)) {
       b[i] = cur;
       cur = nb - mi * cur;
     } else {
@@ -304,15 +380,13 @@ __device__ __forceinline__ bool minor_pass(int n, const double* d, const double*
     double pn = a * p - b * pm;
     double dpn = (a * dp - b * dpm) - p;
     double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
-    if (!(pn > 0.0)) {
+    if (uni(!(pn > 0.0))) {
       ok = false;
       break;
     }
-    if (pn > big) {
-      pn *= small, dpn *= small, ddpn *= small, p *= small, dp *= small, ddp *= small;
-    } else if (pn < small) {
-      pn *= big, dpn *= big, ddpn *= big, p *= big, dp *= big, ddp *= big;
-    }
+    // Exact rescaling by 2^-300, 2^300 or 1.
+    const double sc = pn > big ? small : (pn < small ? big : 1.0);
+    pn *= sc, dpn *= sc, ddpn *= sc, p *= sc, dp *= sc, ddp *= sc;
     pm = p, dpm = dp, ddpm = ddp;
     p = pn, dp = dpn, ddp = ddpn;
   }
@@ -345,7 +419,7 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
   hi = hi + pad;
   const double nn = (double)n;
   double G, Hh;
-  if (hint > lam && hint < hi && minor_pass(n, d, e2, hint, G, Hh)) {
+  if (uni(hint > lam && hint < hi) && minor_pass(n, d, e2, hint, G, Hh)) {
     lam = hint;
   } else if (!minor_pass(n, d, e2, lam, G, Hh)) {
     return lam;
@@ -355,7 +429,7 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
     if (!(disc > 0.0)) disc = 0.0;
     double cand = lam - nn / (G - __builtin_sqrt(disc));
     if (!(cand < hi)) cand = lam + 0.5 * (hi - lam);
-    if (!(cand > lam) || !(cand < hi)) break;
+    if (uni(!(cand > lam) || !(cand < hi))) break;
     double G2, H2;
     if (minor_pass(n, d, e2, cand, G2, H2)) {
       lam = cand, G = G2, Hh = H2;
@@ -363,7 +437,7 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
     }
     hi = cand;
     const double t = cand - 1e-13 * (__builtin_fabs(cand) + 1.0);
-    if (!(t > lam)) break;
+    if (uni(!(t > lam))) break;
     if (minor_pass(n, d, e2, t, G2, H2)) {
       lam = t;
       break;
@@ -672,9 +746,20 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   __syncthreads();
   const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
   if (in_lds) {
-    for (int idx = tid; idx < n * n; idx += bs) {
-      const int c = idx / n, r = idx - c * n;
-      Qlds[idx] = Ms(r, c);
+    // Eight loads in flight per thread (H_sub comes from another kernel's
+    // writes, so each load waits on L2 or the last-level cache).
+    constexpr int U = 8;
+    for (int i0 = tid; i0 < n * n; i0 += U * bs) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = i0 + u * bs;
+        const int c = idx / n, r = idx - c * n;
+        v[u] = idx < n * n ? Ms(r, c) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * bs < n * n) Qlds[i0 + u * bs] = v[u];
     }
     __syncthreads();
   }
