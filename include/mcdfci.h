@@ -142,9 +142,15 @@ int mcdfci_report_get(mcdfci_ctx* ctx, mcdfci_report* r);
  * solver like mcdfci_step), timing the scatter at 1, 2, 4, 8 and 16 column
  * elements per thread and the pick with one-workgroup and merge-tree sorts,
  * and keeps the fastest of each (median kernel time).  choice (may be NULL)
- * receives {elements per thread, 1 if merge-tree pick, iterations run}. */
+ * receives {elements per thread, 1 if merge-tree pick, iterations run}.
+ *
+ * mcdfci_set_launch: sets the same two launch forms directly (elements per
+ * thread 1, 2, 4, 8 or 16; tree_pick 0 or 1, where 0 is refused when
+ * num_of_coordinate * estimate exceeds one workgroup's sort of 4096).  The
+ * forms never change results. */
 int mcdfci_set_graph(mcdfci_ctx* ctx, int iterations_per_graph);
 int mcdfci_autotune(mcdfci_ctx* ctx, int rounds, int64_t* choice);
+int mcdfci_set_launch(mcdfci_ctx* ctx, int elements_per_thread, int tree_pick);
 
 /* Checkpoint and resume (no counterpart in the reference, which restarts
  * from the Hartree-Fock determinant only, src/solver.h:853-873): save the

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mcdfci_kernel_times": (C.c_int, [C.c_void_p, _f64p, C.POINTER(C.c_int64)]),
     "mcdfci_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mcdfci_autotune": (C.c_int, [C.c_void_p, C.c_int, _i64p]),
+    "mcdfci_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mcdfci_access_ceiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double)]),
     "mcdfci_checkpoint_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "mcdfci_checkpoint_load": (C.c_int, [C.c_void_p, C.c_char_p]),
@@ -218,6 +219,10 @@ class Cdfci:
         ch = np.zeros(3, np.int64)
         self._check(self.lib.mcdfci_autotune(self.h, int(rounds), ch))
         return {"elements_per_thread": int(ch[0]), "tree_pick": bool(ch[1]), "iterations": int(ch[2])}
+
+    def set_launch(self, elements_per_thread, tree_pick):
+        """Pin the scatter's elements per thread and the pick form (result-neutral)."""
+        self._check(self.lib.mcdfci_set_launch(self.h, int(elements_per_thread), 1 if tree_pick else 0))
 
     def save(self, path):
         """Write a checkpoint (table, scalars, counters, candidates) to path."""

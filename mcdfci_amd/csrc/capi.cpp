@@ -304,9 +304,8 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->col_xz, (size_t)2 * k))) return rc;
   if ((rc = dalloc(c, &c->cand, (size_t)k * M))) return rc;
   if ((rc = dalloc(c, &c->cand_cnt, k))) return rc;
-  const int L = std::min(M, k);
-  if ((rc = dalloc(c, &c->pk0, (size_t)k * L))) return rc;
-  if ((rc = dalloc(c, &c->pk1, (size_t)k * L))) return rc;
+  if ((rc = dalloc(c, &c->pk0, (size_t)k * k))) return rc;  // pick-tree lists of up to k entries
+  if ((rc = dalloc(c, &c->pk1, (size_t)k * k))) return rc;
   if ((rc = dalloc(c, &c->pc0, k))) return rc;
   if ((rc = dalloc(c, &c->pc1, k))) return rc;
   if ((rc = dalloc(c, &c->ctr, (size_t)kCtrStripes * kCtrStride))) return rc;
@@ -912,6 +911,21 @@ int mcdfci_autotune(mcdfci_ctx* c, int rounds, int64_t* choice) {
   DevState s;
   HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
   return check_state_error(c, s);
+}
+
+int mcdfci_set_launch(mcdfci_ctx* c, int elements_per_thread, int tree_pick) {
+  if (!c) return MCDFCI_E_INPUT;
+  const int e = elements_per_thread;
+  if (!(e == 1 || e == 2 || e == 4 || e == 8 || e == 16) || (tree_pick != 0 && tree_pick != 1))
+    return fail(c, MCDFCI_E_INPUT, "invalid launch form");
+  if (!tree_pick && (long long)c->k * c->M > kSortCap)
+    return fail(c, MCDFCI_E_INPUT, "num_of_coordinate * estimate needs the merge-tree pick");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->scatter_ept = e;
+  c->pick_tree = tree_pick != 0;
+  drop_graph(c);
+  return MCDFCI_OK;
 }
 
 int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {

@@ -1158,10 +1158,11 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
 // found without one global sort: every column list is sorted and cut to
 // L = min(M, k) entries (an entry below position k of its own list has k
 // distinct determinants above it), then lists are merged pairwise, adjacent
-// duplicates dropped and the result cut to L distinct entries, until one
-// list remains.  Same result as sorting all candidates at once.
+// duplicates dropped and the result cut to k distinct entries, until one
+// list remains.  Same result as sorting all candidates at once.  Lists are
+// stored k entries apart.
 template <int E, int T, int W>
-__global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, const CandEnt* cand,
+__global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, int stride, const CandEnt* cand,
                                                  const int* cand_cnt, const SlotW<W>* tab, PickEnt* out,
                                                  int* out_cnt) {
   __shared__ uint64_t ka[T * E];
@@ -1196,13 +1197,13 @@ __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, con
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
-    if (i < keep) out[(size_t)t * L + i] = PickEnt{A[r], B[r], B1[r], P[r], 0};
+    if (i < keep) out[(size_t)t * stride + i] = PickEnt{A[r], B[r], B1[r], P[r], 0};
   }
   if (threadIdx.x == 0) out_cnt[t] = keep;
 }
 
 template <int E, int T, int W>
-__global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, const PickEnt* in,
+__global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int K, const PickEnt* in,
                                                   const int* in_cnt, int nin, PickEnt* out, int* out_cnt) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
@@ -1213,7 +1214,7 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, con
   const int j = blockIdx.x;
   const int la = 2 * j, lb = 2 * j + 1;
   const int na = in_cnt[la], nb = lb < nin ? in_cnt[lb] : 0;
-  const int half = T * E / 2;  // >= L
+  const int half = T * E / 2;  // >= K
   uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
 #pragma unroll
@@ -1225,13 +1226,13 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, con
     P[r] = 0xffffffffu;
     if (i < half) {
       if (i < na) {
-        const PickEnt e = in[(size_t)la * L + i];
+        const PickEnt e = in[(size_t)la * K + i];
         A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
       }
     } else {
       const int q = 2 * half - 1 - i;
       if (q < nb) {
-        const PickEnt e = in[(size_t)lb * L + q];
+        const PickEnt e = in[(size_t)lb * K + q];
         A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
       }
     }
@@ -1260,10 +1261,10 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int L, con
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     if (!first[r]) continue;
-    if (pos < L) out[(size_t)j * L + pos] = PickEnt{A[r], B[r], B1[r], P[r], 0};
+    if (pos < K) out[(size_t)j * K + pos] = PickEnt{A[r], B[r], B1[r], P[r], 0};
     ++pos;
   }
-  if (threadIdx.x == 0) out_cnt[j] = distinct < L ? distinct : L;
+  if (threadIdx.x == 0) out_cnt[j] = distinct < K ? distinct : K;
 }
 
 template <int W>
@@ -1655,7 +1656,7 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
       int wc = 64;
       while (wc < M) wc <<= 1;
 #define MC_COLS(E_, T_)                                                                                       \
-  hipLaunchKernelGGL((k_pick_cols<E_, T_, W>), dim3(k), dim3(T_), 0, s, st, M, L, cand, cand_cnt, tb, pk0, pc0)
+  hipLaunchKernelGGL((k_pick_cols<E_, T_, W>), dim3(k), dim3(T_), 0, s, st, M, L, k, cand, cand_cnt, tb, pk0, pc0)
       switch (wc) {
         case 64: MC_COLS(1, 64); break;
         case 128: MC_COLS(1, 128); break;
@@ -1664,16 +1665,16 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
         default: MC_COLS(4, 256); break;
       }
 #undef MC_COLS
-      // Merge levels: window 2 * next power of two >= L.
+      // Merge levels (lists of up to k): window 2 * next power of two >= k.
       int wm = 64;
-      while (wm < 2 * L) wm <<= 1;
+      while (wm < 2 * k) wm <<= 1;
       PickEnt *in = pk0, *out = pk1;
       int *ic = pc0, *oc = pc1;
       int nin = k;
       while (nin > 1) {
         const int nout = (nin + 1) / 2;
 #define MC_PM(E_, T_) \
-  hipLaunchKernelGGL((k_pick_merge<E_, T_, W>), dim3(nout), dim3(T_), 0, s, st, L, in, ic, nin, out, oc)
+  hipLaunchKernelGGL((k_pick_merge<E_, T_, W>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc)
         switch (wm) {
           case 64: MC_PM(1, 64); break;
           case 128: MC_PM(1, 128); break;
