@@ -224,24 +224,94 @@ __device__ __forceinline__ void wave_merge_top(const SortScratch& x, int xo, int
 }
 
 // Top L (power of two, at most 64) of the workgroup's T*E entries (T = 256,
-// four waves), sorted descending, into lane i (i < L) of wave 0: each wave
-// sorts its entries, stores its top L, and two rounds of pairwise top-L
-// merges (waves 0 and 1, then wave 0) reduce the four lists.  The scratch
-// needs 6L entries.  Returns in every thread; only wave 0's outputs are set.
+// four waves), sorted descending, into lane i (i < L) of wave 0.  Each wave
+// first finds its top L: the L-th largest key K of its 64*E entries, built
+// bit by bit from ballots (the largest P with at least L keys >= P), stopping
+// as soon as at most 64 keys are >= P; those entries (they include the top L
+// under the full order, ties on the key included) are compacted one per lane
+// and sorted across the wave.  A wave with more than 64 entries tied at or
+// above K, or fewer than L nonzero keys, sorts all its entries instead.
+// Either way the wave's top L list is the same.  Then two rounds of pairwise
+// top-L merges (waves 0 and 1, then wave 0) reduce the four lists.  The
+// scratch needs max(256, 6L) entries.  Returns in every thread; only wave 0's
+// outputs are set.
 template <int E, bool C3>
 __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
                                                 uint32_t (&p)[E], SortScratch x, int L, uint64_t& oa,
                                                 uint64_t& ob, uint64_t& oc, uint32_t& op) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  wave_sort_desc3<E, C3>(a, b, c, p);
+  const uint64_t lt = (1ULL << lane) - 1;
+  int nzero = 0;
 #pragma unroll
-  for (int r = 0; r < E; ++r) {
-    const int i = lane * E + r;
-    if (i < L) {
-      x.a[w * L + i] = a[r];
-      x.b[w * L + i] = b[r];
-      x.p[w * L + i] = p[r];
-      if constexpr (C3) x.c[w * L + i] = c[r];
+  for (int r = 0; r < E; ++r) nzero += __popcll(__ballot(a[r] != 0));
+  uint64_t P = 0;
+  int cP = 64 * E;
+  if (nzero >= L) {
+    for (int bit = 62; bit >= 0 && cP > 64; --bit) {
+      const uint64_t cand = P | (1ULL << bit);
+      int cnt = 0;
+#pragma unroll
+      for (int r = 0; r < E; ++r) cnt += __popcll(__ballot(a[r] >= cand));
+      if (cnt >= L) {
+        P = cand;
+        cP = cnt;
+      }
+    }
+  }
+  const bool filtered = cP <= 64;
+  if (filtered) {
+    int base = w * 64;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const bool keep = a[r] >= P;
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int i = base + __popcll(m & lt);
+        x.a[i] = a[r];
+        x.b[i] = b[r];
+        x.p[i] = p[r];
+        if constexpr (C3) x.c[i] = c[r];
+      }
+      base += __popcll(m);
+    }
+  }
+  __syncthreads();
+  uint64_t sa = 0, sb = 0, sc = 0;
+  uint32_t sp = 0xffffffffu;
+  if (filtered) {
+    if (lane < cP) {
+      const int i = w * 64 + lane;
+      sa = x.a[i];
+      sb = x.b[i];
+      sp = x.p[i];
+      if constexpr (C3) sc = x.c[i];
+    }
+    uint64_t ra[1] = {sa}, rb[1] = {sb}, rc[1] = {sc};
+    uint32_t rp[1] = {sp};
+    wave_sort_desc3<1, C3>(ra, rb, rc, rp);
+    sa = ra[0], sb = rb[0], sc = rc[0], sp = rp[0];
+  } else {
+    wave_sort_desc3<E, C3>(a, b, c, p);
+  }
+  __syncthreads();
+  if (filtered) {
+    if (lane < L) {
+      const int i = w * L + lane;
+      x.a[i] = sa;
+      x.b[i] = sb;
+      x.p[i] = sp;
+      if constexpr (C3) x.c[i] = sc;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      const int i = lane * E + r;
+      if (i < L) {
+        x.a[w * L + i] = a[r];
+        x.b[w * L + i] = b[r];
+        x.p[w * L + i] = p[r];
+        if constexpr (C3) x.c[w * L + i] = c[r];
+      }
     }
   }
   __syncthreads();
