@@ -539,9 +539,9 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
                                                StageEnt* stage, int* stage_cnt, int64_t* part,
                                                const uint32_t* eslot) {
   __shared__ ColCtxW<W> C;
-  // Selection scratch: the top-L tournament needs 6L <= 384 entries, a full
+  // Selection scratch: the top-L tournament needs 512 entries, a full
   // sort the whole chunk (less LDS per workgroup, more workgroups per CU).
-  constexpr int kSel = SEL == 1 ? 6 * 64 : (SEL == 2 ? 512 : kChunk);
+  constexpr int kSel = SEL == 1 ? 8 * 64 : (SEL == 2 ? 512 : kChunk);
   __shared__ uint64_t ka[kSel];
   __shared__ uint64_t kb[kSel];
   __shared__ uint64_t kc[W == 2 ? kSel : 1];

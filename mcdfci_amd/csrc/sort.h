@@ -224,17 +224,19 @@ __device__ __forceinline__ void wave_merge_top(const SortScratch& x, int xo, int
 }
 
 // Top L (power of two, at most 64) of the workgroup's T*E entries (T = 256,
-// four waves), sorted descending, into lane i (i < L) of wave 0.  Each wave
-// first finds its top L: the L-th largest key K of its 64*E entries, built
-// bit by bit from ballots (the largest P with at least L keys >= P), stopping
-// as soon as at most 64 keys are >= P; those entries (they include the top L
-// under the full order, ties on the key included) are compacted one per lane
-// and sorted across the wave.  A wave with more than 64 entries tied at or
-// above K, or fewer than L nonzero keys, sorts all its entries instead.
-// Either way the wave's top L list is the same.  Then two rounds of pairwise
-// top-L merges (waves 0 and 1, then wave 0) reduce the four lists.  The
-// scratch needs max(256, 6L) entries.  Returns in every thread; only wave 0's
-// outputs are set.
+// four waves, E = 4), sorted descending, into lane i (i < L) of wave 0.  Each
+// wave first finds its top L: the L-th largest key K of its 64*E entries,
+// built bit by bit from ballots (the largest P with at least L keys >= P),
+// stopping as soon as at most S keys are >= P (S = 64 for L <= 32, else 128,
+// so that a tie across rank L -- determinants related by spin flip share
+// their key -- still stops); those entries (they include the top L under the
+// full order, ties on the key included) are compacted one or two per lane
+// and sorted across the wave.  A wave with more than 128 entries at or above
+// K, or fewer than L nonzero keys, sorts all its entries instead.  Either way
+// the wave's top L list is the same.  Then two rounds of pairwise top-L
+// merges (waves 0 and 1, then wave 0) reduce the four lists.  The scratch
+// needs max(512, 6L) entries.  Returns in every thread; only wave 0's outputs
+// are set.
 template <int E, bool C3>
 __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[E], uint64_t (&c)[E],
                                                 uint32_t (&p)[E], SortScratch x, int L, uint64_t& oa,
@@ -246,8 +248,9 @@ __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[
   for (int r = 0; r < E; ++r) nzero += __popcll(__ballot(a[r] != 0));
   uint64_t P = 0;
   int cP = 64 * E;
+  const int S = L <= 32 ? 64 : 128;
   if (nzero >= L) {
-    for (int bit = 62; bit >= 0 && cP > 64; --bit) {
+    for (int bit = 62; bit >= 0 && cP > S; --bit) {
       const uint64_t cand = P | (1ULL << bit);
       int cnt = 0;
 #pragma unroll
@@ -258,9 +261,10 @@ __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[
       }
     }
   }
-  const bool filtered = cP <= 64;
+  // one survivor per lane (f1), two (f2), or the full sort
+  const bool f1 = cP <= 64, f2 = !f1 && cP <= 128, filtered = f1 || f2;
   if (filtered) {
-    int base = w * 64;
+    int base = w * 128;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const bool keep = a[r] >= P;
@@ -276,31 +280,46 @@ __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[
     }
   }
   __syncthreads();
-  uint64_t sa = 0, sb = 0, sc = 0;
-  uint32_t sp = 0xffffffffu;
+  // survivors in blocked order (entry lane * R + r), fillers past cP
+  uint64_t sa[2] = {0, 0}, sb[2] = {0, 0}, sc[2] = {0, 0};
+  uint32_t sp[2] = {0xffffffffu, 0xffffffffu};
   if (filtered) {
-    if (lane < cP) {
-      const int i = w * 64 + lane;
-      sa = x.a[i];
-      sb = x.b[i];
-      sp = x.p[i];
-      if constexpr (C3) sc = x.c[i];
+    const int R = f1 ? 1 : 2;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int j = lane * R + r;
+      if (r < R && j < cP) {
+        const int i = w * 128 + j;
+        sa[r] = x.a[i];
+        sb[r] = x.b[i];
+        sp[r] = x.p[i];
+        if constexpr (C3) sc[r] = x.c[i];
+      }
     }
-    uint64_t ra[1] = {sa}, rb[1] = {sb}, rc[1] = {sc};
-    uint32_t rp[1] = {sp};
-    wave_sort_desc3<1, C3>(ra, rb, rc, rp);
-    sa = ra[0], sb = rb[0], sc = rc[0], sp = rp[0];
+    if (f1) {
+      uint64_t ra[1] = {sa[0]}, rb[1] = {sb[0]}, rc[1] = {sc[0]};
+      uint32_t rp[1] = {sp[0]};
+      wave_sort_desc3<1, C3>(ra, rb, rc, rp);
+      sa[0] = ra[0], sb[0] = rb[0], sc[0] = rc[0], sp[0] = rp[0];
+    } else {
+      wave_sort_desc3<2, C3>(sa, sb, sc, sp);
+    }
   } else {
     wave_sort_desc3<E, C3>(a, b, c, p);
   }
   __syncthreads();
   if (filtered) {
-    if (lane < L) {
-      const int i = w * L + lane;
-      x.a[i] = sa;
-      x.b[i] = sb;
-      x.p[i] = sp;
-      if constexpr (C3) x.c[i] = sc;
+    const int R = f1 ? 1 : 2;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int j = lane * R + r;
+      if (r < R && j < L) {
+        const int i = w * L + j;
+        x.a[i] = sa[r];
+        x.b[i] = sb[r];
+        x.p[i] = sp[r];
+        if constexpr (C3) x.c[i] = sc[r];
+      }
     }
   } else {
 #pragma unroll
@@ -330,16 +349,24 @@ __device__ __forceinline__ void block_top_desc3(uint64_t (&a)[E], uint64_t (&b)[
 }
 
 // Top 256 of the workgroup's 1024 entries (T = 256, E = 4), sorted
-// descending, into wave 0's registers (blocked, entry lane*4 + r): every
-// wave sorts its 256 entries, then waves 1 and 3 hand their lists to waves 0
-// and 2 through LDS, each of which keeps the top 256 of the pair (the
-// larger of X[i] and Y[255-i], then a bitonic merge), and wave 2 hands its
-// result to wave 0.  The scratch needs 512 entries.
+// descending, into wave 0's registers (blocked, entry lane*4 + r).
+// First the 256th largest key K of the block, built bit by bit from ballots
+// summed over the four waves (the largest P with at least 256 keys >= P),
+// stopping once at most 512 keys are >= P.  Those entries (they include the
+// top 256 under the full order, ties on the key included) are compacted in
+// LDS; with at most 256 of them wave 0 sorts them alone, with at most 512
+// waves 0 and 1 each sort 256 and wave 0 keeps the top 256 of the pair (the
+// larger of X[i] and Y[255-i], then a bitonic merge).  Otherwise (fewer
+// than 256 nonzero keys, or more than 512 entries tied at or above K) every
+// wave sorts its 256 entries, waves 1 and 3 hand their lists to waves 0 and
+// 2, each keeps the top 256 of its pair, and wave 2 hands its result to
+// wave 0.  The scratch needs 512 entries.
 template <bool C3>
 __device__ __forceinline__ void block_top256_desc3(uint64_t (&a)[4], uint64_t (&b)[4], uint64_t (&c)[4],
                                                    uint32_t (&p)[4], SortScratch x) {
+  __shared__ int wc[2][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  wave_sort_desc3<4, C3>(a, b, c, p);
+  const uint64_t lt = (1ULL << lane) - 1;
   auto publish = [&](int region) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -365,6 +392,82 @@ __device__ __forceinline__ void block_top256_desc3(uint64_t (&a)[4], uint64_t (&
     }
     wave_bitonic_merge<4, C3>(a, b, c, p, 256);
   };
+  // Block-wide counts: every wave writes its count, one barrier, every
+  // thread sums the four (two buffers, so one barrier per count).
+  int it = 0;
+  auto block_count = [&](int wave_count) {
+    const int buf = (it++) & 1;
+    if (lane == 0) wc[buf][w] = wave_count;
+    __syncthreads();
+    return wc[buf][0] + wc[buf][1] + wc[buf][2] + wc[buf][3];
+  };
+  int nz = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) nz += __popcll(__ballot(a[r] != 0));
+  uint64_t P = 0;
+  int cP = 1024;
+  if (block_count(nz) >= 256) {
+    for (int bit = 62; bit >= 0 && cP > 512; --bit) {
+      const uint64_t cand = P | (1ULL << bit);
+      int cnt = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cnt += __popcll(__ballot(a[r] >= cand));
+      const int tot = block_count(cnt);
+      if (tot >= 256) {
+        P = cand;
+        cP = tot;
+      }
+    }
+  }
+  if (cP <= 512) {
+    // compact the entries >= P in wave order, then waves 0 (and 1) load
+    // 256 each, blocked, fillers past cP
+    int sw = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sw += __popcll(__ballot(a[r] >= P));
+    const int buf = (it++) & 1;
+    if (lane == 0) wc[buf][w] = sw;
+    __syncthreads();
+    int base = 0;
+    for (int v = 0; v < w; ++v) base += wc[buf][v];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool keep = a[r] >= P;
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int i = base + __popcll(m & lt);
+        x.a[i] = a[r];
+        x.b[i] = b[r];
+        x.p[i] = p[r];
+        if constexpr (C3) x.c[i] = c[r];
+      }
+      base += __popcll(m);
+    }
+    __syncthreads();
+    const bool two = cP > 256;
+    if (w == 0 || (w == 1 && two)) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = w * 256 + lane * 4 + r;
+        a[r] = 0, b[r] = 0, c[r] = 0, p[r] = 0xffffffffu;
+        if (j < cP) {
+          a[r] = x.a[j];
+          b[r] = x.b[j];
+          p[r] = x.p[j];
+          if constexpr (C3) c[r] = x.c[j];
+        }
+      }
+      wave_sort_desc3<4, C3>(a, b, c, p);
+    }
+    __syncthreads();
+    if (two) {
+      if (w == 1) publish(0);
+      __syncthreads();
+      if (w == 0) absorb(0);
+    }
+    return;
+  }
+  wave_sort_desc3<4, C3>(a, b, c, p);
   if (w & 1) publish(w >> 1);
   __syncthreads();
   if (!(w & 1)) absorb(w >> 1);

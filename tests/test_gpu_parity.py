@@ -136,6 +136,9 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10), up
     # 42 orbitals: two-word determinants
     ("c5", 8, 16, 1e-5, 12),
     ("c5", 64, 64, 1e-5, 4),
+    # two words with the top-256 selection (estimate 65..256)
+    ("c5", 16, 128, 1e-5, 4),
+    ("c5", 32, 256, 1e-5, 3),
 ])
 def test_trajectory_bit_exact(name, k, M, z_thr, iters, workloads, h2o_fcidump):
     print("case", name, k, M, z_thr, iters, flush=True)
