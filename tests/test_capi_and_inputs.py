@@ -8,6 +8,7 @@ import json
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -88,3 +89,23 @@ def test_driver_argument_and_input_errors(tmp_path):
                                                 "report_interval": 10}}})
     assert r.returncode == 1 and "report_interval >= 1000" in r.stderr
     assert "Warning: z cut-off threshold auto search" in r.stdout
+
+
+def test_bench_reads_the_committed_traffic_summary():
+    """bench.py's roofline.traffic comes from the counter summary under
+    profiles/: the line must parse, and the measured bytes per k_scatter
+    launch must exceed the algorithmic bytes (32 B per H-element)."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    t = bench.measured_traffic("c4", 64, "mixed")
+    assert isinstance(t, int)
+    assert t > 1903168 * 32
+    assert bench.measured_traffic("c4", 63, "mixed") is None
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout + r.stderr
