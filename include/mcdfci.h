@@ -96,6 +96,10 @@ typedef struct {
   int64_t table_slots;  /* device table capacity (slots) */
   double max_size;      /* the reference's max_wavefunction_size */
   int64_t candidates;   /* length of the candidate list for the next pick (sub_xz.size()) */
+  /* subspace solves whose Ritz vector failed the residual test after the
+   * last restart and was used regardless (no counterpart: the reference
+   * uses its LAPACK eigenvector unchecked, src/solver.h:384-445) */
+  int64_t eig_unconverged;
 } mcdfci_report;
 
 const char* mcdfci_last_error(const mcdfci_ctx* ctx);
@@ -195,6 +199,17 @@ int mcdfci_eig_update(mcdfci_ctx* ctx, int k, const double* H, const double* x, 
 /* Contents of the wavefunction table, sorted by determinant
  * (WaveFunctionCuckoo::fetch_val over all entries, src/wavefunction.h:631-651). */
 int mcdfci_table_dump(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);
+/* Order-independent digest of the whole table (the same entries as
+ * mcdfci_table_dump, with z exact): out3 = {entries, sum of key hashes,
+ * sum of (key, x, z) hashes}, sums modulo 2^64.  The entry hash is
+ *   e = mix64(w0 + 0x9e3779b97f4a7c15); e = mix64(e ^ w1); key hash = e;
+ *   e = mix64(e ^ bits(x), with 0 for x == 0); e = mix64(e ^ z_hi);
+ *   entry hash = mix64(e ^ z_lo)
+ * over the determinant words w0, w1 (w1 = 0 for one word) and the 128-bit
+ * z in units of 2^-64; mix64 is the MurmurHash3 64-bit finaliser.  Compares
+ * tables too large to dump (WaveFunctionCuckoo contents,
+ * src/wavefunction.h:631-651). */
+int mcdfci_table_digest(mcdfci_ctx* ctx, uint64_t* out3);
 /* The candidate list passed to the next pick (sub_xz, src/wavefunction.h:567-589). */
 int mcdfci_candidates(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);
 

@@ -64,7 +64,7 @@ class Report(C.Structure):
                 ("sum_inv_iter", C.c_int64), ("curr_inv_iter", C.c_int64),
                 ("sum_diff_inv_iter", C.c_double), ("curr_diff_inv_iter", C.c_double),
                 ("max_diff_inv_iter", C.c_double), ("table_slots", C.c_int64),
-                ("max_size", C.c_double), ("candidates", C.c_int64)]
+                ("max_size", C.c_double), ("candidates", C.c_int64), ("eig_unconverged", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -102,6 +102,7 @@ SIGNATURES = {
     "mcdfci_eig_update": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
     "mcdfci_table_dump": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]),
+    "mcdfci_table_digest": (C.c_int, [C.c_void_p, _u64p]),
     "mcdfci_candidates": (C.c_int, [C.c_void_p, _u64p, _f64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_energy_gradient": (C.c_int, [C.c_void_p, _f64p, _f64p, C.POINTER(C.c_double)]),
 }
@@ -279,6 +280,13 @@ class Cdfci:
                                                C.byref(n)))
         m = min(m, n.value)
         return dets[: m * self.nwords].reshape(m, self.nwords), x[:m], z[:m]
+
+    def table_digest(self):
+        """{entries, key-hash sum, entry-hash sum} of the whole table, order-independent
+        and exact in z (mcdfci_table_digest)."""
+        out = np.zeros(3, np.uint64)
+        self._check(self.lib.mcdfci_table_digest(self.h, out))
+        return tuple(int(v) for v in out)
 
     def candidates(self):
         cap = self.k * self.estimate

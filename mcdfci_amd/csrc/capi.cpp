@@ -443,6 +443,18 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   if (opts->coordinate_update == MCDFCI_UPDATE_LS && opts->num_of_coordinate != 1)
     return fail(nullptr, MCDFCI_E_UNSUPPORTED,
                 "the line-search coordinate update takes num_of_coordinate = 1 (one coordinate per iteration)");
+  {
+    // Slot indices are 32-bit (staging lists, candidates, probe ends), so
+    // the table holds at most 2^32 slots; refuse before allocating.
+    uint64_t slots;
+    double msz;
+    int pw;
+    table_size(opts->max_memory, 1 + (2 * ham->norb - 1) / 64, slots, msz, pw);
+    if (slots > (1ULL << 32))
+      return fail(nullptr, MCDFCI_E_UNSUPPORTED,
+                  "max_memory gives " + std::to_string(slots) +
+                      " wavefunction slots; the device table holds at most 2^32 (lower max_memory)");
+  }
   mcdfci_ctx* c = new mcdfci_ctx();
   c->opts = *opts;
   c->k = opts->num_of_coordinate;
@@ -644,13 +656,13 @@ namespace {
 // lists, then the table's entries in chunks of at most kCkChunk (per chunk:
 // the count, two words per determinant, x, and the (zA, zB) pairs).
 constexpr char kCkMagic[8] = {'M', 'C', 'D', 'F', 'C', 'I', 'C', 'K'};
-constexpr int kCkVersion = 1;
+constexpr int kCkVersion = 2;
 constexpr long long kCkChunk = 1LL << 22;
 
 struct CkHeader {
   char magic[8];
   int32_t version, state_bytes, nio, nw;
-  int32_t norb, nelec, ms2, na, nb, k, M, precision;
+  int32_t norb, nelec, ms2, na, nb, k, M, precision, coordinate_update, pad;
   double threshold, core;
   uint64_t ham_hash;
   int64_t entries;
@@ -733,6 +745,7 @@ int mcdfci_checkpoint_save(mcdfci_ctx* c, const char* path) {
   hd.k = c->k;
   hd.M = c->M;
   hd.precision = c->opts.precision;
+  hd.coordinate_update = c->opts.coordinate_update;
   hd.threshold = c->opts.threshold;
   hd.core = c->ham.core;
   hd.ham_hash = ham_hash(c->ham);
@@ -775,10 +788,13 @@ int mcdfci_checkpoint_load(mcdfci_ctx* c, const char* path) {
     return fail(c, MCDFCI_E_INPUT, std::string(path) + " is not a checkpoint of this library version");
   if (hd.nio != c->nio || hd.nw != c->nw || hd.norb != c->ham.norb || hd.nelec != c->ham.nelec ||
       hd.ms2 != c->ham.ms2 || hd.k != c->k || hd.M != c->M || hd.precision != c->opts.precision ||
-      hd.threshold != c->opts.threshold || hd.ham_hash != ham_hash(c->ham))
+      hd.coordinate_update != c->opts.coordinate_update || hd.threshold != c->opts.threshold ||
+      hd.ham_hash != ham_hash(c->ham))
     return fail(c, MCDFCI_E_INPUT,
                 "the checkpoint was written for another Hamiltonian or other num_of_coordinate / estimate / "
-                "precision / threshold options");
+                "precision / coordinate_update / threshold options");
+  if (hd.na < 0 || hd.nb < 0 || hd.na + hd.nb != c->ham.nelec || hd.na > c->ham.norb || hd.nb > c->ham.norb)
+    return fail(c, MCDFCI_E_INPUT, "the checkpoint's electron sector does not match NELEC");
   if (fread(&s, sizeof(s), 1, fc.f) != 1) return fail(c, MCDFCI_E_INPUT, "truncated checkpoint");
   std::vector<int> ccnt(c->k);
   std::vector<CandEnt> cand((size_t)c->k * c->M);
@@ -964,6 +980,7 @@ int mcdfci_report_get(mcdfci_ctx* c, mcdfci_report* r) {
   r->max_diff_inv_iter = s.max_diff_inv;
   r->table_slots = (int64_t)c->nslots;
   r->max_size = c->max_size;
+  r->eig_unconverged = s.eig_unconverged;
   {
     std::vector<int> cnt(c->k);
     HIPCHECK(c, hipMemcpy(cnt.data(), c->cand_cnt, sizeof(int) * c->k, hipMemcpyDeviceToHost));
@@ -1043,42 +1060,6 @@ int mcdfci_column(mcdfci_ctx* c, const uint64_t* det, uint64_t* dets, double* va
     store_det(dets + i * c->nio, hd[order[i]], c->nio);
     vals[i] = hv[order[i]];
   }
-  return MCDFCI_OK;
-}
-
-// Diagnostic (not part of the public header): phase timestamps of the last
-// k_eig launch in wall-clock ticks (slots 0-3, 7-9), and the Lanczos vectors
-// of its first attempt (10), its attempts (11) and the last attempt's
-// vectors (12).
-int mcdfci_diag_eig_profile(mcdfci_ctx* c, int64_t* out16) {
-  if (!c || !out16) return MCDFCI_E_INPUT;
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  DevState s;
-  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 16; ++i) out16[i] = s.prof[i];
-  return MCDFCI_OK;
-}
-
-// Diagnostic (not part of the public header): the profile slots of the
-// last mcdfci_eig_update call.
-int mcdfci_diag_hook_profile(mcdfci_ctx* c, int64_t* out16) {
-  if (!c || !out16) return MCDFCI_E_INPUT;
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  DevState s;
-  HIPCHECK(c, hipMemcpy(&s, c->st_scratch, sizeof(s), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 16; ++i) out16[i] = s.prof[i];
-  return MCDFCI_OK;
-}
-
-// Diagnostic (not part of the public header): the last iteration's picked
-// determinants and dx.
-int mcdfci_diag_last_step(mcdfci_ctx* c, uint64_t* dets, double* dx) {
-  if (!c) return MCDFCI_E_INPUT;
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  std::vector<D2> hd(c->k);
-  HIPCHECK(c, hipMemcpy(hd.data(), c->pdet, sizeof(D2) * (size_t)c->k, hipMemcpyDeviceToHost));
-  for (int i = 0; i < c->k; ++i) store_det(dets + (size_t)i * c->nio, hd[i], c->nio);
-  HIPCHECK(c, hipMemcpy(dx, c->dx, 8 * (size_t)c->k, hipMemcpyDeviceToHost));
   return MCDFCI_OK;
 }
 
@@ -1186,6 +1167,23 @@ int mcdfci_table_dump(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
     if (x) x[i] = hx[j];
     if (z) z[i] = fix_to_d(fix_join(hz[2 * j], hz[2 * j + 1]));
   }
+  return MCDFCI_OK;
+}
+
+int mcdfci_table_digest(mcdfci_ctx* c, uint64_t* out3) {
+  if (!c || !out3) return fail(c, MCDFCI_E_INPUT, "null argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  unsigned long long* acc = nullptr;
+  int rc = dalloc(c, &acc, 3);
+  if (rc) return rc;
+  HIPCHECK(c, hipMemsetAsync(acc, 0, 24, c->stream));
+  launch_table_digest(c->stream, c->nw, c->tab, c->nslots, acc);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  unsigned long long h[3];
+  HIPCHECK(c, hipMemcpy(h, acc, 24, hipMemcpyDeviceToHost));
+  dfree(c, acc);
+  for (int i = 0; i < 3; ++i) out3[i] = h[i];
   return MCDFCI_OK;
 }
 

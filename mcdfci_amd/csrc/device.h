@@ -102,7 +102,7 @@ struct DevState {
   double max_residual, curr_max_abs_evec, sum_diff_inv, max_diff_inv, curr_diff_inv;
   long long sum_iterations, sum_inv_iter, curr_inv_iter;
   int lz_prev;     // Lanczos vectors used by the previous subspace solve
-  long long prof[16];  // phase timestamps of the last k_eig (wall clock ticks)
+  long long eig_unconverged;  // subspace solves accepted after the last restart without passing the residual test
 };
 
 enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9 };

@@ -282,6 +282,9 @@ double run(Value& option) {
           r.max_residual, r.curr_max_abs_evec, (int)r.sum_iterations,
           ((double)r.sum_inv_iter) / (double)r.sum_iterations, (int)r.curr_inv_iter,
           r.sum_diff_inv_iter / (double)r.sum_iterations, r.curr_diff_inv_iter, r.max_diff_inv_iter);
+      if (r.eig_unconverged > 0)
+        printf("[LOG] subspace solves used without passing the residual test: %lld.\n",
+               (long long)r.eig_unconverged);
       fflush(stdout);
       if (zts) {
         double inc = 1000.0 * (double)(r.z_size - last_z) / (double)r.candidates / (double)report_interval;

@@ -447,11 +447,6 @@ __device__ __forceinline__ double tri_lowest(int n, const double* d, const doubl
   return lam;
 }
 
-#define EIG_STAMP(i) \
-  do {                 \
-    if (threadIdx.x == 0 && prof) prof[i] = (long long)wall_clock64(); \
-  } while (0)
-
 // Eigenvector of the Lanczos matrix T_m for eigenvalue lam (thread 0; lam
 // within rounding of the root): two inverse-iteration solves from the
 // all-ones vector on one factorisation, each followed by division by the
@@ -477,8 +472,7 @@ __device__ void tri_vector(EigLds& L, int m, double lam) {
 // double-double Ritz vector (first nonzero entry positive) in L.q and the
 // number of Lanczos vectors in L.iv[2].
 template <class Mat>
-__device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, double perturb, double* Q,
-                               long long* prof) {
+__device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, double perturb, double* Q) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int mcap = lz_capacity(n);
   double QC[kRows], QP[kRows], W[kRows];
@@ -640,7 +634,6 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     bprev2 = bprev;
     bprev = beta;
   }
-  EIG_STAMP(3);
   // Ritz vector v = sum_i y_i q_i (every wave, all rows), normalised in
   // double-double.
   double V[kRows];
@@ -697,8 +690,6 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   __shared__ EigLds L;
   __shared__ double Qlds[kQBudget];  // A (n <= kEigLdsN), then the Lanczos basis
   if (st->error) return;
-  long long* prof = st->prof;
-  EIG_STAMP(0);
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int i = tid; i < k; i += bs) {
     L.x[i] = px[i];
@@ -764,8 +755,6 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     __syncthreads();
   }
   const int first0 = st->lz_prev - 1 > 4 ? st->lz_prev - 1 : 4;
-  EIG_STAMP(1);
-  EIG_STAMP(2);
   const MatLds Ml{Qlds, n};
   double* const Qb = Qlds + (in_lds ? n * n : 0);
   // Solve, then accept when |A v - rq v| <= kLzAccept |T|_inf; otherwise
@@ -777,9 +766,9 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   for (;;) {
     // Separate inlined calls keep the LDS case on ds_* instructions.
     if (in_lds)
-      lanczos_lowest(L, Ml, n, first, perturb, Qb, prof);
+      lanczos_lowest(L, Ml, n, first, perturb, Qb);
     else
-      lanczos_lowest(L, Ms, n, first, perturb, Qb, prof);
+      lanczos_lowest(L, Ms, n, first, perturb, Qb);
     for (int i = tid; i < n; i += bs) {
       L.ev[i] = L.q[i].hi;
       L.dl[i] = L.q[i].lo;
@@ -801,19 +790,17 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       return t * t;
     });
     ++attempts;
-    // Diagnostics: Lanczos vectors of the first and last attempt, attempts.
-    if (tid == 0 && prof) {
-      if (attempts == 1) prof[10] = L.iv[2];
-      prof[11] = attempts;
-      prof[12] = L.iv[2];
+    if (__builtin_sqrt(res2) <= kLzAccept * L.sc[1]) break;
+    if (attempts == kLzAttempts) {
+      // accepted without passing the residual test: counted, reported
+      if (tid == 0) st->eig_unconverged += 1;
+      break;
     }
-    if (__builtin_sqrt(res2) <= kLzAccept * L.sc[1] || attempts == kLzAttempts) break;
     for (int i = tid; i < n; i += bs) L.v0[i] = L.ev[i];
     __syncthreads();
     first = 4;
     perturb = 0.0;
   }
-  EIG_STAMP(7);
   if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
   const double lam = rq.hi;
   if (normal) {
@@ -859,7 +846,6 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
       st->sc_lo = 0.0;
     }
   }
-  EIG_STAMP(8);
   if (mode != 0) return;
   __syncthreads();
   // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
@@ -901,7 +887,6 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     st->x_size += n;
     st->k_cur = k;
   }
-  EIG_STAMP(9);
 }
 
 // Line-search update of the one picked coordinate (src/solver.h:254-314):

@@ -50,6 +50,8 @@ void launch_access_probe(hipStream_t s, int nw, void* tab, uint64_t mask, long l
 // Column elements of det: two words per determinant in odet.
 void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double diag, uint64_t* odet, double* oval,
                         unsigned long long* count, long long cap);
+// out[3] += {occupied slots, sum of key hashes, sum of entry hashes} (mod 2^64).
+void launch_table_digest(hipStream_t s, int nw, const void* tab, uint64_t nslots, unsigned long long* out);
 // Occupied slots: two words per determinant in odet, z as (zA, zB) pairs.
 void launch_table_compact(hipStream_t s, int nw, const void* tab, uint64_t nslots, uint64_t* odet, double* ox,
                           int64_t* oz, unsigned long long* count, long long cap);

@@ -1518,6 +1518,54 @@ __global__ void k_table_compact(const SlotW<W>* tab, uint64_t nslots, uint64_t* 
   }
 }
 
+// Order-independent digest of the table: {occupied slots, sum of key
+// hashes, sum of (key, x, z) hashes}, sums modulo 2^64.  The entry hash
+// chains mix64 over word 0 (offset by the golden-ratio constant), word 1,
+// the bits of x (0 for either zero) and the two halves of the 128-bit
+// fixed-point z; the CPU oracle computes the same digest of its table.
+__device__ __forceinline__ void digest_entry(const D2& d, double x, i128 z, uint64_t& kh, uint64_t& eh) {
+  uint64_t e = mix64(d.lo + 0x9e3779b97f4a7c15ULL);
+  e = mix64(e ^ d.hi);
+  kh = e;
+  e = mix64(e ^ (x == 0.0 ? 0ULL : bitsd(x)));
+  e = mix64(e ^ (uint64_t)(z >> 64));
+  eh = mix64(e ^ (uint64_t)z);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_table_digest(const SlotW<W>* tab, uint64_t nslots,
+                                                      unsigned long long* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long cnt = 0, ks = 0, es = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    if (i == 0xffffffffULL) continue;
+    const SlotW<W> s = tab[i];
+    D2 d;
+    if constexpr (W == 1) {
+      if (s.key == 0) continue;
+      d = D2{s.key & kKeyMask, 0};
+    } else {
+      if (s.k1 == 0) continue;
+      d = D2{s.k0, s.k1 & kHiMask};
+    }
+    uint64_t kh, eh;
+    digest_entry(d, s.x, fix_join(s.za, s.zb), kh, eh);
+    cnt += 1;
+    ks += kh;
+    es += eh;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    cnt += __shfl_xor(cnt, off, 64);
+    ks += __shfl_xor(ks, off, 64);
+    es += __shfl_xor(es, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && cnt) {
+    atomicAdd(&out[0], cnt);
+    atomicAdd(&out[1], ks);
+    atomicAdd(&out[2], es);
+  }
+}
+
 // ------------------------------------------------------------------ launch wrappers
 #define MC_W(W_, ...)      \
   do {                     \
@@ -1741,6 +1789,12 @@ void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double d
       hipLaunchKernelGGL((k_column_dump<false, W>), dim3(nchunks), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
                          cap);
   });
+}
+
+void launch_table_digest(hipStream_t s, int nw, const void* tab, uint64_t nslots, unsigned long long* out) {
+  const uint64_t blocks = std::min<uint64_t>((nslots + 255) / 256, kSlotBlocks / 16);
+  MC_W(nw, hipLaunchKernelGGL(k_table_digest<W>, dim3((unsigned)blocks), dim3(256), 0, s, (const SlotW<W>*)tab,
+                              nslots, out));
 }
 
 void launch_table_compact(hipStream_t s, int nw, const void* tab, uint64_t nslots, uint64_t* odet, double* ox,

@@ -204,9 +204,17 @@ struct Ham {
   int ns = 0;  // spin-orbitals
   int nelec = 0, ms2 = 0, nw = 1;
   double core = 0.0;
+  // Mixed precision (the device's precision = mixed): column elements other
+  // than the diagonal are evaluated in fp32 from fp32 roundings of the fp64
+  // integrals, in the device's operation order; the diagonal, the
+  // off-diagonal elements of the subspace matrix and everything downstream
+  // stay fp64.
+  bool mixed = false;
   struct D2 {
     int a, b;
     double v;
+    float v32;         // fp32 value: (float)<ij|ab> - (float)<ij|ba> or (float)<ij|ab>
+    bool ok64, ok32;   // |v| > threshold, |v32| > threshold
   };
   std::vector<std::vector<D2>> dbl;  // ns*ns, i < j
   struct S1 {
@@ -234,14 +242,19 @@ struct Ham {
       int a = (int)(kv.first >> 16) & 0xffff, b = (int)kv.first & 0xffff;
       double v = kv.second;
       if (i == j) continue;
+      // An entry is kept when its fp64 value or its fp32 value passes the
+      // strict threshold; each precision's column uses its own flag.
+      auto keep = [&](int ea, int eb, double w, float w32) {
+        D2 e{ea, eb, w, w32, std::fabs(w) > thr, std::fabs((double)w32) > thr};
+        if (e.ok64 || e.ok32) dbl[i * ns + j].push_back(e);
+      };
       if ((i % 2) ^ (j % 2)) {
-        if (std::fabs(v) > thr) dbl[i * ns + j].push_back({a, b, v});
+        keep(a, b, v, (float)v);
       } else if (a < b) {
-        double w = v - f.get2(i, j, b, a);
-        if (std::fabs(w) > thr) dbl[i * ns + j].push_back({a, b, w});
+        const double u = f.get2(i, j, b, a);
+        keep(a, b, v - u, (float)v - (float)u);
       } else if (b < a && f.g.find(Fcidump::key(i, j, b, a)) == f.g.end()) {
-        double w = -v;
-        if (std::fabs(w) > thr) dbl[i * ns + j].push_back({b, a, w});
+        keep(b, a, -v, -(float)v);
       }
     }
     for (auto& l : dbl) {
@@ -300,8 +313,16 @@ struct Ham {
       for (const S1& e : sgl[i]) {
         int a = e.a;
         if (D.occ(a)) continue;
-        double v = e.h;
-        for (int k : occ) v += e.two[k];
+        double v;
+        if (mixed) {
+          // fp32 sum in the same ascending order as the fp64 one.
+          float f32 = (float)e.h;
+          for (int k : occ) f32 = f32 + (float)e.two[k];
+          v = (double)f32;
+        } else {
+          v = e.h;
+          for (int k : occ) v += e.two[k];
+        }
         Det nd = D;
         nd.clr(i);
         if (parity(nd, i, a)) v = -v;
@@ -313,8 +334,9 @@ struct Ham {
       for (size_t y = x + 1; y < occ.size(); ++y) {
         int i = occ[x], j = occ[y];
         for (const D2& e : dbl[i * ns + j]) {
+          if (!(mixed ? e.ok32 : e.ok64)) continue;
           if (D.occ(e.a) || D.occ(e.b)) continue;
-          double v = e.v;
+          double v = mixed ? (double)e.v32 : e.v;
           Det nd = D;
           nd.clr(i);
           int s1 = parity(nd, i, e.a);
@@ -355,6 +377,7 @@ struct Ham {
     } else if (pc == 4) {
       int i = io[0], j = io[1];
       for (const D2& e : dbl[i * ns + j]) {
+        if (!e.ok64) continue;
         if ((e.a == ao[1] && e.b == ao[0]) || (e.a == ao[0] && e.b == ao[1])) {
           double v = e.v;
           Det nd = Da;
@@ -761,6 +784,10 @@ void oc_info(void* p, int* out) {
 
 double oc_core(void* p) { return ((Ctx*)p)->h.core; }
 
+// Column precision of the following column evaluations: 0 fp64, 1 mixed
+// (fp32 off-diagonal column elements, as the device's mixed precision).
+void oc_set_precision(void* p, int mixed) { ((Ctx*)p)->h.mixed = mixed != 0; }
+
 void oc_hf(void* p, uint64_t* det) {
   Ctx* c = (Ctx*)p;
   det_out(c->h.hartree_fock(), c->h.nw, det);
@@ -920,9 +947,9 @@ int oc_solver_step(void* p, long n) {
   return 0;
 }
 
-// out[0..15]: energy, dx_max, x_size, z_size, cols_size, updates, scale.hi,
+// out[0..16]: energy, dx_max, x_size, z_size, cols_size, updates, scale.hi,
 // xx.hi, xx.lo, scale.lo, xz (double), iterations, max_residual,
-// curr_max_abs_evec, curr_diff_inv_iter, error
+// curr_max_abs_evec, curr_diff_inv_iter, error, unconverged subspace solves
 void oc_solver_report(void* p, double* out) {
   Ctx* c = (Ctx*)p;
   Solver& s = c->s;
@@ -944,6 +971,7 @@ void oc_solver_report(void* p, double* out) {
   out[13] = s.st.curr_max_abs_evec;
   out[14] = s.st.curr_diff_inv_iter;
   out[15] = (double)s.error;
+  out[16] = (double)s.st.eig_unconverged;
 }
 
 void oc_solver_timers(void* p, double* out) {
@@ -974,6 +1002,29 @@ long oc_table_dump(void* p, uint64_t* dets, double* x, double* z, int64_t* zfix,
     }
   }
   return (long)all.size();
+}
+
+// Order-independent digest of the table, the definition of
+// mcdfci_table_digest: {entries, sum of key hashes, sum of entry hashes}.
+void oc_table_digest(void* p, uint64_t* out) {
+  Ctx* c = (Ctx*)p;
+  uint64_t cnt = 0, ks = 0, es = 0;
+  for (auto& sh : c->s.shards)
+    for (auto& en : sh.slots) {
+      if (!en.used) continue;
+      uint64_t e = mix64(en.key.w[0] + 0x9e3779b97f4a7c15ULL);
+      e = mix64(e ^ en.key.w[1]);
+      ks += e;
+      uint64_t xb = 0;
+      if (en.x != 0.0) std::memcpy(&xb, &en.x, 8);
+      e = mix64(e ^ xb);
+      e = mix64(e ^ (uint64_t)(en.z >> 64));
+      es += mix64(e ^ (uint64_t)en.z);
+      ++cnt;
+    }
+  out[0] = cnt;
+  out[1] = ks;
+  out[2] = es;
 }
 
 long oc_candidates(void* p, uint64_t* dets, double* x, double* z, long cap) {

@@ -369,6 +369,7 @@ struct EigStats {
   double max_diff_inv_iter = 0.0;
   double curr_diff_inv_iter = 0.0;
   int lz_prev = 0;  // Lanczos vectors used by the previous solve
+  long eig_unconverged = 0;  // solves accepted after the last restart without passing the residual test
 };
 
 enum { EIG_OK = 0, EIG_NONNEG = 1 };
@@ -414,6 +415,10 @@ static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st)
     S.attempts = attempt + 1;
     if (attempt == 0) S.steps_first = S.ep.steps;
     if (std::sqrt(S.res2) <= kLzAccept * S.ep.tn) break;
+    if (attempt + 1 == kLzAttempts) {
+      st.eig_unconverged += 1;
+      break;
+    }
     v0 = S.ev;
     first = 4;
     pert = 0.0;

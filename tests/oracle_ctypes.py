@@ -35,6 +35,8 @@ def oracle_lib():
         lib = C.CDLL(ORACLE_SO)
         _sig(lib, "oc_last_error", C.c_char_p, [])
         _sig(lib, "oc_create", C.c_void_p, [C.c_char_p, C.c_double])
+        _sig(lib, "oc_set_precision", None, [C.c_void_p, C.c_int])
+        _sig(lib, "oc_table_digest", None, [C.c_void_p, _u64p])
         _sig(lib, "oc_destroy", None, [C.c_void_p])
         _sig(lib, "oc_info", None, [C.c_void_p, _i32p])
         _sig(lib, "oc_core", C.c_double, [C.c_void_p])
@@ -102,7 +104,7 @@ def ref_lib():
 class Oracle:
     """Thin object wrapper over one oracle context."""
 
-    def __init__(self, fcidump: str, threshold: float = 0.0):
+    def __init__(self, fcidump: str, threshold: float = 0.0, precision: str = "fp64"):
         self.lib = oracle_lib()
         self.h = self.lib.oc_create(fcidump.encode(), threshold)
         if not self.h:
@@ -110,6 +112,13 @@ class Oracle:
         info = np.zeros(4, np.int32)
         self.lib.oc_info(self.h, info)
         self.nw, self.nso, self.nelec, self.ms2 = (int(v) for v in info)
+        self.set_precision(precision)
+
+    def set_precision(self, precision: str):
+        """'fp64' or 'mixed' (fp32 off-diagonal column elements, as the device)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(precision)
+        self.lib.oc_set_precision(self.h, 1 if precision == "mixed" else 0)
 
     def close(self):
         if self.h:
@@ -165,11 +174,11 @@ class Oracle:
         return out[0], (out[1], out[2])
 
     def report(self):
-        out = np.zeros(16)
+        out = np.zeros(17)
         self.lib.oc_solver_report(self.h, out)
         keys = ["energy", "dx_max", "x_size", "z_size", "cols_size", "updates", "scale",
                 "xx", "xx_lo", "scale_lo", "xz", "iterations", "max_residual",
-                "max_abs_evec", "diff_inv_iter", "error"]
+                "max_abs_evec", "diff_inv_iter", "error", "eig_unconverged"]
         return dict(zip(keys, out.tolist()))
 
     def table(self):
@@ -181,6 +190,12 @@ class Oracle:
         zf = np.zeros(2 * max(n, 1), np.int64)
         self.lib.oc_table_dump(self.h, dets, x, z, zf.ctypes.data, n)
         return dets[: n * self.nw].reshape(n, self.nw), x[:n], z[:n], zf[: 2 * n].reshape(n, 2)
+
+    def table_digest(self):
+        """{entries, key-hash sum, entry-hash sum} (see mcdfci_table_digest)."""
+        out = np.zeros(3, np.uint64)
+        self.lib.oc_table_digest(self.h, out)
+        return tuple(int(v) for v in out)
 
     def candidates(self):
         cap = 1 << 20

@@ -48,6 +48,18 @@ def test_no_device_fails_loudly_or_runs():
         mcdfci_amd.Cdfci(os.path.join(ROOT, "tests", "golden", "h2o_sto3g.FCIDUMP"), max_memory=0.01)
 
 
+def test_table_above_2_32_slots_is_refused_before_any_allocation():
+    """Slot indices are 32-bit: max_memory = 250 GB on a one-word FCIDUMP
+    asks for 2^33 slots (src/run_wrapper.h:203-213 sizing) and is refused
+    with MCDFCI_E_UNSUPPORTED before the device is touched (so this runs
+    without one); 2^32 slots (C5's 200 GB at two words) are accepted by the
+    same check."""
+    with pytest.raises(mcdfci_amd.McdfciError) as ei:
+        mcdfci_amd.Cdfci(os.path.join(ROOT, "tests", "golden", "h2o_sto3g.FCIDUMP"), max_memory=250.0)
+    assert ei.value.code == 7
+    assert "2^32" in str(ei.value)
+
+
 def test_synthetic_generator_is_deterministic(tmp_path):
     a = synth.generate("c1", str(tmp_path / "a"))
     b = synth.generate("c1", str(tmp_path / "b"))

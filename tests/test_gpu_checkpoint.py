@@ -7,7 +7,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from mcdfci_amd import Cdfci, McdfciError
+from mcdfci_amd import Cdfci, McdfciError, UPDATE_LS
 
 
 def _same_state(a, b):
@@ -61,5 +61,29 @@ def test_resume_refuses_other_options(workloads, tmp_path):
         c.load(ck)
     with pytest.raises(McdfciError):
         b.load(str(tmp_path / "missing.ck"))
+    for x in (a, b, c):
+        x.close()
+
+
+def test_resume_refuses_another_coordinate_update(workloads, tmp_path):
+    """A line-search checkpoint (k = 1) does not load into an eigen-update
+    context of the same k; it does load into a line-search context and
+    continues the run."""
+    path = workloads("c3_synth")["fcidump"]
+    kw = dict(num_of_coordinate=1, estimate=8, max_memory=0.05, threshold=0.0)
+    a = Cdfci(path, coordinate_update=UPDATE_LS, **kw)
+    a.init()
+    a.step(5)
+    ck = str(tmp_path / "ls.ck")
+    a.save(ck)
+    b = Cdfci(path, **kw)
+    with pytest.raises(McdfciError) as ei:
+        b.load(ck)
+    assert ei.value.code == 1 and "coordinate_update" in str(ei.value)
+    c = Cdfci(path, coordinate_update=UPDATE_LS, **kw)
+    c.load(ck)
+    a.step(3)
+    c.step(3)
+    assert a.report()["energy"] == c.report()["energy"]
     for x in (a, b, c):
         x.close()

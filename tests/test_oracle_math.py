@@ -426,3 +426,35 @@ def test_line_search_takes_one_coordinate(workloads):
     o.solver_init(4, 8, 0.0, 103546.0, nthreads=1)
     with pytest.raises(RuntimeError):
         o.set_update(1)
+
+
+def _mix64(x):
+    m = (1 << 64) - 1
+    x ^= x >> 33
+    x = (x * 0xff51afd7ed558ccd) & m
+    x ^= x >> 33
+    x = (x * 0xc4ceb9fe1a85ec53) & m
+    x ^= x >> 33
+    return x
+
+
+def test_table_digest_definition(workloads):
+    """The oracle's table digest is the documented function of the dumped
+    entries (key words, x bits with 0 for a zero x, exact 128-bit z)."""
+    import struct
+    path = workloads("c1")["fcidump"]
+    o = Oracle(path)
+    o.solver_init(8, 8, 0.0, 103546.0, nthreads=2)
+    o.step(40)
+    td, tx, tz, zf = o.table()
+    m = (1 << 64) - 1
+    ks = es = 0
+    for d, x, (zh, zl) in zip(td[:, 0].tolist(), tx.tolist(), zf.tolist()):
+        e = _mix64((d + 0x9e3779b97f4a7c15) & m)
+        e = _mix64(e ^ 0)
+        ks = (ks + e) & m
+        xb = 0 if x == 0.0 else struct.unpack("<Q", struct.pack("<d", x))[0]
+        e = _mix64(e ^ xb)
+        e = _mix64(e ^ (zh & m))
+        es = (es + _mix64(e ^ (zl & m))) & m
+    assert o.table_digest() == (len(td), ks, es)
