@@ -196,6 +196,16 @@ int mcdfci_submatrix(mcdfci_ctx* ctx, int k, const uint64_t* dets, double* H);
  * s[6] = {xx.hi, xx.lo, scale.hi, scale.lo, xz.hi, xz.lo} in, xx/scale out. */
 int mcdfci_eig_update(mcdfci_ctx* ctx, int k, const double* H, const double* x, const double* z,
                       double* s, double* dx);
+/* mcdfci_eig_update for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, results
+ * and errors as mcdfci_eig_update for k <= 256; k above 256 returns
+ * MCDFCI_E_INPUT.  Calls no hipMalloc / hipFree: H_sub, x, z, dx and the
+ * solver state live in a fixed 518 KiB device array that each GPU holds
+ * until the process ends.  Calls that overlap on one GPU (any context)
+ * race on that array. */
+int mcdfci_eig_update_noalloc(mcdfci_ctx* ctx, int k, const double* H, const double* x, const double* z,
+                              double* s, double* dx);
 /* Contents of the wavefunction table, sorted by determinant
  * (WaveFunctionCuckoo::fetch_val over all entries, src/wavefunction.h:631-651). */
 int mcdfci_table_dump(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int64_t cap, int64_t* n);

@@ -100,6 +100,7 @@ SIGNATURES = {
     "mcdfci_column": (C.c_int, [C.c_void_p, _u64p, _u64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_submatrix": (C.c_int, [C.c_void_p, C.c_int, _u64p, _f64p]),
     "mcdfci_eig_update": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
+    "mcdfci_eig_update_noalloc": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]),
     "mcdfci_table_dump": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                     C.POINTER(C.c_int64)]),
     "mcdfci_table_digest": (C.c_int, [C.c_void_p, _u64p]),

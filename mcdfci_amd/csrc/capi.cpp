@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -20,9 +22,35 @@
 
 using namespace mc;
 
+// Scratch of mcdfci_eig_update_noalloc, the subspace update that calls no
+// device allocator: one copy per GPU (each device holds its own at its own
+// address), kept until the process ends, sized for k <= kMaxK:
+//   [DevState, 32 doubles] [H_sub k*k] [x k] [z k] [dx k]  (packed by k).
+// Host code never passes `scratch` itself (a host-side stand-in) to a
+// kernel or a copy: it uses the per-device address from
+// hipGetSymbolAddress, taken once per device at mcdfci_create.
+constexpr size_t kNoAllocState = 32;
+constexpr size_t kNoAllocDoubles = kNoAllocState + (size_t)kMaxK * kMaxK + 3 * (size_t)kMaxK;
+static_assert(sizeof(DevState) <= kNoAllocState * sizeof(double), "DevState outgrows its scratch slot");
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+
 namespace {
 
 std::string g_err;
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;  // device ordinal -> address of its `scratch`
+
+// Mirrors the upper triangle (j > i) of the row-major k x k matrix into the
+// lower one: the subspace update reads both triangles.
+__global__ void k_mirror_upper(double* H, int k) {
+  const long long n = (long long)k * k;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / k), j = (int)(e - (long long)i * k);
+    if (j > i) H[(size_t)j * k + i] = H[(size_t)i * k + j];
+  }
+}
 
 int binom2(int n) { return n < 2 ? 0 : n * (n - 1) / 2; }
 
@@ -57,6 +85,7 @@ struct mcdfci_ctx {
   // solver state and iteration buffers
   DevState* st = nullptr;
   DevState* st_scratch = nullptr;
+  double* noalloc = nullptr;  // this device's copy of `scratch` (mcdfci_eig_update_noalloc)
   int k = 1, M = 1;
   D2* pdet = nullptr;
   uint32_t* pslot = nullptr;
@@ -493,6 +522,21 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   }
   if (!rc) rc = dalloc(c, &c->st, 1);
   if (!rc) rc = dalloc(c, &c->st_scratch, 1);
+  if (!rc) {
+    // Address of this device's `scratch`, looked up once per device (the
+    // device is current here).
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(c->device);
+    if (it == g_scratch_addr.end()) {
+      void* p = nullptr;
+      hipError_t e2 = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e2 != hipSuccess || !p)
+        rc = fail(c, MCDFCI_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e2));
+      else
+        it = g_scratch_addr.emplace(c->device, (double*)p).first;
+    }
+    if (!rc) c->noalloc = it->second;
+  }
   if (!rc)
     for (int i = 0; i < 7 && !rc; ++i)
       if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(c, MCDFCI_E_HIP, "hipEventCreate failed");
@@ -1119,6 +1163,50 @@ int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, 
   HIPCHECK(c, hipMemcpy(&st, c->st_scratch, sizeof(st), hipMemcpyDeviceToHost));
   HIPCHECK(c, hipMemcpy(dxo, ddx, 8 * (size_t)k, hipMemcpyDeviceToHost));
   for (void* p : {(void*)h, (void*)dxx, (void*)dz, (void*)ddx}) dfree(c, p);
+  s[0] = st.xx_hi;
+  s[1] = st.xx_lo;
+  s[2] = st.sc_hi;
+  s[3] = st.sc_lo;
+  if (st.error == ERR_NUMERIC) return fail(c, MCDFCI_E_NUMERIC, "non-negative subspace eigenvalue");
+  return MCDFCI_OK;
+}
+
+// mcdfci_eig_update without a device allocation: the inputs, the output dx
+// and the solver state live in this device's `scratch`.  Every call on one
+// GPU uses the same array (calls that overlap on a GPU race); each call
+// writes all of what it reads before the kernels run.
+int mcdfci_eig_update_noalloc(mcdfci_ctx* c, int k, const double* Hin, const double* x, const double* z,
+                              double* s, double* dxo) {
+  if (!c || !Hin || !x || !z || !s || !dxo) return fail(c, MCDFCI_E_INPUT, "null argument");
+  if (k < 1 || k > kMaxK)
+    return fail(c, MCDFCI_E_INPUT,
+                "mcdfci_eig_update_noalloc: k must be in [1, 256] (its scratch holds a 256 x 256 subspace matrix)");
+  if (!c->noalloc) return fail(c, MCDFCI_E_HIP, "no scratch address for this device");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DevState* sd = (DevState*)c->noalloc;
+  double* h = c->noalloc + kNoAllocState;
+  double* xd = h + (size_t)k * k;
+  double* zd = xd + k;
+  double* dxd = zd + k;
+  DevState st{};
+  st.xx_hi = s[0];
+  st.xx_lo = s[1];
+  st.sc_hi = s[2];
+  st.sc_lo = s[3];
+  const i128 xz = to_fix(s[4]) + to_fix(s[5]);
+  st.xz_hi = (int64_t)(xz >> 64);
+  st.xz_lo = (uint64_t)xz;
+  HIPCHECK(c, hipMemcpyAsync(sd, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(h, Hin, 8 * (size_t)k * k, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(xd, x, 8 * (size_t)k, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(zd, z, 8 * (size_t)k, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_mirror_upper, dim3((unsigned)std::min(64, (k * k + 255) / 256)), dim3(256), 0, c->stream, h,
+                     k);
+  launch_eig(c->stream, sd, k, k, h, xd, zd, dxd, nullptr, 0, nullptr, 1);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpyAsync(&st, sd, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(dxo, dxd, 8 * (size_t)k, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
   s[0] = st.xx_hi;
   s[1] = st.xx_lo;
   s[2] = st.sc_hi;
