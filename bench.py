@@ -7,6 +7,8 @@ wavefunction table sized for max_memory = 64 GB.  A "step" is one CDFCI
 iteration (pick, H_sub, subspace update, column scatter, candidate update).
 
 Phases:
+  0. random-slot access ceiling of the same 64 GB table (insert, update and
+     read rates the scatter and pass 2 are measured against).
   1. throughput: the mixed+batched form (k = 64 coordinates per iteration,
      FP32 integrals and H elements, FP64 z / energy / eigensolve): launch
      forms autotuned on real iterations, iterations replayed from HIP graphs
@@ -16,19 +18,30 @@ Phases:
      solver stream gives the average k_scatter duration; algorithmic bytes =
      H-elements per launch * (8N + 24) B (key 8 per word + x 8 + z read 8 +
      z write 8): 32 B for one-word determinants, 40 B for two.
-  3. time to |E - E_ref| < 1e-5 Ha: plain FP64 (k = 8) and mixed+batched
-     each run for a bounded wall time from the same start, both with the
-     same launch machinery (autotune inside the timed run, graph replay);
-     E_ref is the plain run's final energy.
-  4. cpu_baseline: the CPU oracle (a restatement of the reference's OpenMP
+  3. sweep: k in {8, 64, 256} in both precisions, same machinery.
+  4. time to |E - E_ref| < 1e-5 Ha, three paired repetitions, each running
+     from the same start until its table fills: plain FP64 at k = 8, mixed
+     at k = 64, FP64 at k = 64 (batching alone) and mixed at k = 256.
+     E_ref of a repetition is the plain run's energy at its last report
+     before its 64 GB table filled (not a converged energy; its last-report
+     change is recorded).  Ratios are medians over the repetitions.  A
+     mixed run's reported energy uses its fp32 column elements; replaying
+     the run to its crossing report (runs are deterministic) gives the FP64
+     Rayleigh quotient of the same x, and |E_mixed - E_FP64(x)| is reported.
+  5. norb = 42 (two-word determinants), k = 256, mixed, 200 GB table: rate,
+     scatter roofline at 40 B per element, energy drift against an FP64 run
+     of the same iteration count, and the iteration at which the table fills.
+  6. cpu_baseline: the CPU oracle (a restatement of the reference's OpenMP
      path) on a bounded sample of the same workload on this host.
 """
 
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
+import statistics
 import sys
 import tempfile
 import time
@@ -37,6 +50,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BOX_CPU_SHARE = 16  # host threads one GPU's share of the box may use
 
 
 def bytes_per_element(nwords):
@@ -95,20 +109,38 @@ def first_within(traj, e_ref, tol):
 
 
 def measured_traffic(workload, k, precision):
-    """HBM bytes per k_scatter launch from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy-kernel calibration), or
-    None when no summary covers this workload."""
-    path = os.path.join(ROOT, "profiles", "k_scatter_c4_k64_mixed_hbm_traffic.md")
+    """HBM bytes per k_scatter launch from the committed counter summaries
+    (profiles/*hbm_traffic.md: FETCH_SIZE and WRITE_SIZE runs scaled by a
+    copy-kernel calibration), or None when no summary covers this workload."""
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*hbm_traffic.md"))):
+        try:
+            with open(path) as f:
+                for line in f:
+                    w = line.split()
+                    if len(w) == 6 and w[0] == "TRAFFIC" and w[1] == "k_scatter" and w[2] == workload \
+                            and int(w[3]) == k and w[4] == precision:
+                        return int(w[5])
+        except OSError:
+            pass
+    return None
+
+
+def host_info():
+    """nproc (the CPUs this process may run on) and the CPU model name."""
     try:
-        with open(path) as f:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
             for line in f:
-                w = line.split()
-                if len(w) == 6 and w[0] == "TRAFFIC" and w[1] == "k_scatter" and w[2] == workload \
-                        and int(w[3]) == k and w[4] == precision:
-                    return int(w[5])
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
     except OSError:
         pass
-    return None
+    return nproc, model
 
 
 def cpu_baseline(fcidump, k, M, z_thr, max_size, seconds, threads):
@@ -127,6 +159,200 @@ def cpu_baseline(fcidump, k, M, z_thr, max_size, seconds, threads):
     return k * n / dt, n, dt
 
 
+def timed_rate(ctx, k, warmup, steps, tune_rounds, graph):
+    """Autotune, warm up, then time `steps` iterations (graph replay)."""
+    ch = ctx.autotune(tune_rounds) if tune_rounds else None
+    ctx.set_graph(graph)
+    ctx.step(warmup)
+    ctx.report()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    r = ctx.report()
+    d = time.perf_counter() - t0
+    return k * steps / d, d, r, ch
+
+
+def kernel_segment(ctx, n=20):
+    """Per-kernel device times (HIP events on the solver stream) over n
+    iterations, and the report after them."""
+    ctx.set_timing(True)
+    ctx.step(n)
+    ctx.set_timing(False)
+    kt, nt = ctx.kernel_times()
+    return kt, max(nt, 1), ctx.report()
+
+
+def energy_error_at(fcid, k, z_thr, max_mem, iterations, prec):
+    """Replays a run (deterministic) for `iterations` and returns the energy
+    it reports and the FP64 Rayleigh quotient of its x."""
+    from mcdfci_amd import Cdfci
+    c = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem, threshold=0.0,
+              precision=prec)
+    c.init()
+    c.set_graph(20)
+    c.step(iterations)
+    e = c.report()["energy"]
+    e64 = c.energy_fp64()
+    c.close()
+    return {"iterations": iterations, "e_reported": e, "e_fp64_of_x": e64, "abs_diff": abs(e - e64)}
+
+
+def time_to_threshold(fcid, args, z_thr, max_mem):
+    from mcdfci_amd import Cdfci, PREC_FP64, PREC_MIXED
+    forms = [("plain_fp64", args.plain_k, PREC_FP64), ("mixed", args.k, PREC_MIXED),
+             ("fp64_batched", args.k, PREC_FP64)]
+    forms += [(f"mixed_k{kx}", kx, PREC_MIXED) for kx in [int(v) for v in args.ttt_extra_k.split(",") if v.strip()]]
+    reps = []
+    for rep in range(args.ttt_reps):
+        runs = {}
+        for name, kf, prec in forms:
+            c = Cdfci(fcid, num_of_coordinate=kf, estimate=kf, z_threshold=z_thr, max_memory=max_mem,
+                      threshold=0.0, precision=prec)
+            c.init()
+            every = 200 if kf <= 8 else max(1, 1280 // kf)
+            traj, stop = run_for(c, args.ttt_seconds, every, args.tune_rounds, args.graph)
+            c.close()
+            runs[name] = (traj, stop, kf)
+        tp = runs["plain_fp64"][0]
+        e_ref = tp[-1][2]
+        t_plain, i_plain = first_within(tp, e_ref, 1e-5)
+        rec = {"e_ref": e_ref, "e_ref_iterations": tp[-1][1],
+               "e_ref_last_report_change": tp[-1][2] - tp[-2][2] if len(tp) > 1 else None,
+               "e_ref_report_interval": 200, "plain_stop": runs["plain_fp64"][1]}
+        # intermediate targets: the energies the plain run reached at a
+        # quarter, half and three quarters of its wall time
+        inter = {}
+        for frac in (0.25, 0.5, 0.75):
+            tgt = min(tp, key=lambda r: abs(r[0] - frac * tp[-1][0]))
+            inter[str(frac)] = {"energy": tgt[2], "plain_s": tgt[0]}
+        for name, (traj, stop, kf) in runs.items():
+            t, i = first_within(traj, e_ref, 1e-5)
+            rec[name] = {"k": kf, "seconds": t, "iterations": i, "stop": stop, "final_energy": traj[-1][2],
+                         "final": traj[-1]}
+            for frac, v in inter.items():
+                ti, _ = first_within(traj, v["energy"], 0.0)
+                v[f"{name}_s"] = ti
+        for name in runs:
+            if name != "plain_fp64":
+                t = rec[name]["seconds"]
+                rec[name]["speedup_vs_plain"] = (t_plain / t) if (t_plain and t) else None
+        rec["intermediate"] = inter
+        # energy of each form at equal wall time: the mixed run's last report
+        t_last = runs["mixed"][0][-1][0]
+        rec["mixed_minus_plain_at_equal_time"] = runs["mixed"][0][-1][2] - min(
+            tp, key=lambda r: abs(r[0] - t_last))[2]
+        reps.append(rec)
+        log(f"time-to-threshold rep {rep}: " + json.dumps(rec))
+
+    def med(vals):
+        v = [x for x in vals if x is not None]
+        return statistics.median(v) if v else None
+
+    out = {
+        "tolerance_ha": 1e-5,
+        "e_ref_definition": "energy of the plain FP64 k=%d run at its last report (every 200 iterations) before "
+                            "its 64 GB table filled; not a converged energy (see e_ref_last_report_change)"
+                            % args.plain_k,
+        "repetitions": len(reps),
+        "speedup_median": med([r["mixed"]["speedup_vs_plain"] for r in reps]),
+        "speedup_per_rep": [r["mixed"]["speedup_vs_plain"] for r in reps],
+        "batching_only_fp64_k%d_speedup_median" % args.k: med([r["fp64_batched"]["speedup_vs_plain"]
+                                                                for r in reps]),
+        "precision_only_speedup_median": med([
+            (r["fp64_batched"]["seconds"] / r["mixed"]["seconds"])
+            if (r["fp64_batched"]["seconds"] and r["mixed"]["seconds"]) else None for r in reps]),
+        "plain_k": args.plain_k,
+        "batched_k": args.k,
+        "per_rep": reps,
+    }
+    for name, _, _ in forms[3:]:
+        out[f"{name}_speedup_median"] = med([r[name]["speedup_vs_plain"] for r in reps])
+    # |E_mixed - E_FP64(x)| at the crossing of the last repetition, by replay
+    last = reps[-1]
+    errs = {}
+    for name, kf, prec in forms:
+        if prec != PREC_MIXED or not last[name]["iterations"]:
+            continue
+        errs[name] = energy_error_at(fcid, kf, z_thr, max_mem, last[name]["iterations"], prec)
+        log(f"energy error {name}: {errs[name]}")
+    out["mixed_energy_error_at_crossing"] = errs
+    return out
+
+
+def c5_leg(args):
+    """Two-word determinants at norb = 42: mixed k = 256 on the 200 GB table."""
+    from mcdfci_amd import Cdfci, McdfciError, PREC_FP64, PREC_MIXED, synth
+    spec = synth.CONFIGS["c5"]
+    sol = spec.solver
+    tmp = tempfile.mkdtemp(prefix="mcdfci_bench_c5_")
+    info = make_workload("c5", tmp)
+    fcid = info["fcidump"]
+    kk, n = sol["num_of_coordinate"], args.c5_iters
+    res = {"workload": f"c5: synthetic norb={spec.norb} nelec={spec.nelec} (ERI rank {spec.rank}, seed {spec.seed})",
+           "num_of_coordinate": kk, "estimate": kk, "z_threshold": sol["z_threshold"],
+           "max_memory_gb": sol["max_memory"], "precision": "mixed", "iterations": n}
+    c = Cdfci(fcid, num_of_coordinate=kk, estimate=kk, z_threshold=sol["z_threshold"],
+              max_memory=sol["max_memory"], threshold=0.0, precision=PREC_MIXED)
+    c.init()
+    res["table_slots"] = c.report()["table_slots"]
+    c.set_launch(8, True)
+    c.set_graph(0)
+    warm = 4
+    c.step(warm)
+    c.report()
+    t0 = time.perf_counter()
+    c.step(n - warm - 4)
+    r = c.report()
+    d = time.perf_counter() - t0
+    res["coord_updates_per_s"] = kk * (n - warm - 4) / d
+    res["ms_per_iteration"] = 1e3 * d / (n - warm - 4)
+    kt, nt, r2 = kernel_segment(c, 4)
+    elems = r2["cols_size"]
+    sc = kt["scatter"] / nt
+    bpe = bytes_per_element(2)
+    res["kernel_ms_per_iteration"] = {a: 1e3 * b / nt for a, b in kt.items()}
+    res["h_elements_per_iteration"] = elems
+    res["h_elements_per_s"] = elems / (res["ms_per_iteration"] * 1e-3)
+    zs = [r["z_size"], r2["z_size"]]
+    inserts_per_it = (zs[1] - zs[0]) / 4.0
+    res["roofline"] = {"bound": "hbm", "kernel": "k_scatter", "achieved": elems * bpe / sc / 1e9,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": elems * bpe / sc / 1e9 / HBM_PEAK_GBS,
+                       "traffic": measured_traffic("c5", kk, "mixed"), "bytes_per_element": bpe,
+                       "elements_per_launch": elems, "avg_launch_us": 1e6 * sc}
+    # two 64-bit atomic adds per element (exact z) plus one CAS per new key
+    res["scatter_atomic_ops_per_s"] = (2 * elems + inserts_per_it) / sc
+    res["e_mixed"] = r2["energy"]
+    res["e_fp64_of_x"] = c.energy_fp64()
+    res["mixed_abs_error_vs_fp64_of_x"] = abs(res["e_mixed"] - res["e_fp64_of_x"])
+    res["z_size_at_n"] = r2["z_size"]
+    # run on until the table fills: where the 200 GB table overflows
+    it = n
+    full_at = None
+    while it < 2000:
+        try:
+            c.step(8)
+            c.report()
+            it += 8
+        except McdfciError as e:
+            if e.code == 3:
+                full_at = it
+                break
+            raise
+    res["table_full_after_iterations"] = full_at
+    res["table_max_size"] = r2["max_size"]
+    c.close()
+    f = Cdfci(fcid, num_of_coordinate=kk, estimate=kk, z_threshold=sol["z_threshold"],
+              max_memory=sol["max_memory"], threshold=0.0, precision=PREC_FP64)
+    f.init()
+    f.set_launch(8, True)
+    f.step(n)
+    rf = f.report()
+    f.close()
+    res["e_fp64_run"] = rf["energy"]
+    res["drift_mixed_minus_fp64_at_equal_iterations"] = res["e_mixed"] - rf["energy"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,14 +362,17 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--plain-k", type=int, default=8)
     ap.add_argument("--ttt-seconds", type=float, default=25.0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--ttt-reps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--c5-iters", type=int, default=40)
     ap.add_argument("--skip-ttt", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-c5", action="store_true")
     ap.add_argument("--graph", type=int, default=20, help="iterations per captured HIP graph (0: off)")
-    ap.add_argument("--sweep-k", default="8,256",
-                    help="further num_of_coordinate values for the throughput sweep (empty: none)")
+    ap.add_argument("--sweep-k", default="8,64,256",
+                    help="num_of_coordinate values for the throughput sweep, both precisions (empty: none)")
     ap.add_argument("--ttt-extra-k", default="256",
-                    help="further batched num_of_coordinate values timed to the threshold (empty: none)")
+                    help="further mixed num_of_coordinate values timed to the threshold (empty: none)")
     ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (byte-counter calibration)")
@@ -178,7 +407,6 @@ def main():
     log(f"workload {args.workload}: {fcid} sha256 {info['sha256'][:16]} core {info['core']}")
 
     k = args.k
-    # 1. throughput of the mixed + batched form
     mb = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem, threshold=0.0,
                precision=PREC_MIXED)
     # 0. random-slot access ceiling on the same table (before init clears it)
@@ -186,124 +414,78 @@ def main():
     ceiling = {name: n_ops / mb.access_ceiling(mode, n_ops)
                for mode, name in ((0, "insert_ops_per_s"), (1, "update_ops_per_s"), (2, "read_ops_per_s"))}
     log(f"access ceiling on the {max_mem:g} GB table: " + json.dumps(ceiling))
+    # 1. throughput of the mixed + batched form
     mb.init()
-    choice = mb.autotune(args.tune_rounds) if args.tune_rounds else None
-    log(f"autotune: {choice}")
-    mb.set_graph(args.graph)
-    mb.step(args.warmup)
-    r0 = mb.report()
-    t0 = time.perf_counter()
-    mb.step(args.steps)
-    r1 = mb.report()
-    dt = time.perf_counter() - t0
-    value = k * args.steps / dt
+    value, dt, r1, choice = timed_rate(mb, k, args.warmup, args.steps, args.tune_rounds, args.graph)
     ms_per_step = 1e3 * dt / args.steps
+    log(f"autotune: {choice}")
     log(f"throughput: {args.steps} iterations in {dt:.3f} s -> {value:.1f} coord updates/s, "
         f"E={r1['energy']:.10f} |z|_0={r1['z_size']}")
-
     # 2. per-kernel timing segment for the roofline
-    mb.set_timing(True)
-    mb.step(20)
-    mb.set_timing(False)
-    kt, nt = mb.kernel_times()
-    r2 = mb.report()
+    kt, nt, r2 = kernel_segment(mb, 20)
     elems = r2["cols_size"]
-    scatter_s = kt["scatter"] / max(nt, 1)
+    scatter_s = kt["scatter"] / nt
     bpe = bytes_per_element(mb.nwords)
     achieved = elems * bpe / scatter_s / 1e9
-    log("kernel ms/iter: " + ", ".join(f"{a}={1e3 * b / max(nt, 1):.3f}" for a, b in kt.items()))
+    log("kernel ms/iter: " + ", ".join(f"{a}={1e3 * b / nt:.3f}" for a, b in kt.items()))
     log(f"scatter: {elems} H-elements/launch, {1e6 * scatter_s:.1f} us -> {achieved:.1f} GB/s algorithmic")
     h_elem_rate = elems * args.steps / dt
     mb_energy_steps = r1["energy"]
+    mb_e64 = mb.energy_fp64()
     mb.close()
     torch.cuda.synchronize()
 
-    # 2b. throughput sweep over num_of_coordinate (same machinery, 100 steps)
+    # 3. throughput sweep over num_of_coordinate and precision (100 steps each)
     sweep = [{"k": k, "precision": "mixed", "coord_updates_per_s": value, "ms_per_iteration": ms_per_step,
-              "launch": choice}]
+              "launch": choice, "kernel_ms_per_iteration": {a: 1e3 * b / nt for a, b in kt.items()}}]
     for ks in [int(v) for v in args.sweep_k.split(",") if v.strip()]:
         for prec, pname in ((PREC_MIXED, "mixed"), (PREC_FP64, "fp64")):
-            if pname == "fp64" and ks != args.plain_k:
+            if ks == k and pname == "mixed":
                 continue
             c = Cdfci(fcid, num_of_coordinate=ks, estimate=ks, z_threshold=z_thr, max_memory=max_mem,
                       threshold=0.0, precision=prec)
             c.init()
-            ch = c.autotune(args.tune_rounds) if args.tune_rounds else None
-            c.set_graph(args.graph)
-            c.step(args.warmup)
-            c.report()
-            n = 100
-            t0 = time.perf_counter()
-            c.step(n)
-            c.report()
-            d = time.perf_counter() - t0
+            rate, d, _, ch = timed_rate(c, ks, args.warmup, 100, args.tune_rounds, args.graph)
+            kts, nts, _ = kernel_segment(c, 10)
             c.close()
-            sweep.append({"k": ks, "precision": pname, "coord_updates_per_s": ks * n / d,
-                          "ms_per_iteration": 1e3 * d / n, "launch": ch})
+            sweep.append({"k": ks, "precision": pname, "coord_updates_per_s": rate, "ms_per_iteration": 10 * d,
+                          "launch": ch, "kernel_ms_per_iteration": {a: 1e3 * b / nts for a, b in kts.items()}})
             log(f"sweep: {sweep[-1]}")
 
-    # 3. time to |dE| < 1e-5 Ha
-    ttt = None
-    if not args.skip_ttt:
-        plain = Cdfci(fcid, num_of_coordinate=args.plain_k, estimate=args.plain_k, z_threshold=z_thr,
-                      max_memory=max_mem, threshold=0.0, precision=PREC_FP64)
-        plain.init()
-        tp, stop_p = run_for(plain, args.ttt_seconds, 200, args.tune_rounds, args.graph)
-        plain.close()
-        mb2 = Cdfci(fcid, num_of_coordinate=k, estimate=k, z_threshold=z_thr, max_memory=max_mem,
-                    threshold=0.0, precision=PREC_MIXED)
-        mb2.init()
-        tm, stop_m = run_for(mb2, args.ttt_seconds, 20, args.tune_rounds, args.graph)
-        mb2.close()
-        e_ref = tp[-1][2]
-        t_plain, i_plain = first_within(tp, e_ref, 1e-5)
-        t_mb, i_mb = first_within(tm, e_ref, 1e-5)
-        extra = []
-        for kx in [int(v) for v in args.ttt_extra_k.split(",") if v.strip()]:
-            cx = Cdfci(fcid, num_of_coordinate=kx, estimate=kx, z_threshold=z_thr, max_memory=max_mem,
-                       threshold=0.0, precision=PREC_MIXED)
-            cx.init()
-            tx, stop_x = run_for(cx, args.ttt_seconds, max(1, 1280 // kx), args.tune_rounds, args.graph)
-            cx.close()
-            t_x, i_x = first_within(tx, e_ref, 1e-5)
-            extra.append({"batched_k": kx, "mixed_batched_s": t_x, "mixed_batched_iterations": i_x,
-                          "speedup": (t_plain / t_x) if (t_plain and t_x) else None,
-                          "mixed_final_energy": tx[-1][2], "mixed_stop": stop_x, "mixed_last": tx[-1]})
-            log(f"time-to-threshold k={kx}: {extra[-1]}")
-        # energy of each form at equal wall time (the mixed run's last report)
-        t_last = tm[-1][0]
-        e_plain_at = min(tp, key=lambda r: abs(r[0] - t_last))[2]
-        ttt = {
-            "tolerance_ha": 1e-5,
-            "e_ref_plain_fp64": e_ref,
-            "e_ref_iterations": tp[-1][1],
-            "e_ref_last_report_change": tp[-1][2] - tp[-2][2] if len(tp) > 1 else None,
-            "plain_fp64_s": t_plain,
-            "plain_fp64_iterations": i_plain,
-            "mixed_batched_s": t_mb,
-            "mixed_batched_iterations": i_mb,
-            "speedup": (t_plain / t_mb) if (t_plain and t_mb) else None,
-            "mixed_final_energy": tm[-1][2],
-            "mixed_minus_plain_at_equal_time": tm[-1][2] - e_plain_at,
-            "plain_k": args.plain_k,
-            "batched_k": k,
-            "plain_stop": stop_p,
-            "plain_last": tp[-1],
-            "mixed_stop": stop_m,
-            "mixed_last": tm[-1],
-            "other_batched_k": extra,
-        }
-        log("time-to-threshold: " + json.dumps(ttt))
+    # 4. time to |dE| < 1e-5 Ha
+    ttt = None if args.skip_ttt else time_to_threshold(fcid, args, z_thr, max_mem)
+    if ttt:
+        log("time-to-threshold: " + json.dumps({a: b for a, b in ttt.items() if a != "per_rep"}))
 
-    # 4. CPU baseline: the oracle on this host, bounded sample
+    # 5. norb = 42, two-word determinants
+    c5 = None
+    if not args.skip_c5:
+        c5 = c5_leg(args)
+        log("c5: " + json.dumps(c5))
+
+    # 6. CPU baseline: the oracle on this host, bounded samples
     cpu = None
+    nproc, model = host_info()
     if not args.skip_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
         max_size = float(int(0.79 * (1 << 31)))  # the reference's capacity for 64 GB, 1-word keys
-        rate, n, secs = cpu_baseline(fcid, k, k, z_thr, max_size, args.cpu_seconds, threads)
-        cpu = {"value": rate, "unit": "coord updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n} iterations of the same workload at k={k} (the oracle restating the reference's "
-                         f"OpenMP path), {secs:.1f} s on {threads} host threads"}
+        samples = []
+        for kc in (k, args.plain_k):
+            # the reference runs num_threads(k) with k clamped to the host's
+            # processors (src/run_wrapper.h:161-169); this box gives one GPU
+            # a share of BOX_CPU_SHARE threads
+            k_eff = min(kc, nproc)
+            threads = max(1, min(k_eff, BOX_CPU_SHARE))
+            rate, n, secs = cpu_baseline(fcid, kc, kc, z_thr, max_size, args.cpu_seconds, threads)
+            samples.append({"k": kc, "effective_k": k_eff, "threads": threads, "coord_updates_per_s": rate,
+                            "iterations": n, "seconds": secs})
+            log("cpu sample: " + json.dumps(samples[-1]))
+        s0 = samples[0]
+        cpu = {"value": s0["coord_updates_per_s"], "unit": "coord updates/s", "cores": s0["threads"],
+               "kind": "port", "nproc": nproc, "cpu_model": model, "effective_k": s0["effective_k"],
+               "sample": f"{s0['iterations']} iterations of the same workload at k={k} (the oracle restating the "
+                         f"reference's OpenMP path, num_threads = min(k, nproc, {BOX_CPU_SHARE}) = "
+                         f"{s0['threads']}), {s0['seconds']:.1f} s; nproc={nproc}, {model}",
+               "samples": samples}
         log("cpu baseline: " + json.dumps(cpu))
 
     out = {
@@ -328,20 +510,23 @@ def main():
                    "tree_pick": choice["tree_pick"] if choice else None},
         "h_elements_per_s": h_elem_rate,
         "energy_after_steps": mb_energy_steps,
+        "energy_error_after_steps": {"e_mixed": r2["energy"], "e_fp64_of_x": mb_e64,
+                                     "abs_diff": abs(r2["energy"] - mb_e64)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, k, "mixed"),
                      "algorithmic_bytes_per_launch": elems * bpe,
                      "kernel": "k_scatter", "elements_per_launch": elems,
                      "bytes_per_element": bpe, "avg_launch_us": 1e6 * scatter_s},
-        "kernel_ms_per_iteration": {a: 1e3 * b / max(nt, 1) for a, b in kt.items()},
+        "kernel_ms_per_iteration": {a: 1e3 * b / nt for a, b in kt.items()},
         "access_ceiling": dict(ceiling, **{
             "scatter_elements_per_s": elems / scatter_s,
             "scatter_frac_of_update": elems / scatter_s / ceiling["update_ops_per_s"],
-            "pass2_elements_per_s": elems / (kt["pass2"] / max(nt, 1)),
-            "pass2_frac_of_read": elems / (kt["pass2"] / max(nt, 1)) / ceiling["read_ops_per_s"]}),
+            "pass2_elements_per_s": elems / (kt["pass2"] / nt),
+            "pass2_frac_of_read": elems / (kt["pass2"] / nt) / ceiling["read_ops_per_s"]}),
         "sweep": sweep,
         "time_to_threshold": ttt,
+        "c5": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -170,8 +170,10 @@ int mcdfci_checkpoint_load(mcdfci_ctx* ctx, const char* path);
 /* Measurement (no counterpart in the reference): the device time of n_ops
  * random-slot operations on this context's own table -- mode 0 a new-key
  * insert (key load, CAS, two 64-bit atomic adds), 1 an update (key load,
- * two atomic adds), 2 a read of key, x and z -- the access ceiling of the
- * column scatter and pass 2 on the same footprint.  Overwrites the table:
+ * two atomic adds), 2 a read of key, x and z, 3 an update by one FP64
+ * atomic add (key load, one 64-bit floating add: the cost of a z kept as an
+ * order-dependent double) -- the access ceiling of the column scatter and
+ * pass 2 on the same footprint.  Overwrites the table:
  * mcdfci_init must follow before mcdfci_step. */
 int mcdfci_access_ceiling(mcdfci_ctx* ctx, int mode, int64_t n_ops, double* seconds);
 
@@ -238,6 +240,14 @@ int mcdfci_candidates(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int
  * folded onto the unique parameters.  grad_h has norb^2 entries and
  * grad_eri norb^4; energy may be NULL. */
 int mcdfci_energy_gradient(mcdfci_ctx* ctx, double* grad_h, double* grad_eri, double* energy);
+
+/* The FP64 Rayleigh quotient x.Hx / x.x of the current wavefunction x, with
+ * H built from the fp64 integrals whatever the context's precision (the
+ * energy a mixed-precision run would report with exact H elements).  The
+ * reference reports xz / xx only (src/wavefunction.h:195-198); this is the
+ * measure of the mixed form's energy error.  Sums are exact fixed point, so
+ * the value does not depend on thread order. */
+int mcdfci_energy_fp64(mcdfci_ctx* ctx, double* energy);
 
 #ifdef __cplusplus
 }

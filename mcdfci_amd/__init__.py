@@ -106,6 +106,7 @@ SIGNATURES = {
     "mcdfci_table_digest": (C.c_int, [C.c_void_p, _u64p]),
     "mcdfci_candidates": (C.c_int, [C.c_void_p, _u64p, _f64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_energy_gradient": (C.c_int, [C.c_void_p, _f64p, _f64p, C.POINTER(C.c_double)]),
+    "mcdfci_energy_fp64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 }
 
 
@@ -236,7 +237,8 @@ class Cdfci:
 
     def access_ceiling(self, mode, n_ops=1 << 25):
         """Device seconds for n_ops random-slot operations on this context's table
-        (mode 0 insert, 1 update, 2 read).  Overwrites the table: call init() after."""
+        (mode 0 insert, 1 update, 2 read, 3 update by one FP64 atomic).  Overwrites the
+        table: call init() after."""
         sec = C.c_double()
         self._check(self.lib.mcdfci_access_ceiling(self.h, int(mode), int(n_ops), C.byref(sec)))
         return sec.value
@@ -310,3 +312,10 @@ class Cdfci:
         e = C.c_double()
         self._check(self.lib.mcdfci_energy_gradient(self.h, gh, ge, C.byref(e)))
         return e.value, gh.reshape(n, n), ge.reshape(n, n, n, n)
+
+    def energy_fp64(self):
+        """FP64 Rayleigh quotient x.Hx / x.x of the current wavefunction (fp64
+        integrals whatever the precision of the run)."""
+        e = C.c_double()
+        self._check(self.lib.mcdfci_energy_fp64(self.h, C.byref(e)))
+        return e.value

@@ -669,7 +669,7 @@ int mcdfci_step(mcdfci_ctx* c, int64_t n) {
 }
 
 int mcdfci_access_ceiling(mcdfci_ctx* c, int mode, int64_t n_ops, double* seconds) {
-  if (!c || !seconds || mode < 0 || mode > 2 || n_ops < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  if (!c || !seconds || mode < 0 || mode > 3 || n_ops < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   HIPCHECK(c, hipSetDevice(c->device));
   unsigned long long* sink = nullptr;
   int rc = dalloc(c, &sink, 1);
@@ -1308,6 +1308,69 @@ int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
 }
 
 }  // extern "C"
+
+// FP64 Rayleigh quotient x.Hx / x.x of the table's x (fp64 integrals
+// whatever the context's precision).
+int mcdfci_energy_fp64(mcdfci_ctx* c, double* energy) {
+  if (!c || !energy) return fail(c, MCDFCI_E_INPUT, "null argument");
+  if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevState s;
+  HIPCHECK(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  if (s.error) return check_state_error(c, s);
+  long long ub = (long long)std::min<uint64_t>(c->nslots, (uint64_t)(s.x_size + c->k + 1024));
+  D2* od = nullptr;
+  double* ox = nullptr;
+  unsigned long long* cnt = nullptr;
+  int64_t* acc = nullptr;
+  int rc;
+  auto release = [&]() {
+    for (void* p : {(void*)od, (void*)ox, (void*)cnt, (void*)acc}) dfree(c, p);
+  };
+  unsigned long long m = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if ((rc = dalloc(c, &od, ub)) || (rc = dalloc(c, &ox, ub)) || (rc = dalloc(c, &cnt, 1))) {
+      release();
+      return rc;
+    }
+    HIPCHECK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+    launch_compact_x(c->stream, c->nw, c->tab, c->nslots, od, ox, cnt, ub);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&m, cnt, 8, hipMemcpyDeviceToHost));
+    if ((long long)m <= ub) break;
+    for (void* p : {(void*)od, (void*)ox, (void*)cnt}) dfree(c, p);
+    od = nullptr, ox = nullptr, cnt = nullptr;
+    ub = (long long)m;
+  }
+  if ((rc = dalloc(c, &acc, 4))) {
+    release();
+    return rc;
+  }
+  HIPCHECK(c, hipMemsetAsync(acc, 0, 32, c->stream));
+  int64_t q[4];
+  launch_sum_sq(c->stream, ox, (long long)m, 1.0, acc);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(q, acc, 16, hipMemcpyDeviceToHost));
+  const double xx = fix_to_d(fix_join(q[0], q[1]));
+  if (!(xx > 0.0)) {
+    release();
+    return fail(c, MCDFCI_E_NUMERIC, "the wavefunction is zero");
+  }
+  const double scale = 1.0 / std::sqrt(xx);
+  HIPCHECK(c, hipMemsetAsync(acc, 0, 32, c->stream));
+  launch_sum_sq(c->stream, ox, (long long)m, scale, acc);
+  DevHam H = c->dh;
+  launch_rayleigh(c->stream, c->nw, H, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc + 2);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  HIPCHECK(c, hipMemcpy(q, acc, 32, hipMemcpyDeviceToHost));
+  release();
+  *energy = dd_div(fix_to_dd(fix_join(q[2], q[3])), fix_to_dd(fix_join(q[0], q[1]))).hi;
+  return MCDFCI_OK;
+}
 
 // Gradient of the variational energy of the current wavefunction with
 // respect to the unique integrals (see grad.hip).

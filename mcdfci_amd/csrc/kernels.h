@@ -50,6 +50,10 @@ void launch_access_probe(hipStream_t s, int nw, void* tab, uint64_t mask, long l
 // Column elements of det: two words per determinant in odet.
 void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double diag, uint64_t* odet, double* oval,
                         unsigned long long* count, long long cap);
+// acc[2] (zA, zB fixed point) += sum over dets (x != 0, c = x * scale) and their
+// column elements D' of c_D * H_{D'D} (fp64 integrals) * c_D'.
+void launch_rayleigh(hipStream_t s, int nw, const DevHam& H, const void* tab, uint64_t mask, const D2* dets,
+                     const double* xs, long long n, double scale, int64_t* acc);
 // out[3] += {occupied slots, sum of key hashes, sum of entry hashes} (mod 2^64).
 void launch_table_digest(hipStream_t s, int nw, const void* tab, uint64_t nslots, unsigned long long* out);
 // Occupied slots: two words per determinant in odet, z as (zA, zB) pairs.

@@ -1434,8 +1434,10 @@ __global__ void k_cand_resolve(const SlotW<W>* tab, uint64_t mask, CandEnt* cand
 // slot mix64(i + seed) & mask.  mode 0 (insert): relaxed load of the key
 // word, CAS of an empty key, two 64-bit atomic adds; mode 1 (update): load
 // of the key word and the two atomic adds; mode 2 (read): load of the key,
-// x and both z words.  A per-thread sum of what was read goes to *sink so
-// that no load is dead.
+// x and both z words; mode 3 (update, one FP64 atomic): load of the key
+// word and one 64-bit floating-point atomic add, the cost of a z kept as a
+// plain double (whose sums would depend on the order of the adds).  A
+// per-thread sum of what was read goes to *sink so that no load is dead.
 template <int W>
 __global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t mask, long long n, int mode,
                                                       uint64_t seed, unsigned long long* sink) {
@@ -1452,6 +1454,11 @@ __global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t ma
     }
     const uint64_t k = rload(kw);
     acc += k;
+    if (mode == 3) {
+      // one floating-point atomic add in place of the two exact int64 adds
+      unsafeAtomicAdd((double*)&sl->za, 1.0);
+      continue;
+    }
     if (mode == 0 && k == 0) {
       uint64_t expect = 0;
       __hip_atomic_compare_exchange_strong(kw, &expect, ((uint64_t)i & kHiMask) | kOcc, __ATOMIC_RELAXED,
@@ -1514,6 +1521,62 @@ __global__ void k_table_compact(const SlotW<W>* tab, uint64_t nslots, uint64_t* 
       ox[pos] = s.x;
       oz[2 * pos] = s.za;
       oz[2 * pos + 1] = s.zb;
+    }
+  }
+}
+
+// FP64 Rayleigh quotient numerator of the table's x: sum over the
+// determinants D with x != 0 (dets, xs; c = x * scale) and every element
+// D' of D's column of c_D * H_{D'D} * c_D', with H_{D'D} from the fp64
+// integrals (gen_elem<false>, get_diagonal's sequential order for D' = D)
+// whatever precision the context runs in.  Each product is rounded once to
+// the 2^-64 fixed point and summed exactly, so the result does not depend
+// on the visiting order.  One workgroup per (column chunk, determinant).
+template <int W>
+__global__ __launch_bounds__(kWG) void k_rayleigh(DevHam H, const SlotW<W>* tab, uint64_t mask, const D2* dets,
+                                                  const double* xs, long long n, double scale, int64_t* acc) {
+  __shared__ ColCtxW<W> C;
+  __shared__ i128 red[kWG / 64];
+  const int chunk = blockIdx.x;
+  i128 local = 0;
+  for (long long di = blockIdx.y; di < n; di += gridDim.y) {
+    setup_ctx<W>(C, H, dets[di]);
+    const double cD = xs[di] * scale;
+    for (int r = 0; r < kEPT; ++r) {
+      const int e = chunk * kChunk + r * kWG + threadIdx.x;
+      if (e >= H.n_enum) continue;
+      D2 D;
+      double V;
+      if (e == 0) {
+        double v = 0.0;
+        for (int a = 0; a < C.ne; ++a) {
+          v += H.d1[C.occ[a]];
+          for (int b = a + 1; b < C.ne; ++b) v += H.d2[C.occ[a] * H.ns + C.occ[b]];
+        }
+        V = v + H.core;
+        D = C.det;
+      } else if (!gen_elem<false, W>(H, C, e, D, V)) {
+        continue;
+      }
+      const long long si = find_slot<W>(tab, mask, D);
+      if (si < 0) continue;
+      const double xo = tab[si].x;
+      if (!(__builtin_fabs(xo) != 0.0)) continue;
+      local += to_fix(cD * V * (xo * scale));
+    }
+    __syncthreads();  // C is rebuilt for the next determinant
+  }
+  local = wave_sum_i128(local);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i128 t = 0;
+    for (int w = 0; w < kWG / 64; ++w) t += red[w];
+    if (t != 0) {
+      int64_t a, b;
+      fix_split(t, a, b);
+      __hip_atomic_fetch_add(&acc[0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&acc[1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1789,6 +1852,15 @@ void launch_column_dump(hipStream_t s, int nw, const DevHam& H, D2 det, double d
       hipLaunchKernelGGL((k_column_dump<false, W>), dim3(nchunks), dim3(kWG), 0, s, H, det, diag, odet, oval, count,
                          cap);
   });
+}
+
+void launch_rayleigh(hipStream_t s, int nw, const DevHam& H, const void* tab, uint64_t mask, const D2* dets,
+                     const double* xs, long long n, double scale, int64_t* acc) {
+  if (n <= 0) return;
+  const int nchunks = (H.n_enum + kChunk - 1) / kChunk;
+  const long long gy = n < 65535 ? n : 65535;
+  MC_W(nw, hipLaunchKernelGGL(k_rayleigh<W>, dim3(nchunks, (unsigned)gy), dim3(kWG), 0, s, H, (const SlotW<W>*)tab,
+                              mask, dets, xs, n, scale, acc));
 }
 
 void launch_table_digest(hipStream_t s, int nw, const void* tab, uint64_t nslots, unsigned long long* out) {

@@ -1201,7 +1201,9 @@ int oc_energy_gradient(void* p, const uint64_t* dets, const double* x, long n, d
     const double ci = x[i] * scale;
     q += to_fixed(ci * ci);
   }
-  std::vector<i128> gh(nh, 0), ge(n4, 0);
+  // Exact fixed-point sums, so the per-thread partial arrays can be added
+  // in any order.
+  std::vector<i128> gh_all(nh, 0), ge_all(n4, 0);
   auto hi = [&](int a, int b) { return a >= b ? (size_t)a * M + b : (size_t)b * M + a; };
   auto ei = [&](int a, int b, int cc, int d) {
     int p1 = std::max(a, b), q1 = std::min(a, b), p2 = std::max(cc, d), q2 = std::min(cc, d);
@@ -1211,6 +1213,12 @@ int oc_energy_gradient(void* p, const uint64_t* dets, const double* x, long n, d
     }
     return (((size_t)p1 * M + q1) * M + p2) * M + q2;
   };
+#pragma omp parallel
+  {
+  std::vector<i128> gh_t(nh, 0), ge_t(n4, 0);
+  auto& gh = gh_t;
+  auto& ge = ge_t;
+#pragma omp for schedule(dynamic, 256)
   for (long di = 0; di < n; ++di) {
     const Det& d = D[di];
     const double cD = x[di] * scale;
@@ -1277,8 +1285,16 @@ int oc_energy_gradient(void* p, const uint64_t* dets, const double* x, long n, d
           }
       }
   }
+#pragma omp critical
+  {
+    for (size_t i = 0; i < nh; ++i) gh_all[i] += gh_t[i];
+    for (size_t i = 0; i < n4; ++i) ge_all[i] += ge_t[i];
+  }
+  }
   const dd den = fixed_to_dd(q);
   double e = c->h.core;
+  const std::vector<i128>& gh = gh_all;
+  const std::vector<i128>& ge = ge_all;
   for (int a = 0; a < M; ++a)
     for (int b = 0; b < M; ++b) {
       const size_t i = (size_t)a * M + b;

@@ -22,7 +22,7 @@ HEADER = os.path.join(ROOT, "include", "mcdfci.h")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(mcdfci_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mcdfci_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():

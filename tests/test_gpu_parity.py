@@ -287,9 +287,34 @@ def test_mixed_energy_error_vs_fp64_rayleigh_quotient(workloads):
     g.step(300)
     e_mixed = g.report()["energy"]
     e64, _, _ = g.energy_gradient()
-    print("mixed", e_mixed, "fp64(x)", e64, "diff", e_mixed - e64, flush=True)
+    e64b = g.energy_fp64()
+    print("mixed", e_mixed, "fp64(x)", e64, e64b, "diff", e_mixed - e64, flush=True)
+    assert abs(e64b - e64) < 1e-11
     assert abs(e_mixed - e64) < 1e-5
     assert e_mixed != e64
+
+
+@pytest.mark.parametrize("prec", [0, PREC_MIXED])
+def test_energy_fp64_is_the_dense_rayleigh_quotient(prec, workloads):
+    """mcdfci_energy_fp64 equals x.Hx / x.x with H the second-quantised fp64
+    matrix and x the device table, for both precisions of the run."""
+    import ed
+    path = workloads("c1")["fcidump"]
+    norb, nelec, ms2, h, eri, core = ed.read_fcidump(path)
+    dets, H = ed.hamiltonian(h, eri, core, nelec // 2, nelec // 2)
+    idx = {int(d): i for i, d in enumerate(dets)}
+    g = Cdfci(path, num_of_coordinate=8, estimate=8, max_memory=0.005, threshold=0.0, precision=prec)
+    g.init()
+    g.step(120)
+    td, tx, _ = g.table()
+    x = np.zeros(len(dets))
+    for d, a in zip(td[:, 0], tx):
+        x[idx[int(d)]] = a
+    ref = float(x @ H @ x / (x @ x))
+    e = g.energy_fp64()
+    assert abs(e - ref) < 1e-12, (e, ref)
+    if prec == 0:
+        assert abs(e - g.report()["energy"]) < 1e-11  # z = Hx exactly at z_threshold = 0
 
 
 def test_driver_end_to_end(workloads, tmp_path):
