@@ -169,11 +169,12 @@ int mcdfci_checkpoint_load(mcdfci_ctx* ctx, const char* path);
 
 /* Measurement (no counterpart in the reference): the device time of n_ops
  * random-slot operations on this context's own table -- mode 0 a new-key
- * insert (key load, CAS, two 64-bit atomic adds), 1 an update (key load,
- * two atomic adds), 2 a read of key, x and z, 3 an update by one FP64
- * atomic add (key load, one 64-bit floating add: the cost of a z kept as an
- * order-dependent double) -- the access ceiling of the column scatter and
- * pass 2 on the same footprint.  Overwrites the table:
+ * insert (key load, CAS, one 64-bit atomic add), 1 an update (key load, one
+ * int64 atomic add: the scatter's exact z update for a contribution below
+ * 2^-11), 2 a read of key, x and z, 3 an update by one FP64 atomic add (the
+ * cost of a z kept as an order-dependent double), 4 an update by two int64
+ * atomic adds (a large contribution) -- the access ceiling of the column
+ * scatter and pass 2 on the same footprint.  Overwrites the table:
  * mcdfci_init must follow before mcdfci_step. */
 int mcdfci_access_ceiling(mcdfci_ctx* ctx, int mode, int64_t n_ops, double* seconds);
 
@@ -245,9 +246,12 @@ int mcdfci_energy_gradient(mcdfci_ctx* ctx, double* grad_h, double* grad_eri, do
  * H built from the fp64 integrals whatever the context's precision (the
  * energy a mixed-precision run would report with exact H elements).  The
  * reference reports xz / xx only (src/wavefunction.h:195-198); this is the
- * measure of the mixed form's energy error.  Sums are exact fixed point, so
- * the value does not depend on thread order. */
-int mcdfci_energy_fp64(mcdfci_ctx* ctx, double* energy);
+ * measure of the mixed form's energy error.  ham NULL takes the context's
+ * integrals; otherwise ham's h, (pq|rs) and core (same norb) are used for
+ * this call, which evaluates the same x under other integrals (e.g. finite
+ * differences of the energy at a fixed wavefunction).  Sums are exact fixed
+ * point, so the value does not depend on thread order. */
+int mcdfci_energy_fp64(mcdfci_ctx* ctx, const mcdfci_ham* ham, double* energy);
 
 #ifdef __cplusplus
 }

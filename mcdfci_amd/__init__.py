@@ -106,7 +106,7 @@ SIGNATURES = {
     "mcdfci_table_digest": (C.c_int, [C.c_void_p, _u64p]),
     "mcdfci_candidates": (C.c_int, [C.c_void_p, _u64p, _f64p, _f64p, C.c_int64, C.POINTER(C.c_int64)]),
     "mcdfci_energy_gradient": (C.c_int, [C.c_void_p, _f64p, _f64p, C.POINTER(C.c_double)]),
-    "mcdfci_energy_fp64": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "mcdfci_energy_fp64": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
 }
 
 
@@ -237,8 +237,8 @@ class Cdfci:
 
     def access_ceiling(self, mode, n_ops=1 << 25):
         """Device seconds for n_ops random-slot operations on this context's table
-        (mode 0 insert, 1 update, 2 read, 3 update by one FP64 atomic).  Overwrites the
-        table: call init() after."""
+        (mode 0 insert, 1 update, 2 read, 3 update by one FP64 atomic, 4 update by two
+        int64 atomics).  Overwrites the table: call init() after."""
         sec = C.c_double()
         self._check(self.lib.mcdfci_access_ceiling(self.h, int(mode), int(n_ops), C.byref(sec)))
         return sec.value
@@ -313,9 +313,18 @@ class Cdfci:
         self._check(self.lib.mcdfci_energy_gradient(self.h, gh, ge, C.byref(e)))
         return e.value, gh.reshape(n, n), ge.reshape(n, n, n, n)
 
-    def energy_fp64(self):
+    def energy_fp64(self, integrals=None):
         """FP64 Rayleigh quotient x.Hx / x.x of the current wavefunction (fp64
-        integrals whatever the precision of the run)."""
+        integrals whatever the precision of the run).  integrals = (h, eri, core)
+        evaluates the same x under those integrals instead of the context's."""
         e = C.c_double()
-        self._check(self.lib.mcdfci_energy_fp64(self.h, C.byref(e)))
+        if integrals is None:
+            self._check(self.lib.mcdfci_energy_fp64(self.h, None, C.byref(e)))
+        else:
+            h1, eri, core = integrals
+            h1 = np.ascontiguousarray(h1, dtype=np.float64)
+            eri = np.ascontiguousarray(eri, dtype=np.float64)
+            hs = Ham(self.norb, self.nelec, self.ms2, float(core), h1.ctypes.data_as(C.POINTER(C.c_double)),
+                     eri.ctypes.data_as(C.POINTER(C.c_double)), None)
+            self._check(self.lib.mcdfci_energy_fp64(self.h, C.byref(hs), C.byref(e)))
         return e.value

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -108,6 +109,8 @@ struct mcdfci_ctx {
   int* pc1 = nullptr;
   long long* ctr = nullptr;  // striped iteration counters
   uint32_t* eslot = nullptr;  // per column element: the slot where the scatter's probe ended
+  uint32_t* znorm = nullptr;  // slots whose z word B pass 2 found near its range (kZNormCap)
+  int znorm_bits = 61;        // fold threshold (MCDFCI_ZNORM_BITS lowers it to exercise the fold)
   size_t iter_bytes = 0;
   // launch choices (result-neutral; set by mcdfci_autotune)
   int scatter_ept = 4;
@@ -223,53 +226,72 @@ int check_det(mcdfci_ctx* c, const D2& det, const char* what) {
   return MCDFCI_OK;
 }
 
-int build_tables(mcdfci_ctx* c) {
-  const SpatialHam& S = c->ham;
+// The fp64 tables derived from a spin-restricted Hamiltonian: single-
+// excitation sums S_same/S_opp [p][r][k], the single-excitation filter and
+// the per-spin-orbital diagonal terms.
+struct Fp64Tables {
+  std::vector<double> ss, so, d1, d2;
+  std::vector<uint8_t> sok;
+};
+
+Fp64Tables make_tables(const SpatialHam& S, double threshold) {
   const int M = S.norb, ns = 2 * M;
-  const size_t M2 = (size_t)M * M, M3 = M2 * M, M4 = M2 * M2;
+  const size_t M2 = (size_t)M * M, M3 = M2 * M;
   auto E = [&](int p, int q, int r, int s) { return S.eri[(((size_t)p * M + q) * M + r) * M + s]; };
-  std::vector<double> ss(M3), so(M3), d1(ns), d2((size_t)ns * ns, 0.0);
-  std::vector<uint8_t> sok(M2);
+  Fp64Tables T;
+  T.ss.assign(M3, 0.0);
+  T.so.assign(M3, 0.0);
+  T.d1.assign(ns, 0.0);
+  T.d2.assign((size_t)ns * ns, 0.0);
+  T.sok.assign(M2, 0);
   for (int p = 0; p < M; ++p)
     for (int r = 0; r < M; ++r) {
       double mx = std::fabs(S.h[p * M + r]);
       for (int k = 0; k < M; ++k) {
         double a = E(p, r, k, k);
         double b = E(p, k, k, r);
-        ss[((size_t)p * M + r) * M + k] = a - b;
-        so[((size_t)p * M + r) * M + k] = a;
+        T.ss[((size_t)p * M + r) * M + k] = a - b;
+        T.so[((size_t)p * M + r) * M + k] = a;
         mx = std::max(mx, std::fabs(a - b));
         mx = std::max(mx, std::fabs(a));
       }
-      sok[p * M + r] = mx > c->opts.threshold ? 1 : 0;
+      T.sok[p * M + r] = mx > threshold ? 1 : 0;
     }
-  for (int i = 0; i < ns; ++i) d1[i] = S.h[(i >> 1) * M + (i >> 1)];
+  for (int i = 0; i < ns; ++i) T.d1[i] = S.h[(i >> 1) * M + (i >> 1)];
   for (int i = 0; i < ns; ++i)
     for (int j = i + 1; j < ns; ++j) {
       int p = i >> 1, q = j >> 1;
       double v = E(p, p, q, q);
       if ((i & 1) == (j & 1)) v = v - E(p, q, q, p);
-      d2[(size_t)i * ns + j] = v;
+      T.d2[(size_t)i * ns + j] = v;
     }
+  return T;
+}
+
+int build_tables(mcdfci_ctx* c) {
+  const SpatialHam& S = c->ham;
+  const int M = S.norb, ns = 2 * M;
+  const size_t M2 = (size_t)M * M, M3 = M2 * M, M4 = M2 * M2;
+  const Fp64Tables T = make_tables(S, c->opts.threshold);
   std::vector<float> eri32(M4), h132(M2), ss32(M3), so32(M3);
   for (size_t i = 0; i < M4; ++i) eri32[i] = (float)S.eri[i];
   for (size_t i = 0; i < M2; ++i) h132[i] = (float)S.h[i];
   for (size_t i = 0; i < M3; ++i) {
-    ss32[i] = (float)ss[i];
-    so32[i] = (float)so[i];
+    ss32[i] = (float)T.ss[i];
+    so32[i] = (float)T.so[i];
   }
   int rc;
   if ((rc = upload(c, &c->d_eri, S.eri))) return rc;
   if ((rc = upload(c, &c->d_eri32, eri32))) return rc;
   if ((rc = upload(c, &c->d_h1, S.h))) return rc;
   if ((rc = upload(c, &c->d_h132, h132))) return rc;
-  if ((rc = upload(c, &c->d_ss, ss))) return rc;
-  if ((rc = upload(c, &c->d_so, so))) return rc;
+  if ((rc = upload(c, &c->d_ss, T.ss))) return rc;
+  if ((rc = upload(c, &c->d_so, T.so))) return rc;
   if ((rc = upload(c, &c->d_ss32, ss32))) return rc;
   if ((rc = upload(c, &c->d_so32, so32))) return rc;
-  if ((rc = upload(c, &c->d_sok, sok))) return rc;
-  if ((rc = upload(c, &c->d_d1, d1))) return rc;
-  if ((rc = upload(c, &c->d_d2, d2))) return rc;
+  if ((rc = upload(c, &c->d_sok, T.sok))) return rc;
+  if ((rc = upload(c, &c->d_d1, T.d1))) return rc;
+  if ((rc = upload(c, &c->d_d2, T.d2))) return rc;
   DevHam& H = c->dh;
   H.M = M;
   H.ns = ns;
@@ -310,7 +332,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   for (void* p : {(void*)c->pdet, (void*)c->pslot, (void*)c->px, (void*)c->pz, (void*)c->dx, (void*)c->Hs,
                   (void*)c->vis, (void*)c->stage, (void*)c->stage_cnt, (void*)c->part,
                   (void*)c->col_xz, (void*)c->cand, (void*)c->cand_cnt, (void*)c->pk0, (void*)c->pk1,
-                  (void*)c->pc0, (void*)c->pc1, (void*)c->ctr, (void*)c->eslot})
+                  (void*)c->pc0, (void*)c->pc1, (void*)c->ctr, (void*)c->eslot, (void*)c->znorm})
     dfree(c, p);
   const int k = c->k, M = c->M;
   const DevHam& H = c->dh;
@@ -339,6 +361,7 @@ int alloc_iteration(mcdfci_ctx* c) {
   if ((rc = dalloc(c, &c->pc1, k))) return rc;
   if ((rc = dalloc(c, &c->ctr, (size_t)kCtrStripes * kCtrStride))) return rc;
   if ((rc = dalloc(c, &c->eslot, (size_t)k * c->vis_words * 64))) return rc;
+  if ((rc = dalloc(c, &c->znorm, (size_t)kZNormCap))) return rc;
   HIPCHECK(c, hipMemsetAsync(c->cand_cnt, 0, sizeof(int) * k, c->stream));
   HIPCHECK(c, hipMemsetAsync(c->ctr, 0, sizeof(long long) * kCtrStripes * kCtrStride, c->stream));
   return MCDFCI_OK;
@@ -380,10 +403,10 @@ void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
                  k, c->scatter_ept, c->ctr, c->eslot);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[4], s);
   launch_pass2(s, c->nw, c->dh, c->st, c->pdet, c->dx, c->Hs, c->k, c->tab, c->nslots - 1, c->vis, c->vis_words,
-               c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part, c->eslot);
+               c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part, c->eslot, c->znorm);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[5], s);
   launch_merge(s, c->nw, c->st, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
-               c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration, c->ctr);
+               c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration, c->ctr, c->nslots, c->znorm);
 }
 
 // One iteration of src/solver.h:701-759.  With timing enabled every kernel
@@ -486,6 +509,11 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   }
   mcdfci_ctx* c = new mcdfci_ctx();
   c->opts = *opts;
+  if (const char* zb = std::getenv("MCDFCI_ZNORM_BITS")) {
+    // test hook: fold z word B into A from 2^bits on (53..61) instead of 2^61
+    const int v = std::atoi(zb);
+    if (v >= 53 && v <= 61) c->znorm_bits = v;
+  }
   c->k = opts->num_of_coordinate;
   c->M = opts->estimate;
   c->device = opts->device;
@@ -632,6 +660,7 @@ int mcdfci_init(mcdfci_ctx* c, const uint64_t* ref_det) {
   DevState s{};
   s.z_thr = c->opts.z_threshold;
   s.max_size = c->max_size;
+  s.znorm_bits = c->znorm_bits;
   HIPCHECK(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   launch_init_ref(c->stream, c->nw, c->st, det, c->tab, c->nslots - 1, c->pdet, c->pslot, c->px, c->pz, c->dx);
   launch_submatrix(c->stream, c->nw, c->dh, c->st, c->pdet, 1, c->Hs, c->k);
@@ -669,7 +698,7 @@ int mcdfci_step(mcdfci_ctx* c, int64_t n) {
 }
 
 int mcdfci_access_ceiling(mcdfci_ctx* c, int mode, int64_t n_ops, double* seconds) {
-  if (!c || !seconds || mode < 0 || mode > 3 || n_ops < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
+  if (!c || !seconds || mode < 0 || mode > 4 || n_ops < 1) return fail(c, MCDFCI_E_INPUT, "invalid argument");
   HIPCHECK(c, hipSetDevice(c->device));
   unsigned long long* sink = nullptr;
   int rc = dalloc(c, &sink, 1);
@@ -700,7 +729,7 @@ namespace {
 // lists, then the table's entries in chunks of at most kCkChunk (per chunk:
 // the count, two words per determinant, x, and the (zA, zB) pairs).
 constexpr char kCkMagic[8] = {'M', 'C', 'D', 'F', 'C', 'I', 'C', 'K'};
-constexpr int kCkVersion = 2;
+constexpr int kCkVersion = 3;
 constexpr long long kCkChunk = 1LL << 22;
 
 struct CkHeader {
@@ -1253,7 +1282,7 @@ int mcdfci_table_dump(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
     size_t j = order[i];
     if (dets) store_det(dets + i * c->nio, hd[j], c->nio);
     if (x) x[i] = hx[j];
-    if (z) z[i] = fix_to_d(fix_join(hz[2 * j], hz[2 * j + 1]));
+    if (z) z[i] = fix_to_d(tz_join(hz[2 * j], hz[2 * j + 1]));
   }
   return MCDFCI_OK;
 }
@@ -1299,7 +1328,7 @@ int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
       if (m < cap) {
         if (dets) store_det(dets + m * c->nio, D2{e.det, e.det1}, c->nio);
         if (x) x[m] = xz[0];
-        if (z) z[m] = fix_to_d(fix_join(za, zb));
+        if (z) z[m] = fix_to_d(tz_join(za, zb));
       }
       ++m;
     }
@@ -1311,9 +1340,11 @@ int mcdfci_candidates(mcdfci_ctx* c, uint64_t* dets, double* x, double* z, int64
 
 // FP64 Rayleigh quotient x.Hx / x.x of the table's x (fp64 integrals
 // whatever the context's precision).
-int mcdfci_energy_fp64(mcdfci_ctx* c, double* energy) {
+int mcdfci_energy_fp64(mcdfci_ctx* c, const mcdfci_ham* ham, double* energy) {
   if (!c || !energy) return fail(c, MCDFCI_E_INPUT, "null argument");
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  if (ham && (ham->norb != c->ham.norb || !ham->h || !ham->eri))
+    return fail(c, MCDFCI_E_INPUT, "the integrals must have the context's NORB");
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   DevState s;
@@ -1348,6 +1379,41 @@ int mcdfci_energy_fp64(mcdfci_ctx* c, double* energy) {
     release();
     return rc;
   }
+  // Integrals: the context's, or the caller's (fp64 tables built here for
+  // this call only).
+  DevHam H = c->dh;
+  std::vector<void*> tmp;
+  auto release_tmp = [&]() {
+    for (void* p : tmp) dfree(c, p);
+    tmp.clear();
+  };
+  if (ham) {
+    SpatialHam S;
+    S.norb = c->ham.norb;
+    S.core = ham->core;
+    const size_t n2 = (size_t)S.norb * S.norb;
+    S.h.assign(ham->h, ham->h + n2);
+    S.eri.assign(ham->eri, ham->eri + n2 * n2);
+    const Fp64Tables T = make_tables(S, c->opts.threshold);
+    double *de = nullptr, *dh = nullptr, *dss = nullptr, *dso = nullptr, *dd1 = nullptr, *dd2 = nullptr;
+    uint8_t* dsok = nullptr;
+    if ((rc = upload(c, &de, S.eri)) || (rc = upload(c, &dh, S.h)) || (rc = upload(c, &dss, T.ss)) ||
+        (rc = upload(c, &dso, T.so)) || (rc = upload(c, &dsok, T.sok)) || (rc = upload(c, &dd1, T.d1)) ||
+        (rc = upload(c, &dd2, T.d2))) {
+      for (void* p : {(void*)de, (void*)dh, (void*)dss, (void*)dso, (void*)dsok, (void*)dd1, (void*)dd2}) dfree(c, p);
+      release();
+      return rc;
+    }
+    tmp = {de, dh, dss, dso, dsok, dd1, dd2};
+    H.core = S.core;
+    H.eri = de;
+    H.h1 = dh;
+    H.s_same = dss;
+    H.s_opp = dso;
+    H.single_ok = dsok;
+    H.d1 = dd1;
+    H.d2 = dd2;
+  }
   HIPCHECK(c, hipMemsetAsync(acc, 0, 32, c->stream));
   int64_t q[4];
   launch_sum_sq(c->stream, ox, (long long)m, 1.0, acc);
@@ -1357,17 +1423,18 @@ int mcdfci_energy_fp64(mcdfci_ctx* c, double* energy) {
   const double xx = fix_to_d(fix_join(q[0], q[1]));
   if (!(xx > 0.0)) {
     release();
+    release_tmp();
     return fail(c, MCDFCI_E_NUMERIC, "the wavefunction is zero");
   }
   const double scale = 1.0 / std::sqrt(xx);
   HIPCHECK(c, hipMemsetAsync(acc, 0, 32, c->stream));
   launch_sum_sq(c->stream, ox, (long long)m, scale, acc);
-  DevHam H = c->dh;
   launch_rayleigh(c->stream, c->nw, H, c->tab, c->nslots - 1, od, ox, (long long)m, scale, acc + 2);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(q, acc, 32, hipMemcpyDeviceToHost));
   release();
+  release_tmp();
   *energy = dd_div(fix_to_dd(fix_join(q[2], q[3])), fix_to_dd(fix_join(q[0], q[1]))).hi;
   return MCDFCI_OK;
 }

@@ -6,8 +6,8 @@
 //                4 slots per 128 B line), key 0 = empty, bit 63 marks a slot
 //                inserted during the current iteration (cleared in pass 2).
 //                W = 2 (up to 63): {k0, k1, x, zA, zB} (40 B), claim, mark
-//                and empty conventions of det.h.  z = (zA * 2^32 + zB) * 2^-64
-//                exactly.
+//                and empty conventions of det.h.  z = (zA * 2^53 + zB) * 2^-64
+//                exactly (num.h: tz_join).
 //   integrals  : dense spatial (pq|rs) [M^4] and h [M^2] in fp64 and fp32,
 //                single-excitation sums S_same/S_opp [M^3], diagonal terms
 //                per spin-orbital; all read through L2/MALL.
@@ -103,7 +103,12 @@ struct DevState {
   long long sum_iterations, sum_inv_iter, curr_inv_iter;
   int lz_prev;     // Lanczos vectors used by the previous subspace solve
   long long eig_unconverged;  // subspace solves accepted after the last restart without passing the residual test
+  unsigned int znorm_cnt;     // slots listed by pass 2 for folding B into A (k_znorm)
+  int znorm_full;             // the list overflowed: k_znorm sweeps the whole table
+  int znorm_bits;             // fold when |B| > 2^znorm_bits (61; lower only to test the fold)
 };
+
+constexpr unsigned int kZNormCap = 1 << 16;  // slots the fold list holds per iteration
 
 enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9 };
 

@@ -19,11 +19,12 @@
 // sums over the rows) keeps them orthogonal, so a convergence test made
 // late costs steps but not accuracy.  The LDS holds lz_capacity(n) vectors
 // (12288 doubles shared with A when A is resident); a solve reaching that
-// many stops and restarts from its Ritz vector.  At scheduled steps (from one less than the previous
-// solve's count, then every second step) thread 0 finds the lowest
-// eigenvalue of the Lanczos matrix T_m by Laguerre's iteration on its
-// leading minors (warm-started from the previous check) and its eigenvector
-// by two inverse-iteration solves; the solve stops when
+// many stops and restarts from its Ritz vector.  At scheduled steps (from
+// one less than the previous solve's count, then every second step) wave 0
+// finds the lowest eigenvalue of the Lanczos matrix T_m by multisection on
+// Sturm counts (two counts per lane, 128 points per round, warm-started from
+// the previous check) and thread 0 its eigenvector by two inverse-iteration
+// solves; the solve stops when
 // beta_j |y_m| <= 1e-15 |T_m|_inf |y|, at breakdown or at m = n.  The Ritz
 // vector is normalised in double-double; the solve is accepted when
 // |A v - rq v| <= 8e-15 |T_m|_inf (rq the Rayleigh quotient) and restarted
@@ -358,93 +359,139 @@ __device__ __forceinline__ void tri_solve_d(int n, const double* rdg, const doub
   }
 }
 
-// Leading-minor recurrence of T - lam*I, p_i = (d_i - lam) p_{i-1} -
-// e_{i-1}^2 p_{i-2}, with first and second lam-derivatives, rescaled by
-// exact powers of two whenever p_i leaves [2^-300, 2^300] (no division on
-// the chain).  False as soon as some p_i is not positive; otherwise
-// G = f'/f and Hh = (f'/f)^2 - f''/f for f = det(T - lam*I).
-__device__ __forceinline__ bool minor_pass(int n, const double* d, const double* e2, double lam, double& G,
-                                           double& Hh) {
-  const double big = 0x1p300, small = 0x1p-300;
-  double pm = 1.0, dpm = 0.0, ddpm = 0.0;
-  double p = d[0] - lam, dp = -1.0, ddp = 0.0;
-  if (!(p > 0.0)) return false;
-  bool ok = true;
-  double d_n = n > 1 ? d[1] : 0.0, e_n = n > 1 ? e2[0] : 0.0;
-  for (int i = 1; i < n; ++i) {
-    const double a = d_n - lam, b = e_n;
-    if (i + 1 < n) {
-      d_n = d[i + 1];
-      e_n = e2[i];
-    }
-    double pn = a * p - b * pm;
-    double dpn = (a * dp - b * dpm) - p;
-    double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
-    if (uni(!(pn > 0.0))) {
-      ok = false;
-      break;
-    }
-    // Exact rescaling by 2^-300, 2^300 or 1.
-    const double sc = pn > big ? small : (pn < small ? big : 1.0);
-    pn *= sc, dpn *= sc, ddpn *= sc, p *= sc, dp *= sc, ddp *= sc;
-    pm = p, dpm = dp, ddpm = ddp;
-    p = pn, dp = dpn, ddp = ddpn;
-  }
-  if (!ok) return false;
-  const double g = dp / p;
-  G = g;
-  Hh = g * g - ddp / p;
-  return true;
+// Broadcast of lane `l` (wave-uniform) of a double held across the wave.
+__device__ __forceinline__ double rl_double(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
-// Lowest eigenvalue of tridiag(e, d, e): Laguerre from the Gershgorin lower
-// bound, or from `hint` when it lies above that bound and passes
-// minor_pass; lam always passes minor_pass.  A failed candidate is within
-// rounding of the root (the exact iteration never passes it): the point
-// 1e-13 * (|cand| + 1) below it is tried once and, if it passes, is the
-// answer; otherwise the failed points bound a bisection.
-__device__ __forceinline__ double tri_lowest(int n, const double* d, const double* e, const double* e2,
-                                             double hint) {
-  if (n == 1) return d[0];
-  double lo = d[0], hi = d[0];
-  for (int i = 0; i < n; ++i) {
-    double r = 0.0;
-    if (i > 0) r = r + __builtin_fabs(e[i - 1]);
-    if (i < n - 1) r = r + __builtin_fabs(e[i]);
-    lo = fmin(lo, d[i] - r);
-    hi = fmax(hi, d[i] + r);
-  }
-  const double pad = (hi - lo) * 1e-6 + 1e-200;
-  double lam = lo - pad;
-  hi = hi + pad;
-  const double nn = (double)n;
-  double G, Hh;
-  if (uni(hint > lam && hint < hi) && minor_pass(n, d, e2, hint, G, Hh)) {
-    lam = hint;
-  } else if (!minor_pass(n, d, e2, lam, G, Hh)) {
-    return lam;
-  }
-  for (int it = 0; it < 100; ++it) {
-    double disc = (nn - 1.0) * (nn * Hh - G * G);
-    if (!(disc > 0.0)) disc = 0.0;
-    double cand = lam - nn / (G - __builtin_sqrt(disc));
-    if (!(cand < hi)) cand = lam + 0.5 * (hi - lam);
-    if (uni(!(cand > lam) || !(cand < hi))) break;
-    double G2, H2;
-    if (minor_pass(n, d, e2, cand, G2, H2)) {
-      lam = cand, G = G2, Hh = H2;
-      continue;
+// Eigenvalues of tridiag(e, d, e) below s0 and below s1: the sign changes
+// of the leading minors p_0 = 1, p_i = (d_i - s) p_{i-1} - e_{i-1}^2 p_{i-2}
+// (a zero minor counts as positive), the two recurrences interleaved.  After
+// every eighth step (i = 8, 16, ...) the pair (p_i, p_{i-1}) is rescaled by
+// 2^-300 when max(|p_i|, |p_{i-1}|) > 2^300, by 2^300 when it is nonzero
+// and below 2^-300 (exact powers of two: signs are unchanged).  The lane's
+// registers dv[b], ev[b] hold d and e^2 of index 64 b + lane.
+template <int NB>
+__device__ __forceinline__ void sturm_count2(int m, const double (&dv)[NB], const double (&ev)[NB], double s0,
+                                             double s1, int& c0, int& c1) {
+  const double big = 0x1p300, small = 0x1p-300;
+  const double d0 = rl_double(dv[0], 0);
+  double pmA = 1.0, pA = d0 - s0, pmB = 1.0, pB = d0 - s1;
+  int cA = pA < 0.0 ? 1 : 0, cB = pB < 0.0 ? 1 : 0;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (64 * b >= m) break;
+    const int lim = m - 64 * b < 64 ? m - 64 * b : 64;
+    for (int t = b == 0 ? 1 : 0; t < lim; ++t) {
+      // index i = 64 b + t, 1 <= i < m; e2[i - 1] is lane i - 1 of block (i - 1) / 64
+      const double di = rl_double(dv[b], t);
+      const double e2 = t > 0 ? rl_double(ev[b], t - 1) : rl_double(ev[b - 1], 63);
+      const double tA = (di - s0) * pA, uA = e2 * pmA;
+      const double tB = (di - s1) * pB, uB = e2 * pmB;
+      const double nA = tA - uA, nB = tB - uB;
+      cA += ((nA < 0.0) != (pA < 0.0)) ? 1 : 0;
+      cB += ((nB < 0.0) != (pB < 0.0)) ? 1 : 0;
+      pmA = pA;
+      pA = nA;
+      pmB = pB;
+      pB = nB;
+      if (((64 * b + t) & 7) == 0) {
+        const double aA = fmax(__builtin_fabs(pA), __builtin_fabs(pmA));
+        const double aB = fmax(__builtin_fabs(pB), __builtin_fabs(pmB));
+        const double sA = aA > big ? small : ((aA < small && aA != 0.0) ? big : 1.0);
+        const double sB = aB > big ? small : ((aB < small && aB != 0.0) ? big : 1.0);
+        pA = pA * sA;
+        pmA = pmA * sA;
+        pB = pB * sB;
+        pmB = pmB * sB;
+      }
     }
-    hi = cand;
-    const double t = cand - 1e-13 * (__builtin_fabs(cand) + 1.0);
-    if (uni(!(t > lam))) break;
-    if (minor_pass(n, d, e2, t, G2, H2)) {
-      lam = t;
-      break;
-    }
-    hi = t;
   }
-  return lam;
+  c0 = cA;
+  c1 = cB;
+}
+
+// Lowest eigenvalue of the Lanczos matrix tridiag(e, d, e) of order m by
+// multisection, one wave (every lane active): the bracket [lo, hi) keeps
+// count(lo) = 0 and count(hi) >= 1; each round counts at the 128 points
+// lo + (hi - lo) * ((j + 1) / 129), j = 0..127 (lane l takes j = l and
+// j = l + 64), and the bracket shrinks to the first cell whose right end
+// counts an eigenvalue (the last cell when none does).  The start is the
+// Gershgorin interval padded by 1e-6 of its width, narrowed when warm (a
+// previous check of this solve) by a first round that counts at `hint`
+// (becomes lo when it counts 0) and at prev + |prev| 2^-30 (becomes hi when
+// it counts an eigenvalue), each only when inside the bracket.  It stops
+// when hi - lo <= 2^-50 max(|lo|, |hi|) + 2^-1000, when a round no longer
+// shrinks the bracket, or after 40 rounds, and returns lo.  Every lane
+// returns the same value.
+__device__ double wave_lowest(int m, const double* d, const double* e, const double* e2g, bool warm, double hint,
+                              double prev) {
+  constexpr int NB = (kNMax + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  double dv[NB], ev[NB];
+  double glo = __builtin_huge_val(), ghi = -__builtin_huge_val();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int i = 64 * b + lane;
+    dv[b] = i < m ? d[i] : 0.0;
+    ev[b] = i + 1 < m ? e2g[i] : 0.0;
+    if (i < m) {
+      double r = 0.0;
+      if (i > 0) r = r + __builtin_fabs(e[i - 1]);
+      if (i < m - 1) r = r + __builtin_fabs(e[i]);
+      glo = fmin(glo, dv[b] - r);
+      ghi = fmax(ghi, dv[b] + r);
+    }
+  }
+  if (m == 1) return rl_double(dv[0], 0);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, off, 64));
+    ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
+  }
+  const double pad = (ghi - glo) * 1e-6 + 1e-200;
+  double lo = glo - pad, hi = ghi + pad;
+  if (warm) {
+    const double hu = prev + __builtin_fabs(prev) * 0x1p-30;
+    int c0, c1;
+    sturm_count2<NB>(m, dv, ev, hint, hu, c0, c1);
+    const double nlo = (hint > lo && hint < hi && c0 == 0) ? hint : lo;
+    const double nhi = (hu > lo && hu < hi && c1 >= 1) ? hu : hi;
+    if (nlo < nhi) {
+      lo = nlo;
+      hi = nhi;
+    }
+  }
+  const double fa = (double)(lane + 1) / 129.0, fb = (double)(lane + 65) / 129.0;
+  for (int round = 0; round < 40; ++round) {
+    const double w = hi - lo;
+    const double tol = fmax(__builtin_fabs(lo), __builtin_fabs(hi)) * 0x1p-50 + 0x1p-1000;
+    if (!(w > tol)) break;
+    const double sa = lo + w * fa, sb = lo + w * fb;
+    int ca, cb;
+    sturm_count2<NB>(m, dv, ev, sa, sb, ca, cb);
+    const unsigned long long ba = __ballot(ca >= 1), bb = __ballot(cb >= 1);
+    double nlo, nhi;
+    if (ba != 0) {
+      const int l = __ffsll((long long)ba) - 1;
+      nhi = rl_double(sa, l);
+      nlo = l > 0 ? rl_double(sa, l - 1) : lo;
+    } else if (bb != 0) {
+      const int l = __ffsll((long long)bb) - 1;
+      nhi = rl_double(sb, l);
+      nlo = l > 0 ? rl_double(sb, l - 1) : rl_double(sa, 63);
+    } else {
+      nlo = rl_double(sb, 63);
+      nhi = hi;
+    }
+    if (!(nhi - nlo < w)) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo;
 }
 
 // Eigenvector of the Lanczos matrix T_m for eigenvalue lam (thread 0; lam
@@ -508,7 +555,7 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     }
   }
   double aprev = 0.0, bprev = 0.0, bprev2 = 0.0, tn_done = 0.0, tn = 0.0;
-  double hint = -__builtin_huge_val();  // thread 0: warm start of the next check
+  double hint = -__builtin_huge_val(), prev = 0.0;  // warm start of the next check (every thread)
   int m = 0;
   for (int j = 0; j < n; ++j) {
     mat_partials(A, n, [&](int i) { return L.qw[w][i]; }, L.part[j & 1]);
@@ -602,8 +649,10 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     }
     const bool breakdown = !(beta > kLzTol * tn);
     if (m == n || breakdown || m == mcap || (m >= first_check && ((m - first_check) & 1) == 0)) {
+      __syncthreads();  // this step's alpha, beta visible to wave 0
+      double theta = 0.0;
+      if (w == 0) theta = wave_lowest(m, L.alpha, L.beta, L.e2, hint > -__builtin_huge_val(), hint, prev);
       if (tid == 0) {
-        const double theta = tri_lowest(m, L.alpha, L.beta, L.e2, hint);
         tri_vector(L, m, theta);
         double ss = 0.0;
         for (int i = 0; i < m; ++i) ss = ss + L.y[i] * L.y[i];
@@ -614,9 +663,12 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
         L.sc[0] = theta;
         // Some eigenvalue of A lies within est / |y| of theta, and the next
         // lowest Ritz value lies in [lambda_min(A), theta].
-        hint = theta - est / rn;
+        L.sc[2] = theta - est / rn;
+        L.sc[3] = theta;
       }
       __syncthreads();
+      hint = L.sc[2];
+      prev = L.sc[3];
       if (L.iv[0]) break;
     }
 #pragma unroll
@@ -678,6 +730,347 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
   if (L.iv[1])
     for (int i = tid; i < n; i += blockDim.x) L.q[i] = dd_neg(L.q[i]);
   __syncthreads();
+}
+
+// ---------------------------------------------------------------- one-wave form
+// For n <= kEigLdsN the same solve runs on one wave: lane l owns rows l and
+// l + 64 and keeps for each the four interleaved partial sums S_g (columns
+// c = g mod 4, ascending) that the four-wave form spreads over its waves,
+// so every row sum, dot product and Gram-Schmidt coefficient is formed in
+// the same order and the results are bit-identical.  With one wave the
+// barriers below only order the LDS accesses of its lanes.
+constexpr int kR1 = 2;  // rows per lane (n <= 128)
+
+// Row sums of A v, (S0 + S1) + (S2 + S3) with S_g the ascending sum over
+// c = g mod 4 of A(r, c) v_c; A(r, c) = a[c * lda + r]; v in LDS.
+__device__ __forceinline__ void matvec1(const double* a, int lda, int n, const double* v, double (&out)[kR1]) {
+  const int lane = threadIdx.x & 63;
+  double S[kR1][4];
+#pragma unroll
+  for (int t = 0; t < kR1; ++t)
+#pragma unroll
+    for (int g = 0; g < 4; ++g) S[t][g] = 0.0;
+  int rr[kR1];
+#pragma unroll
+  for (int t = 0; t < kR1; ++t) {
+    const int r = lane + 64 * t;
+    rr[t] = r < n ? r : n - 1;  // loads stay in A; rows beyond n are not used
+  }
+  for (int c0 = 0; c0 < n; c0 += 4) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int c = c0 + g;
+      if (c < n) {
+        const double vc = v[c];
+        const double* col = a + (size_t)c * lda;
+#pragma unroll
+        for (int t = 0; t < kR1; ++t) S[t][g] = S[t][g] + col[rr[t]] * vc;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kR1; ++t) out[t] = (S[t][0] + S[t][1]) + (S[t][2] + S[t][3]);
+}
+
+// lanczos_lowest for A held in LDS (column-major, n x n), one wave.
+__device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, double perturb, double* Q) {
+  const int lane = threadIdx.x & 63;
+  const int mcap = lz_capacity(n);
+  double QC[kR1], QP[kR1], W[kR1];
+  {
+    double p0 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) p0 = p0 + L.v0[r] * L.v0[r];
+    }
+    const double s0 = __builtin_sqrt(bfly_sum(p0));
+    double p1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      QP[t] = 0.0;
+      QC[t] = 0.0;
+      if (r < n) {
+        const double q = (s0 > 0.0 ? L.v0[r] / s0 : 0.0) + perturb * lz_hash_unit(r);
+        QC[t] = q;
+        p1 = p1 + q * q;
+      }
+    }
+    const double s1 = __builtin_sqrt(bfly_sum(p1));
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) {
+        QC[t] = QC[t] / s1;
+        Q[r] = QC[t];
+      }
+    }
+  }
+  __syncthreads();
+  double aprev = 0.0, bprev = 0.0, bprev2 = 0.0, tn_done = 0.0, tn = 0.0;
+  double hint = -__builtin_huge_val(), prev = 0.0;
+  int m = 0;
+  for (int j = 0; j < n; ++j) {
+    double Av[kR1];
+    matvec1(A, n, n, Q + (size_t)j * n, Av);
+    double pa = 0.0;
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      W[t] = 0.0;
+      if (r < n) {
+        double sv = Av[t];
+        if (j > 0) sv = sv - bprev * QP[t];
+        W[t] = sv;
+        pa = pa + QC[t] * sv;
+      }
+    }
+    const double alpha = bfly_sum(pa);
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      W[t] = W[t] - alpha * QC[t];
+      if (r < n) L.wv[r] = W[t];
+    }
+    __syncthreads();
+    // Gram-Schmidt coefficients h_i = q_i.w, lane 4i' + g forming S_g over
+    // the rows r = g mod 4 (16 coefficients per pass), as the four-wave form.
+    for (int i0 = 0; i0 <= j; i0 += 16) {
+      const int i = i0 + (lane >> 2), g = lane & 3;
+      double S = 0.0;
+      if (i <= j) {
+        const double* qi = Q + (size_t)i * n;
+        int r = g;
+        for (; r + 12 < n; r += 16) {
+          double qa[4], wa[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            qa[u] = qi[r + 4 * u];
+            wa[u] = L.wv[r + 4 * u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) S = S + qa[u] * wa[u];
+        }
+        for (; r < n; r += 4) S = S + qi[r] * L.wv[r];
+      }
+      const double s01 = S + lane_xor<1>(S);
+      const double h = s01 + lane_xor<2>(s01);
+      if (g == 0 && i <= j) L.hbuf[i] = h;
+    }
+    __syncthreads();
+    double pb = 0.0;
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n) {
+        double acc = W[t];
+        int i = 0;
+        for (; i + 7 <= j; i += 8) {
+          double qa[8], ha[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            qa[u] = Q[(size_t)(i + u) * n + r];
+            ha[u] = L.hbuf[i + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+        }
+        for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
+        W[t] = acc;
+        pb = pb + acc * acc;
+      }
+    }
+    const double beta = __builtin_sqrt(bfly_sum(pb));
+    if (j > 0) {
+      double rj = __builtin_fabs(aprev);
+      if (j > 1) rj = rj + __builtin_fabs(bprev2);
+      rj = rj + __builtin_fabs(bprev);
+      tn_done = fmax(tn_done, rj);
+    }
+    double last = __builtin_fabs(alpha);
+    if (j > 0) last = last + __builtin_fabs(bprev);
+    tn = fmax(tn_done, last);
+    m = j + 1;
+    if (lane == 0) {
+      L.alpha[j] = alpha;
+      L.beta[j] = beta;
+      L.e2[j] = beta * beta;
+    }
+    const bool breakdown = !(beta > kLzTol * tn);
+    if (m == n || breakdown || m == mcap || (m >= first_check && ((m - first_check) & 1) == 0)) {
+      __syncthreads();
+      const double theta = wave_lowest(m, L.alpha, L.beta, L.e2, hint > -__builtin_huge_val(), hint, prev);
+      if (lane == 0) {
+        tri_vector(L, m, theta);
+        double ss = 0.0;
+        for (int i = 0; i < m; ++i) ss = ss + L.y[i] * L.y[i];
+        const double rn = __builtin_sqrt(ss);
+        const double est = beta * __builtin_fabs(L.y[m - 1]);
+        const bool cv = m == n || breakdown || est <= (kLzTol * tn) * rn;
+        L.iv[0] = (cv || m == mcap) ? 1 : 0;
+        L.sc[0] = theta;
+        L.sc[2] = theta - est / rn;
+        L.sc[3] = theta;
+      }
+      __syncthreads();
+      hint = L.sc[2];
+      prev = L.sc[3];
+      if (L.iv[0]) break;
+    }
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) {
+      const int r = lane + 64 * t;
+      QP[t] = QC[t];
+      if (r < n) {
+        const double q = W[t] / beta;
+        QC[t] = q;
+        Q[(size_t)(j + 1) * n + r] = q;
+      }
+    }
+    __syncthreads();
+    aprev = alpha;
+    bprev2 = bprev;
+    bprev = beta;
+  }
+  // Ritz vector v = sum_i y_i q_i, normalised in double-double.
+  double V[kR1];
+  dd pn{0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < kR1; ++t) {
+    const int r = lane + 64 * t;
+    V[t] = 0.0;
+    if (r < n) {
+      double acc = 0.0;
+      int i = 0;
+      for (; i + 7 < m; i += 8) {
+        double qa[8], ya[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          qa[u] = Q[(size_t)(i + u) * n + r];
+          ya[u] = L.y[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+      }
+      for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
+      V[t] = acc;
+      pn = dd_add(pn, two_prod(acc, acc));
+    }
+  }
+  const dd nrm = dd_sqrt(bfly_sum_dd(pn));
+#pragma unroll
+  for (int t = 0; t < kR1; ++t) {
+    const int r = lane + 64 * t;
+    if (r < n) L.q[r] = dd_div(mkdd(V[t]), nrm);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int lead = 0;
+    while (lead < n - 1 && dd_zero_p(L.q[lead])) ++lead;
+    L.iv[1] = dd_neg_p(L.q[lead]) ? 1 : 0;
+    L.iv[2] = m;
+    L.sc[1] = tn;
+  }
+  __syncthreads();
+  if (L.iv[1])
+    for (int i = lane; i < n; i += 64) L.q[i] = dd_neg(L.q[i]);
+  __syncthreads();
+}
+
+// The update of src/solver.h:510-538 from the accepted Ritz vector (L.q,
+// L.ev) and its double-double Rayleigh quotient rq, then update_x: shared by
+// the four-wave and the one-wave solvers (block size bs, any multiple of 64
+// up to kWG).
+__device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n, bool normal, double y1, dd rq,
+                                           double res2, double* dx, double* tabx, int slot_stride,
+                                           const uint32_t* pslot, int mode) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const double lam = rq.hi;
+  if (normal) {
+    const dd lam_q = dd_neg(rq);
+    if (!(lam_q.hi > 0.0)) {
+      if (tid == 0) st->error = ERR_NUMERIC;
+      return;
+    }
+    const dd gam = dd_div_d(L.q[0], y1);
+    for (int i = tid; i < k; i += bs) dx[i] = dd_add_d(dd_div(L.q[i + 1], gam), -L.x[i]).hi;
+    if (tid == 0) {
+      const dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
+      double mev = 0.0;
+      for (int i = 1; i < n; ++i) mev = fmax(mev, __builtin_fabs(L.ev[i]));
+      st->max_residual = fmax(st->max_residual, __builtin_sqrt(res2));
+      st->curr_max_abs_evec = mev;
+      const double sc_d = L.ev[0] / y1 * __builtin_sqrt(-lam);
+      const dd diff = dd_add_d(dd_abs(sc_q), -__builtin_fabs(sc_d));
+      const double ad = __builtin_fabs(diff.hi);
+      st->sum_iterations += 1;
+      st->sum_inv_iter += 1;
+      st->curr_inv_iter = 1;
+      st->sum_diff_inv += ad;
+      st->max_diff_inv = fmax(st->max_diff_inv, ad);
+      st->curr_diff_inv = ad;
+      const dd xx = dd_div(dd_div(lam_q, sc_q), sc_q);
+      st->sc_hi = sc_q.hi;
+      st->sc_lo = sc_q.lo;
+      st->xx_hi = xx.hi;
+      st->xx_lo = xx.lo;
+    }
+  } else {
+    if (!(rq.hi < 0.0)) {
+      if (tid == 0) st->error = ERR_NUMERIC;
+      return;
+    }
+    const double r = __builtin_sqrt(-rq.hi);
+    for (int i = tid; i < k; i += bs) dx[i] = L.ev[i] * r - L.x[i];
+    if (tid == 0) {
+      st->xx_hi = -rq.hi;
+      st->xx_lo = -rq.lo;
+      st->sc_hi = 1.0;
+      st->sc_lo = 0.0;
+    }
+  }
+  if (mode != 0) return;
+  __syncthreads();
+  // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
+  // Picked determinants are distinct, so every thread owns its slots; the
+  // x.z increments are exact fixed-point integers (order-free sum).
+  __shared__ long long nx_w[kWG / 64];
+  __shared__ i128 xz_w[kWG / 64];
+  long long nx = 0;
+  i128 inc = 0;
+  for (int t = tid; t < k; t += bs) {
+    double* xs = tabx + (size_t)pslot[t] * slot_stride;
+    const double xo = *xs;
+    if (!(__builtin_fabs(xo) != 0.0)) ++nx;
+    *xs = xo + dx[t];
+    inc += to_fix(L.z[t] * dx[t]);
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    nx += __shfl_xor(nx, off, 64);
+    int64_t hi = (int64_t)(inc >> 64);
+    uint64_t lo = (uint64_t)inc;
+    int64_t ohi = __shfl_xor(hi, off, 64);
+    uint64_t olo = (uint64_t)__shfl_xor((long long)lo, off, 64);
+    inc += (((i128)ohi) << 64) | (i128)olo;
+  }
+  if ((tid & 63) == 0) {
+    nx_w[tid >> 6] = nx;
+    xz_w[tid >> 6] = inc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
+    long long n = 0;
+    for (int w = 0; w < (bs >> 6); ++w) {
+      xz += xz_w[w];
+      n += nx_w[w];
+    }
+    st->xz_hi = (int64_t)(xz >> 64);
+    st->xz_lo = (uint64_t)xz;
+    st->x_size += n;
+    st->k_cur = k;
+  }
 }
 
 // mode 0: solver iteration (reads picked x/z, writes dx, updates the state
@@ -802,91 +1195,122 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     perturb = 0.0;
   }
   if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
-  const double lam = rq.hi;
-  if (normal) {
-    const dd lam_q = dd_neg(rq);
-    if (!(lam_q.hi > 0.0)) {
-      if (tid == 0) st->error = ERR_NUMERIC;
-      return;
-    }
-    const dd gam = dd_div_d(L.q[0], y1);
-    for (int i = tid; i < k; i += bs) dx[i] = dd_add_d(dd_div(L.q[i + 1], gam), -L.x[i]).hi;
-    if (tid == 0) {
-      const dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
-      double mev = 0.0;
-      for (int i = 1; i < n; ++i) mev = fmax(mev, __builtin_fabs(L.ev[i]));
-      st->max_residual = fmax(st->max_residual, __builtin_sqrt(res2));
-      st->curr_max_abs_evec = mev;
-      const double sc_d = L.ev[0] / y1 * __builtin_sqrt(-lam);
-      const dd diff = dd_add_d(dd_abs(sc_q), -__builtin_fabs(sc_d));
-      const double ad = __builtin_fabs(diff.hi);
-      st->sum_iterations += 1;
-      st->sum_inv_iter += 1;
-      st->curr_inv_iter = 1;
-      st->sum_diff_inv += ad;
-      st->max_diff_inv = fmax(st->max_diff_inv, ad);
-      st->curr_diff_inv = ad;
-      const dd xx = dd_div(dd_div(lam_q, sc_q), sc_q);
-      st->sc_hi = sc_q.hi;
-      st->sc_lo = sc_q.lo;
-      st->xx_hi = xx.hi;
-      st->xx_lo = xx.lo;
-    }
-  } else {
-    if (!(rq.hi < 0.0)) {
-      if (tid == 0) st->error = ERR_NUMERIC;
-      return;
-    }
-    const double r = __builtin_sqrt(-rq.hi);
-    for (int i = tid; i < k; i += bs) dx[i] = L.ev[i] * r - L.x[i];
-    if (tid == 0) {
-      st->xx_hi = -rq.hi;
-      st->xx_lo = -rq.lo;
-      st->sc_hi = 1.0;
-      st->sc_lo = 0.0;
-    }
-  }
-  if (mode != 0) return;
-  __syncthreads();
-  // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
-  // Picked determinants are distinct, so every thread owns its slots; the
-  // x.z increments are exact fixed-point integers (order-free sum).
-  __shared__ long long nx_w[kWG / 64];
-  __shared__ i128 xz_w[kWG / 64];
-  long long nx = 0;
-  i128 inc = 0;
-  for (int t = tid; t < k; t += bs) {
-    double* xs = tabx + (size_t)pslot[t] * slot_stride;
-    const double xo = *xs;
-    if (!(__builtin_fabs(xo) != 0.0)) ++nx;
-    *xs = xo + dx[t];
-    inc += to_fix(L.z[t] * dx[t]);
-  }
-  for (int off = 32; off >= 1; off >>= 1) {
-    nx += __shfl_xor(nx, off, 64);
-    int64_t hi = (int64_t)(inc >> 64);
-    uint64_t lo = (uint64_t)inc;
-    int64_t ohi = __shfl_xor(hi, off, 64);
-    uint64_t olo = (uint64_t)__shfl_xor((long long)lo, off, 64);
-    inc += (((i128)ohi) << 64) | (i128)olo;
-  }
-  if ((tid & 63) == 0) {
-    nx_w[tid >> 6] = nx;
-    xz_w[tid >> 6] = inc;
+  eig_finish(st, L, k, n, normal, y1, rq, res2, dx, tabx, slot_stride, pslot, mode);
+}
+
+// k_eig on one wave for n <= kEigLdsN (k + 1 <= 90): the same setup,
+// solve, restart rule and update in the same arithmetic order.
+__global__ __launch_bounds__(64) void k_eig1(DevState* st, int k, int ldh, const double* Hs, const double* px,
+                                             const double* pz, double* dx, double* tabx, int slot_stride,
+                                             const uint32_t* pslot, int mode) {
+  __shared__ EigLds L;
+  __shared__ double Qlds[kQBudget];  // A, then the Lanczos basis
+  if (st->error) return;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int i = tid; i < k; i += bs) {
+    L.x[i] = px[i];
+    L.z[i] = pz[i];
   }
   __syncthreads();
   if (tid == 0) {
-    i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
-    long long n = 0;
-    for (int w = 0; w < (bs >> 6); ++w) {
-      xz += xz_w[w];
-      n += nx_w[w];
-    }
-    st->xz_hi = (int64_t)(xz >> 64);
-    st->xz_lo = (uint64_t)xz;
-    st->x_size += n;
-    st->k_cur = k;
+    double xTx = 0.0;
+    for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
+    dd xx{st->xx_hi, st->xx_lo};
+    const double yy = dd_add_d(xx, -xTx).hi;
+    L.sc[4] = yy;
+    L.sc[5] = __builtin_sqrt(yy);
   }
+  __syncthreads();
+  const double yy = L.sc[4], y1 = L.sc[5];
+  const bool normal = __builtin_fabs(yy) > 1e-14;
+  const int n = normal ? k + 1 : k;
+  const int o = normal ? 1 : 0;
+  {
+    // H x, H read from Hs in place (both triangles)
+    double hx[kR1];
+    matvec1(Hs, ldh, k, L.x, hx);
+#pragma unroll
+    for (int t = 0; t < kR1; ++t)
+      if (tid + 64 * t < k) L.Hx[tid + 64 * t] = hx[t];
+  }
+  __syncthreads();
+  if (normal) {
+    if (tid == 0) {
+      double xTz = 0.0, xTHx = 0.0;
+      for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
+      for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
+      dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
+      dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+      L.sc[6] = dd_div_d(a00, yy).hi;
+      L.v0[0] = y1;
+    }
+    for (int i = tid; i < k; i += bs) {
+      L.b0[i] = (L.z[i] - L.Hx[i]) / y1;
+      L.v0[i + 1] = L.x[i];
+    }
+  } else {
+    for (int i = tid; i < k; i += bs) L.v0[i] = 0.0;
+  }
+  __syncthreads();
+  const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
+  {
+    constexpr int U = 8;
+    for (int i0 = tid; i0 < n * n; i0 += U * bs) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = i0 + u * bs;
+        const int c = idx / n, r = idx - c * n;
+        v[u] = idx < n * n ? Ms(r, c) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * bs < n * n) Qlds[i0 + u * bs] = v[u];
+    }
+  }
+  __syncthreads();
+  const int first0 = st->lz_prev - 1 > 4 ? st->lz_prev - 1 : 4;
+  double* const Qb = Qlds + n * n;
+  int first = first0, attempts = 0;
+  double perturb = kLzPerturb;
+  dd rq{0.0, 0.0};
+  double res2 = 0.0;
+  for (;;) {
+    lanczos1(L, Qlds, n, first, perturb, Qb);
+    for (int i = tid; i < n; i += bs) {
+      L.ev[i] = L.q[i].hi;
+      L.dl[i] = L.q[i].lo;
+    }
+    __syncthreads();
+    {
+      double ae[kR1];
+      matvec1(Qlds, n, n, L.ev, ae);
+#pragma unroll
+      for (int t = 0; t < kR1; ++t)
+        if (tid + 64 * t < n) L.Ae[tid + 64 * t] = ae[t];
+    }
+    __syncthreads();
+    const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
+    const dd t2 = red64_dd(n, [&](int i) { return two_prod(L.dl[i], L.Ae[i]); });
+    rq = dd_add(t1, dd_mul_d(t2, 2.0));
+    const double lrq = rq.hi;
+    res2 = red64(n, [&](int i) {
+      double t = L.Ae[i] + (-lrq) * L.ev[i];
+      return t * t;
+    });
+    ++attempts;
+    if (__builtin_sqrt(res2) <= kLzAccept * L.sc[1]) break;
+    if (attempts == kLzAttempts) {
+      if (tid == 0) st->eig_unconverged += 1;
+      break;
+    }
+    for (int i = tid; i < n; i += bs) L.v0[i] = L.ev[i];
+    __syncthreads();
+    first = 4;
+    perturb = 0.0;
+  }
+  if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
+  eig_finish(st, L, k, n, normal, y1, rq, res2, dx, tabx, slot_stride, pslot, mode);
 }
 
 // Line-search update of the one picked coordinate (src/solver.h:254-314):
@@ -947,9 +1371,14 @@ void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, 
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
-  // The four-way partial sums assume exactly four waves.
-  hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride, pslot,
-                     mode);
+  // Up to n = k + 1 = kEigLdsN one wave; above, four (the four-way partial
+  // sums assume exactly four waves there).
+  if (k + 1 <= kEigLdsN)
+    hipLaunchKernelGGL(k_eig1, dim3(1), dim3(64), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride, pslot,
+                       mode);
+  else
+    hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride,
+                       pslot, mode);
 }
 
 }  // namespace mc

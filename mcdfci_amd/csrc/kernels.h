@@ -8,7 +8,7 @@
 
 namespace mc {
 
-void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st, const D2* dets, int k,
+void launch_submatrix(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* dets, int k,
                       double* Hs, int ldh);
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode);
@@ -23,12 +23,17 @@ void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, 
 void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                     const double* Hs, int ldh, void* tab, uint64_t mask, uint64_t* vis, int vis_words, int k,
                     int ept, long long* ctr, uint32_t* eslot);
+// znorm_list (kZNormCap slots): pass 2 lists the slots whose z word B has
+// grown past 2^znorm_bits; the merge folds them into word A, and the next
+// k_submatrix empties the list.
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
-                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot);
+                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot,
+                  uint32_t* znorm_list);
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
-                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr);
+                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr,
+                  uint64_t nslots, const uint32_t* znorm_list);
 // tree: per-column sorts and a merge tree even where one workgroup could
 // sort all k * M candidates (same result either way).
 void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,

@@ -37,10 +37,16 @@ __device__ __forceinline__ uint64_t rload(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// z += v (2^-64 units): one atomic on B for |v| < 2^53, else A and B.
 template <class S>
 __device__ __forceinline__ void zadd(S* s, i128 v) {
+  constexpr i128 kOne = ((i128)1) << kZShift;
+  if (v > -kOne && v < kOne) {
+    if (v) __hip_atomic_fetch_add(&s->zb, (int64_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
   int64_t a, b;
-  fix_split(v, a, b);
+  tz_split(v, a, b);
   if (a) __hip_atomic_fetch_add(&s->za, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (b) __hip_atomic_fetch_add(&s->zb, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -271,7 +277,7 @@ __device__ double off_diagonal_of(const DevHam& H, const D2& D1, const D2& D2b) 
 // gathered into LDS by all threads, then summed by one.  Two-word sectors
 // hold at most 94 electrons (63 of one spin, 31 of the other).
 template <int W>
-__global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st, const D2* dets, int k,
+__global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, DevState* st, const D2* dets, int k,
                                                    double* Hs, int ldh) {
   constexpr int kNe = W == 1 ? 2 * kMaxOrb : 94;
   __shared__ double terms[kNe + kNe * (kNe - 1) / 2];
@@ -279,6 +285,10 @@ __global__ __launch_bounds__(kWG) void k_submatrix(DevHam H, const DevState* st,
   __shared__ int nocc;
   if (st && st->error) return;
   const int i = blockIdx.x;
+  if (st && i == 0 && threadIdx.x == 0) {  // the previous merge has folded the z words pass 2 listed
+    st->znorm_cnt = 0;
+    st->znorm_full = 0;
+  }
   const D2 Di = dets[i];
   if (threadIdx.x == 0) {
     int ne = 0;
@@ -537,7 +547,7 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
                                                const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
                                                StageEnt* stage, int* stage_cnt, int64_t* part,
-                                               const uint32_t* eslot) {
+                                               const uint32_t* eslot, uint32_t* znorm_list) {
   __shared__ ColCtxW<W> C;
   // Selection scratch: the top-L tournament needs 512 entries, a full
   // sort the whole chunk (less LDS per workgroup, more workgroups per CU).
@@ -639,8 +649,15 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
         }
         x = cur.x;
         if ((vw[r] >> (e & 63)) & 1) {
-          double z = fix_to_d(fix_join(cur.za, cur.zb));
+          double z = fix_to_d(tz_join(cur.za, cur.zb));
           val = grad_key(s, x, z);
+        }
+        // B near its range: list the slot for k_znorm (duplicates are harmless)
+        const int64_t lim = (int64_t)1 << st->znorm_bits;
+        if (cur.zb > lim || cur.zb < -lim) {
+          const unsigned int pos = atomicAdd(&st->znorm_cnt, 1u);
+          if (pos < kZNormCap) znorm_list[pos] = (uint32_t)si;
+          else st->znorm_full = 1;
         }
         slot = (uint32_t)si;
       }
@@ -815,7 +832,8 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
                                                int nchunks, StageEnt* stage,
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
-                                               const double* dx, int count_iteration, long long* ctr) {
+                                               const double* dx, int count_iteration, long long* ctr,
+                                               uint64_t nslots, const uint32_t* znorm_list) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
   __shared__ uint64_t kc[W == 2 ? T * E : 1];
@@ -827,6 +845,40 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
   const int t = blockIdx.x;
   const int k = gridDim.x;
   const int mw = M < kChunk ? M : kChunk;
+  {
+    // Fold z word B into A (B -= c 2^53, A += c, c = floor(B / 2^53)) for
+    // the slots pass 2 listed with |B| > 2^znorm_bits, or, when the list
+    // overflowed, for every slot of the table.  Nothing else touches z here
+    // except this kernel's own reinsert of the picked coordinates, whose
+    // slots are skipped (the reinsert writes their z in folded form).  A slot
+    // listed twice is folded once: the CAS succeeds for one observed B, and
+    // the B re-read after it is small.  k_submatrix empties the list.
+    const unsigned int zcnt = st->znorm_cnt;
+    if (zcnt) {
+      const bool full = st->znorm_full != 0;
+      const uint64_t n = full ? nslots : (zcnt < kZNormCap ? zcnt : kZNormCap);
+      const int64_t lim = (int64_t)1 << st->znorm_bits;
+      const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+      for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t si = full ? i : (uint64_t)znorm_list[i];
+        SlotW<W>* sl = &tab[si];
+        int64_t b = __hip_atomic_load(&sl->zb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(b > lim || b < -lim)) continue;
+        bool picked = false;
+        for (int j = 0; j < k; ++j) picked = picked || pslot[j] == (uint32_t)si;
+        if (picked) continue;
+        while (b > lim || b < -lim) {
+          const int64_t c = b >> kZShift;  // floor
+          const int64_t nb = b - (c << kZShift);
+          if (__hip_atomic_compare_exchange_strong(&sl->zb, &b, nb, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_fetch_add(&sl->za, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+    }
+  }
   __shared__ int wtot[T / 64 + 1];
   {
     // nchunks <= 1024 <= 4 * T: up to four chunk counts per thread, in order.
@@ -1008,8 +1060,10 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
       xt += sums[w][1];
     }
     SlotW<W>* sp = &tab[pslot[t]];
-    sp->za = (int64_t)(nz >> 32);
-    sp->zb = (int64_t)(uint64_t)(nz & 0xffffffffULL);
+    int64_t za, zb;
+    tz_split(nz, za, zb);
+    sp->za = za;
+    sp->zb = zb;
     col_xz[2 * t] = (int64_t)(xt >> 64);
     col_xz[2 * t + 1] = (int64_t)(uint64_t)xt;
     __threadfence();
@@ -1112,7 +1166,7 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
       }
       const CandEnt ce = cand[(size_t)lo * M + (g - offs[lo])];
       const SlotW<W>& sl = tab[ce.slot];
-      double z = fix_to_d(fix_join(sl.za, sl.zb));
+      double z = fix_to_d(tz_join(sl.za, sl.zb));
       A[r] = bitsd(grad_key(s, sl.x, z));
       B[r] = ce.det;
       B1[r] = ce.det1;
@@ -1148,7 +1202,7 @@ __global__ __launch_bounds__(1024) void k_pick(DevState* st, int k, int M, const
     pslot[pos] = P[r];
     const SlotW<W>& sl = tab[P[r]];
     px[pos] = sl.x;
-    pz[pos] = fix_to_d(fix_join(sl.za, sl.zb));
+    pz[pos] = fix_to_d(tz_join(sl.za, sl.zb));
     ++pos;
   }
 }
@@ -1185,7 +1239,7 @@ __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, int
     if (g < n) {
       const CandEnt ce = cand[(size_t)t * M + g];
       const SlotW<W>& sl = tab[ce.slot];
-      const double z = fix_to_d(fix_join(sl.za, sl.zb));
+      const double z = fix_to_d(tz_join(sl.za, sl.zb));
       A[r] = bitsd(grad_key(s, sl.x, z));
       B[r] = ce.det;
       B1[r] = ce.det1;
@@ -1281,7 +1335,7 @@ __global__ void k_pick_emit(DevState* st, int k, const PickEnt* in, const int* i
     pslot[i] = e.slot;
     const SlotW<W>& sl = tab[e.slot];
     px[i] = sl.x;
-    pz[i] = fix_to_d(fix_join(sl.za, sl.zb));
+    pz[i] = fix_to_d(tz_join(sl.za, sl.zb));
   }
 }
 
@@ -1432,12 +1486,14 @@ __global__ void k_cand_resolve(const SlotW<W>* tab, uint64_t mask, CandEnt* cand
 // Random-slot access rates on the table's own footprint, the ceiling the
 // scatter and pass 2 are measured against.  n operations, operation i on
 // slot mix64(i + seed) & mask.  mode 0 (insert): relaxed load of the key
-// word, CAS of an empty key, two 64-bit atomic adds; mode 1 (update): load
-// of the key word and the two atomic adds; mode 2 (read): load of the key,
-// x and both z words; mode 3 (update, one FP64 atomic): load of the key
-// word and one 64-bit floating-point atomic add, the cost of a z kept as a
-// plain double (whose sums would depend on the order of the adds).  A
-// per-thread sum of what was read goes to *sink so that no load is dead.
+// word, CAS of an empty key, one 64-bit atomic add; mode 1 (update): load
+// of the key word and one atomic add (the scatter's z update for a
+// contribution below 2^-11); mode 2 (read): load of the key, x and both z
+// words; mode 3: load of the key word and one 64-bit floating-point atomic
+// add (a z kept as a plain double, whose sums would depend on the order of
+// the adds); mode 4: load of the key word and two int64 atomic adds (a
+// large contribution, both z words).  A per-thread sum of what was read
+// goes to *sink so that no load is dead.
 template <int W>
 __global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t mask, long long n, int mode,
                                                       uint64_t seed, unsigned long long* sink) {
@@ -1455,7 +1511,7 @@ __global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t ma
     const uint64_t k = rload(kw);
     acc += k;
     if (mode == 3) {
-      // one floating-point atomic add in place of the two exact int64 adds
+      // one floating-point atomic add (a z kept as a plain double)
       unsafeAtomicAdd((double*)&sl->za, 1.0);
       continue;
     }
@@ -1464,7 +1520,9 @@ __global__ __launch_bounds__(kWG) void k_access_probe(SlotW<W>* tab, uint64_t ma
       __hip_atomic_compare_exchange_strong(kw, &expect, ((uint64_t)i & kHiMask) | kOcc, __ATOMIC_RELAXED,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_fetch_add(&sl->za, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the scatter's z update: one int64 add to B, both words for a large
+    // contribution (mode 4)
+    if (mode == 4) __hip_atomic_fetch_add(&sl->za, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(&sl->zb, (int64_t)3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (acc == 0x5a5a5a5a5a5a5a5aULL) atomicAdd(sink, 1ULL);
@@ -1612,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_table_digest(const SlotW<W>* tab, uint6
       d = D2{s.k0, s.k1 & kHiMask};
     }
     uint64_t kh, eh;
-    digest_entry(d, s.x, fix_join(s.za, s.zb), kh, eh);
+    digest_entry(d, s.x, tz_join(s.za, s.zb), kh, eh);
     cnt += 1;
     ks += kh;
     es += eh;
@@ -1641,7 +1699,7 @@ __global__ __launch_bounds__(256) void k_table_digest(const SlotW<W>* tab, uint6
     }                      \
   } while (0)
 
-void launch_submatrix(hipStream_t s, int nw, const DevHam& H, const DevState* st, const D2* dets, int k,
+void launch_submatrix(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* dets, int k,
                       double* Hs, int ldh) {
   MC_W(nw, hipLaunchKernelGGL(k_submatrix<W>, dim3(k), dim3(kWG), 0, s, H, st, dets, k, Hs, ldh));
 }
@@ -1677,14 +1735,15 @@ void launch_scatter(hipStream_t s, int nw, const DevHam& H, DevState* st, const 
 
 void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* pdet, const double* dx,
                   const double* Hs, int ldh, void* tab, uint64_t mask, const uint64_t* vis, int vis_words, int M,
-                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot) {
+                  int k, int nchunks, StageEnt* stage, int* stage_cnt, int64_t* part, const uint32_t* eslot,
+                  uint32_t* znorm_list) {
   // Kept entries per chunk: up to 64 the top-L tournament, up to 256 the
   // top-256 merge, above a full sort.
   const int mw = M < kChunk ? M : kChunk;
   const int sel = mw <= 64 ? 1 : (mw <= 256 ? 2 : 0);
 #define MC_P2(MX_, SEL_)                                                                                 \
   hipLaunchKernelGGL((k_pass2<MX_, W, SEL_>), dim3(nchunks, k), dim3(kWG), 0, s, H, st, pdet, dx, Hs, ldh, \
-                     (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part, eslot)
+                     (SlotW<W>*)tab, mask, vis, vis_words, M, nchunks, stage, stage_cnt, part, eslot, znorm_list)
   MC_W(nw, {
     if (H.mixed) {
       if (sel == 1) MC_P2(true, 1);
@@ -1701,14 +1760,15 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
 
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
-                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr) {
+                  CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr,
+                  uint64_t nslots, const uint32_t* znorm_list) {
   // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
   // entries, then four entries per thread).  Beyond: the streaming merge
   // with a window of 2 * Mp entries.
   const int mw = M < kChunk ? M : kChunk;
 #define MC_MERGE(E_, T_, S_)                                                                                 \
   hipLaunchKernelGGL((k_merge<E_, T_, S_, W>), dim3(k), dim3(T_), 0, s, st, pslot, (SlotW<W>*)tab, M, nchunks, \
-                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration, ctr)
+                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration, ctr, nslots, znorm_list)
   MC_W(nw, {
     if ((long long)nchunks * mw <= kSortCap) {
       int w = kWG;

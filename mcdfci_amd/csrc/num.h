@@ -178,6 +178,19 @@ MC_HD void fix_split(i128 v, int64_t& a, int64_t& b) {
 
 MC_HD i128 fix_join(int64_t a, int64_t b) { return (((i128)a) << 32) + (i128)b; }
 
+// A table z is two int64 words, z = (A * 2^53 + B) * 2^-64 exactly.  The
+// scatter adds a contribution |v| < 2^53 to B alone (one atomic) and splits
+// a larger one into A and B in [0, 2^53) (two).  One determinant receives at
+// most one add per column, k <= 256 per iteration, so B moves by less than
+// 2^61 per iteration; every iteration folds each touched B with |B| > 2^61
+// into A (pass 2 lists them, the merge folds them), so |B| stays below 2^62.
+constexpr int kZShift = 53;
+MC_HD i128 tz_join(int64_t a, int64_t b) { return (((i128)a) << kZShift) + (i128)b; }
+MC_HD void tz_split(i128 v, int64_t& a, int64_t& b) {
+  a = (int64_t)(v >> kZShift);                  // floor
+  b = (int64_t)(v - (((i128)a) << kZShift));    // in [0, 2^53)
+}
+
 // ---------------------------------------------------------------- hashing
 MC_HD uint64_t mix64(uint64_t x) {
   x ^= x >> 33;

@@ -18,15 +18,14 @@
 //      c = j mod 4, combined as (S0 + S1) + (S2 + S3) (dot4), every dot
 //      product in the canonical 64-lane order (red64);
 //   2. at scheduled steps (from one less than the previous solve's
-//      count, then every second step) the lowest eigenvalue theta of the Lanczos matrix
-//      T_m by Laguerre's iteration on det(T_m - theta I) from the
-//      Gershgorin lower bound, or from theta - est of the previous check
-//      when that point is certified below the root (monotone and cubically
-//      convergent for the
-//      smallest root; each pass is one division-free leading-minor
-//      recurrence), and its eigenvector y by two inverse-iteration
-//      solves on one LU factorisation with partial pivoting (dgttrf
-//      style; back substitution by reciprocal pivots);
+//      count, then every second step) the lowest eigenvalue theta of the
+//      Lanczos matrix T_m by multisection on Sturm counts (128 points per
+//      round, each count one division-free leading-minor recurrence; the
+//      device runs two points on each lane of one wave), warm-started
+//      from the previous check's theta - est and theta, and its
+//      eigenvector y by two inverse-iteration solves on one LU
+//      factorisation with partial pivoting (dgttrf style; back substitution
+//      by reciprocal pivots);
 //   3. stop when beta_j |y_m| <= 1e-15 |T_m|_inf |y|, at breakdown
 //      (beta_j <= 1e-15 |T_m|_inf), at m = n or at the on-chip capacity
 //      lz_capacity(n); the Ritz vector
@@ -65,7 +64,7 @@ static inline int lz_capacity(int n) {
 }
 
 struct EigPair {
-  double lambda = 0.0;  // Laguerre eigenvalue of the Lanczos matrix
+  double lambda = 0.0;  // lowest eigenvalue of the Lanczos matrix (multisection)
   std::vector<dd> q;    // normalised Ritz vector, first nonzero entry > 0
   int steps = 0;        // Lanczos vectors used
   double tn = 0.0;      // |T_m|_inf at the stop
@@ -173,88 +172,91 @@ static void tri_solve_d(int n, const TriLU& F, const double* rdg, double* b) {
   }
 }
 
-// One pass of the leading-minor recurrence of T - lam*I,
-//   p_i = (d_i - lam) p_{i-1} - e_{i-1}^2 p_{i-2},
-// with its first and second lam-derivatives, rescaled by exact powers of two
-// (2^-300 / 2^300) whenever p_i leaves [2^-300, 2^300].  Returns false as
-// soon as some p_i is not positive (T - lam*I is not positive definite in
-// floating point); otherwise G = f'/f and Hh = (f'/f)^2 - f''/f for
-// f = det(T - lam*I).
-static bool minor_pass(int n, const double* d, const double* e2, double lam, double& G, double& Hh) {
+// Eigenvalues of tridiag(e, d, e) (order m) below s: sign changes of the
+// leading minors p_0 = 1, p_i = (d_i - s) p_{i-1} - e_{i-1}^2 p_{i-2} (a
+// zero minor counts as positive).  After every eighth step (i = 8, 16, ...)
+// the pair (p_i, p_{i-1}) is rescaled by 2^-300 when max(|p_i|, |p_{i-1}|)
+// exceeds 2^300, by 2^300 when it is nonzero and below 2^-300 (exact powers
+// of two).
+static int sturm_count(int m, const double* d, const double* e2, double s) {
   const double big = 0x1p300, small = 0x1p-300;
-  double pm = 1.0, dpm = 0.0, ddpm = 0.0;
-  double p = d[0] - lam, dp = -1.0, ddp = 0.0;
-  if (!(p > 0.0)) return false;
-  for (int i = 1; i < n; ++i) {
-    const double a = d[i] - lam, b = e2[i - 1];
-    double pn = a * p - b * pm;
-    double dpn = (a * dp - b * dpm) - p;
-    double ddpn = (a * ddp - b * ddpm) - 2.0 * dp;
-    if (!(pn > 0.0)) return false;
-    if (pn > big) {
-      pn *= small, dpn *= small, ddpn *= small, p *= small, dp *= small, ddp *= small;
-    } else if (pn < small) {
-      pn *= big, dpn *= big, ddpn *= big, p *= big, dp *= big, ddp *= big;
+  double pm = 1.0, p = d[0] - s;
+  int cnt = p < 0.0 ? 1 : 0;
+  for (int i = 1; i < m; ++i) {
+    const double t = (d[i] - s) * p, u = e2[i - 1] * pm;
+    const double pn = t - u;
+    cnt += ((pn < 0.0) != (p < 0.0)) ? 1 : 0;
+    pm = p;
+    p = pn;
+    if ((i & 7) == 0) {
+      const double a = std::fmax(std::fabs(p), std::fabs(pm));
+      const double sc = a > big ? small : ((a < small && a != 0.0) ? big : 1.0);
+      p = p * sc;
+      pm = pm * sc;
     }
-    pm = p, dpm = dp, ddpm = ddp;
-    p = pn, dp = dpn, ddp = ddpn;
   }
-  G = dp / p;
-  Hh = G * G - ddp / p;
-  return true;
+  return cnt;
 }
 
-// Lowest eigenvalue of tridiag(e, d, e) by Laguerre's iteration from the
-// Gershgorin lower bound, or from `hint` when hint lies above that bound and
-// passes minor_pass.  lam always passes minor_pass (a lower bound).  In
-// exact arithmetic the iteration never passes the smallest root, so a
-// candidate that fails is within rounding of it: the point
-// 1e-13 * (|cand| + 1) below it is tried once, and if it passes it is the
-// answer; otherwise the failed points bound the bisection that follows.  At
-// most 100 iterations.
-static double tri_lowest(int n, const double* d, const double* e, double hint) {
-  if (n == 1) return d[0];
-  std::vector<double> e2(n - 1);
-  for (int i = 0; i + 1 < n; ++i) e2[i] = e[i] * e[i];
-  double lo = d[0], hi = d[0];
-  for (int i = 0; i < n; ++i) {
+// Lowest eigenvalue of tridiag(e, d, e) by multisection with 128 points per
+// round (the device evaluates them two per lane on one wave): the bracket
+// [lo, hi) keeps count(lo) = 0 and count(hi) >= 1; a round counts at
+// lo + (hi - lo) * ((j + 1) / 129), j = 0..127, and keeps the first cell
+// whose right end counts an eigenvalue (the last cell when none does).  The
+// start is the Gershgorin interval padded by 1e-6 of its width; when warm
+// (a previous check of this solve), a first round counts at `hint` and at
+// prev + |prev| 2^-30, which become lo (count 0) and hi (count >= 1) when
+// they lie inside the bracket.  Stops when hi - lo <= 2^-50 max(|lo|, |hi|)
+// + 2^-1000, when a round does not shrink the bracket, or after 40 rounds;
+// returns lo.
+static double ms_lowest(int m, const double* d, const double* e, bool warm, double hint, double prev) {
+  if (m == 1) return d[0];
+  std::vector<double> e2(m, 0.0);
+  for (int i = 0; i + 1 < m; ++i) e2[i] = e[i] * e[i];
+  double glo = HUGE_VAL, ghi = -HUGE_VAL;
+  for (int i = 0; i < m; ++i) {
     double r = 0.0;
     if (i > 0) r = r + std::fabs(e[i - 1]);
-    if (i < n - 1) r = r + std::fabs(e[i]);
-    lo = std::min(lo, d[i] - r);
-    hi = std::max(hi, d[i] + r);
+    if (i < m - 1) r = r + std::fabs(e[i]);
+    glo = std::fmin(glo, d[i] - r);
+    ghi = std::fmax(ghi, d[i] + r);
   }
-  const double pad = (hi - lo) * 1e-6 + 1e-200;
-  double lam = lo - pad;
-  hi = hi + pad;
-  const double nn = (double)n;
-  double G, Hh;
-  if (hint > lam && hint < hi && minor_pass(n, d, e2.data(), hint, G, Hh)) {
-    lam = hint;
-  } else if (!minor_pass(n, d, e2.data(), lam, G, Hh)) {
-    return lam;
-  }
-  for (int it = 0; it < 100; ++it) {
-    double disc = (nn - 1.0) * (nn * Hh - G * G);
-    if (!(disc > 0.0)) disc = 0.0;
-    double cand = lam - nn / (G - std::sqrt(disc));
-    if (!(cand < hi)) cand = lam + 0.5 * (hi - lam);
-    if (!(cand > lam) || !(cand < hi)) break;
-    double G2, H2;
-    if (minor_pass(n, d, e2.data(), cand, G2, H2)) {
-      lam = cand, G = G2, Hh = H2;
-      continue;
+  const double pad = (ghi - glo) * 1e-6 + 1e-200;
+  double lo = glo - pad, hi = ghi + pad;
+  if (warm) {
+    const double hu = prev + std::fabs(prev) * 0x1p-30;
+    const int c0 = sturm_count(m, d, e2.data(), hint);
+    const int c1 = sturm_count(m, d, e2.data(), hu);
+    const double nlo = (hint > lo && hint < hi && c0 == 0) ? hint : lo;
+    const double nhi = (hu > lo && hu < hi && c1 >= 1) ? hu : hi;
+    if (nlo < nhi) {
+      lo = nlo;
+      hi = nhi;
     }
-    hi = cand;
-    const double t = cand - 1e-13 * (std::fabs(cand) + 1.0);
-    if (!(t > lam)) break;
-    if (minor_pass(n, d, e2.data(), t, G2, H2)) {
-      lam = t;
-      break;
-    }
-    hi = t;
   }
-  return lam;
+  double sv[128];
+  for (int round = 0; round < 40; ++round) {
+    const double w = hi - lo;
+    const double tol = std::fmax(std::fabs(lo), std::fabs(hi)) * 0x1p-50 + 0x1p-1000;
+    if (!(w > tol)) break;
+    int first = -1;
+    for (int j = 0; j < 128; ++j) {
+      sv[j] = lo + w * ((double)(j + 1) / 129.0);
+      if (first < 0 && sturm_count(m, d, e2.data(), sv[j]) >= 1) first = j;
+    }
+    double nlo, nhi;
+    if (first < 0) {
+      nlo = sv[127];
+      nhi = hi;
+    } else {
+      nhi = sv[first];
+      nlo = first > 0 ? sv[first - 1] : lo;
+    }
+    if (!(nhi - nlo < w)) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  return lo;
 }
 
 // Eigenvector of tridiag(e, d, e) for eigenvalue lam (lam within rounding
@@ -290,7 +292,7 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
   }
   const int mcap = lz_capacity(n);
   double tn_done = 0.0;  // max row sum of the completed rows of T
-  double theta = 0.0, hint = -HUGE_VAL, tn = 0.0;
+  double theta = 0.0, hint = -HUGE_VAL, prev = 0.0, tn = 0.0;
   int m = 0;
   for (int j = 0; j < n; ++j) {
     const double* qj = &Q[(size_t)j * n];
@@ -321,7 +323,7 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
     tn = std::max(tn_done, last);
     const bool breakdown = !(beta[j] > kLzTol * tn);
     if (m == n || breakdown || m == mcap || (m >= first_check && (m - first_check) % 2 == 0)) {
-      theta = tri_lowest(m, alpha.data(), beta.data(), hint);
+      theta = ms_lowest(m, alpha.data(), beta.data(), hint > -HUGE_VAL, hint, prev);
       y = tri_vector(m, alpha.data(), beta.data(), theta);
       double ss = 0.0;
       for (int i = 0; i < m; ++i) ss = ss + y[i] * y[i];
@@ -332,6 +334,7 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
       // Some eigenvalue of A lies within est / |y| of theta, and the next
       // lowest Ritz value lies in [lambda_min(A), theta].
       hint = theta - est / rn;
+      prev = theta;
     }
     double* qn = &Q[(size_t)(j + 1) * n];
     for (int r = 0; r < n; ++r) qn[r] = w[r] / beta[j];

@@ -392,3 +392,19 @@ def test_driver_z_threshold_search_restarts(workloads, tmp_path):
         ro = o.report()
         assert row[1] == "%.10f" % ro["energy"]
         assert int(row[4]) == int(ro["z_size"])
+
+
+@pytest.mark.parametrize("name,k,M,z_thr,iters,bits", [
+    ("c3_synth", 8, 8, 0.0, 120, 54),   # folds through pass 2's list
+    ("c4", 64, 64, 1e-6, 4, 53),        # the list overflows: whole-table sweeps
+])
+def test_z_word_fold_keeps_the_trajectory(name, k, M, z_thr, iters, bits, workloads, monkeypatch):
+    """A table z is A * 2^53 + B with single-atomic adds to B; B is folded
+    into A once it passes 2^61.  Lowering the fold point to 2^54 / 2^53
+    (MCDFCI_ZNORM_BITS) makes folds constant, through the slot list and
+    through the whole-table sweep that replaces it when it overflows: the
+    trajectory and the table stay bit-exact with the oracle."""
+    print("case fold", name, k, bits, flush=True)
+    monkeypatch.setenv("MCDFCI_ZNORM_BITS", str(bits))
+    mem = 8.0 if name == "c4" else 0.05
+    _run_pair(workloads(name)["fcidump"], k, M, z_thr, iters, max_memory=mem, checkpoints=(1, 3))
