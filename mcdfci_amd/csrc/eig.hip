@@ -122,6 +122,7 @@ struct EigLds {
   int sw[kNMax];
   dd q[kNMax];
   double ev[kNMax], dl[kNMax], Ae[kNMax];
+  double red[2][2][64];  // cross-wave partial sums of the row-owner form
   double sc[16];
   int iv[8];
 };
@@ -367,48 +368,75 @@ __device__ __forceinline__ double rl_double(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+// f(a_i, b_i) for i = 0..n-1 in ascending order, a_i and b_i broadcast by
+// readlane from the lane holding them (lane i mod 64 of a[i / 64]).
+template <class F>
+__device__ __forceinline__ void bcast_each2(const double (&a)[2], const double (&b)[2], int n, F f) {
+  const int n0 = n < 64 ? n : 64;
+  for (int i = 0; i < n0; ++i) f(rl_double(a[0], i), rl_double(b[0], i));
+  for (int i = 64; i < n; ++i) f(rl_double(a[1], i - 64), rl_double(b[1], i - 64));
+}
+
 // Eigenvalues of tridiag(e, d, e) below s0 and below s1: the sign changes
 // of the leading minors p_0 = 1, p_i = (d_i - s) p_{i-1} - e_{i-1}^2 p_{i-2}
 // (a zero minor counts as positive), the two recurrences interleaved.  After
 // every eighth step (i = 8, 16, ...) the pair (p_i, p_{i-1}) is rescaled by
 // 2^-300 when max(|p_i|, |p_{i-1}|) > 2^300, by 2^300 when it is nonzero
-// and below 2^-300 (exact powers of two: signs are unchanged).  The lane's
-// registers dv[b], ev[b] hold d and e^2 of index 64 b + lane.
-template <int NB>
-__device__ __forceinline__ void sturm_count2(int m, const double (&dv)[NB], const double (&ev)[NB], double s0,
-                                             double s1, int& c0, int& c1) {
+// and below 2^-300 (exact powers of two: signs are unchanged, so the sign of
+// the previous minor is carried as a flag).  d and e2 (e^2) are in LDS,
+// read as broadcasts eight steps at a time; the steps run in blocks of
+// eight ending at the rescaling points.
+__device__ __forceinline__ void sturm_count2(int m, const double* d, const double* e2, double s0, double s1,
+                                             int& c0, int& c1) {
   const double big = 0x1p300, small = 0x1p-300;
-  const double d0 = rl_double(dv[0], 0);
+  const double d0 = d[0];
   double pmA = 1.0, pA = d0 - s0, pmB = 1.0, pB = d0 - s1;
-  int cA = pA < 0.0 ? 1 : 0, cB = pB < 0.0 ? 1 : 0;
+  bool negA = pA < 0.0, negB = pB < 0.0;
+  int cA = negA ? 1 : 0, cB = negB ? 1 : 0;
+  auto step = [&](double di, double ei) {
+    const double tA = (di - s0) * pA, uA = ei * pmA;
+    const double tB = (di - s1) * pB, uB = ei * pmB;
+    const double nA = tA - uA, nB = tB - uB;
+    const bool ngA = nA < 0.0, ngB = nB < 0.0;
+    cA += ngA != negA ? 1 : 0;
+    cB += ngB != negB ? 1 : 0;
+    negA = ngA;
+    negB = ngB;
+    pmA = pA;
+    pA = nA;
+    pmB = pB;
+    pB = nB;
+  };
+  int i0 = 1;
+  for (; i0 + 8 <= m; i0 += 8) {  // steps i0 .. i0 + 7, the last one i = 0 mod 8
+    double dv[8], ev[8];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (64 * b >= m) break;
-    const int lim = m - 64 * b < 64 ? m - 64 * b : 64;
-    for (int t = b == 0 ? 1 : 0; t < lim; ++t) {
-      // index i = 64 b + t, 1 <= i < m; e2[i - 1] is lane i - 1 of block (i - 1) / 64
-      const double di = rl_double(dv[b], t);
-      const double e2 = t > 0 ? rl_double(ev[b], t - 1) : rl_double(ev[b - 1], 63);
-      const double tA = (di - s0) * pA, uA = e2 * pmA;
-      const double tB = (di - s1) * pB, uB = e2 * pmB;
-      const double nA = tA - uA, nB = tB - uB;
-      cA += ((nA < 0.0) != (pA < 0.0)) ? 1 : 0;
-      cB += ((nB < 0.0) != (pB < 0.0)) ? 1 : 0;
-      pmA = pA;
-      pA = nA;
-      pmB = pB;
-      pB = nB;
-      if (((64 * b + t) & 7) == 0) {
-        const double aA = fmax(__builtin_fabs(pA), __builtin_fabs(pmA));
-        const double aB = fmax(__builtin_fabs(pB), __builtin_fabs(pmB));
-        const double sA = aA > big ? small : ((aA < small && aA != 0.0) ? big : 1.0);
-        const double sB = aB > big ? small : ((aB < small && aB != 0.0) ? big : 1.0);
-        pA = pA * sA;
-        pmA = pmA * sA;
-        pB = pB * sB;
-        pmB = pmB * sB;
-      }
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i0 + u];
+      ev[u] = e2[i0 - 1 + u];
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) step(dv[u], ev[u]);
+    const double aA = fmax(__builtin_fabs(pA), __builtin_fabs(pmA));
+    const double aB = fmax(__builtin_fabs(pB), __builtin_fabs(pmB));
+    const double sA = aA > big ? small : ((aA < small && aA != 0.0) ? big : 1.0);
+    const double sB = aB > big ? small : ((aB < small && aB != 0.0) ? big : 1.0);
+    pA = pA * sA;
+    pmA = pmA * sA;
+    pB = pB * sB;
+    pmB = pmB * sB;
+  }
+  if (i0 < m) {  // at most seven steps, none at a rescaling point
+    double dv[7], ev[7];
+#pragma unroll
+    for (int u = 0; u < 7; ++u) {
+      const int i = i0 + u < m ? i0 + u : m - 1;
+      dv[u] = d[i];
+      ev[u] = e2[i - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+      if (i0 + u < m) step(dv[u], ev[u]);
   }
   c0 = cA;
   c1 = cB;
@@ -431,13 +459,12 @@ __device__ double wave_lowest(int m, const double* d, const double* e, const dou
                               double prev) {
   constexpr int NB = (kNMax + 63) / 64;
   const int lane = threadIdx.x & 63;
-  double dv[NB], ev[NB];
+  double dv[NB];
   double glo = __builtin_huge_val(), ghi = -__builtin_huge_val();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int i = 64 * b + lane;
     dv[b] = i < m ? d[i] : 0.0;
-    ev[b] = i + 1 < m ? e2g[i] : 0.0;
     if (i < m) {
       double r = 0.0;
       if (i > 0) r = r + __builtin_fabs(e[i - 1]);
@@ -457,7 +484,7 @@ __device__ double wave_lowest(int m, const double* d, const double* e, const dou
   if (warm) {
     const double hu = prev + __builtin_fabs(prev) * 0x1p-30;
     int c0, c1;
-    sturm_count2<NB>(m, dv, ev, hint, hu, c0, c1);
+    sturm_count2(m, d, e2g, hint, hu, c0, c1);
     const double nlo = (hint > lo && hint < hi && c0 == 0) ? hint : lo;
     const double nhi = (hu > lo && hu < hi && c1 >= 1) ? hu : hi;
     if (nlo < nhi) {
@@ -472,7 +499,7 @@ __device__ double wave_lowest(int m, const double* d, const double* e, const dou
     if (!(w > tol)) break;
     const double sa = lo + w * fa, sb = lo + w * fb;
     int ca, cb;
-    sturm_count2<NB>(m, dv, ev, sa, sb, ca, cb);
+    sturm_count2(m, d, e2g, sa, sb, ca, cb);
     const unsigned long long ba = __ballot(ca >= 1), bb = __ballot(cb >= 1);
     double nlo, nhi;
     if (ba != 0) {
@@ -510,6 +537,97 @@ __device__ void tri_vector(EigLds& L, int m, double lam) {
     for (int i = 0; i < m; ++i) mx = fmax(mx, __builtin_fabs(L.y[i]));
     for (int i = 0; i < m; ++i) L.y[i] = L.y[i] / mx;
   }
+}
+
+// v with the entry of lane l (wave-uniform) replaced by x (x is the same
+// in every lane).
+__device__ __forceinline__ double wl_double(double v, double x, int l) { return (int)(threadIdx.x & 63) == l ? x : v; }
+
+// tri_vector on one wave for m <= 64, with the same factorisation, solves
+// and scalings in the same order: the vectors are held one entry per lane
+// (entry i in lane i), the serial recurrences read their entries by
+// readlane and write results by writelane, and the reciprocals of the
+// pivots, the maxima and the divisions, independent per entry, run across
+// the lanes.  Leaves y in L.y; ss = sum_i y_i^2 (ascending), ylast = y_{m-1}.
+__device__ void tri_vector_w(EigLds& L, int m, double lam, double& ss, double& ylast) {
+  const int lane = threadIdx.x & 63;
+  const double D = lane < m ? L.alpha[lane] : 0.0;
+  const double E = lane + 1 < m ? L.beta[lane] : 0.0;
+  double DG = 0.0, DU = 0.0, DU2 = 0.0, MUL = 0.0;
+  int SW = 0;
+  const double tiny = 1e-290;
+  double cur_dg = rl_double(D, 0) - lam;
+  double cur_du = m > 1 ? rl_double(E, 0) : 0.0;
+  for (int i = 0; i + 1 < m; ++i) {
+    const double dl = rl_double(E, i);
+    double next_dg = rl_double(D, i + 1) - lam;
+    double next_du = i + 2 < m ? rl_double(E, i + 1) : 0.0;
+    if (uni(__builtin_fabs(cur_dg) >= __builtin_fabs(dl))) {
+      if (cur_dg == 0.0) cur_dg = tiny;
+      const double mm = dl / cur_dg;
+      MUL = wl_double(MUL, mm, i);
+      DG = wl_double(DG, cur_dg, i);
+      DU = wl_double(DU, cur_du, i);
+      next_dg = next_dg - mm * cur_du;
+    } else {
+      const double mm = cur_dg / dl;
+      MUL = wl_double(MUL, mm, i);
+      SW = (int)(threadIdx.x & 63) == i ? 1 : SW;
+      DG = wl_double(DG, dl, i);
+      DU = wl_double(DU, next_dg, i);
+      const double ndg1 = cur_du - mm * next_dg;
+      if (i + 2 < m) {
+        DU2 = wl_double(DU2, next_du, i);
+        next_du = -mm * next_du;
+      }
+      next_dg = ndg1;
+    }
+    cur_dg = next_dg;
+    cur_du = next_du;
+  }
+  if (cur_dg == 0.0) cur_dg = tiny;
+  DG = wl_double(DG, cur_dg, m - 1);
+  const double RDG = lane < m ? 1.0 / DG : 0.0;
+  double Y = 1.0;
+  for (int it = 0; it < 2; ++it) {
+    double cur = rl_double(Y, 0);
+    for (int i = 0; i + 1 < m; ++i) {
+      const double nb = rl_double(Y, i + 1), mi = rl_double(MUL, i);
+      if (!uni(__builtin_amdgcn_readlane(SW, i) != 0)) {
+        Y = wl_double(Y, cur, i);
+        cur = nb - mi * cur;
+      } else {
+        Y = wl_double(Y, nb, i);
+        cur = cur - mi * nb;
+      }
+    }
+    double b1 = cur * rl_double(RDG, m - 1);
+    Y = wl_double(Y, b1, m - 1);
+    if (m >= 2) {
+      double b0 = (rl_double(Y, m - 2) - rl_double(DU, m - 2) * b1) * rl_double(RDG, m - 2);
+      Y = wl_double(Y, b0, m - 2);
+      for (int i = m - 3; i >= 0; --i) {
+        double t = rl_double(Y, i) - rl_double(DU, i) * b0;
+        t = t - rl_double(DU2, i) * b1;
+        const double nb = t * rl_double(RDG, i);
+        Y = wl_double(Y, nb, i);
+        b1 = b0;
+        b0 = nb;
+      }
+    }
+    double mx = lane < m ? __builtin_fabs(Y) : 0.0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    Y = Y / mx;
+  }
+  if (lane < m) L.y[lane] = Y;
+  double acc = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double yi = rl_double(Y, i);
+    acc = acc + yi * yi;
+  }
+  ss = acc;
+  ylast = rl_double(Y, m - 1);
 }
 
 // Lowest eigenpair of the n x n matrix A by Lanczos from L.v0 (normalised,
@@ -732,79 +850,86 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
   __syncthreads();
 }
 
-// ---------------------------------------------------------------- one-wave form
-// For n <= kEigLdsN the same solve runs on one wave: lane l owns rows l and
-// l + 64 and keeps for each the four interleaved partial sums S_g (columns
-// c = g mod 4, ascending) that the four-wave form spreads over its waves,
-// so every row sum, dot product and Gram-Schmidt coefficient is formed in
-// the same order and the results are bit-identical.  With one wave the
-// barriers below only order the LDS accesses of its lanes.
-constexpr int kR1 = 2;  // rows per lane (n <= 128)
+// ---------------------------------------------------------------- row-owner form
+// For n <= kEigLdsN the same solve runs with one row per lane: wave w of
+// nw = ceil(n / 64) (one or two) owns rows 64 w + lane and keeps, for its
+// row, the four interleaved partial sums S_g (columns c = g mod 4,
+// ascending) that the four-wave form spreads over its waves, so every row
+// sum, dot product and Gram-Schmidt coefficient is formed in the same order
+// and the results are bit-identical.  A dot product adds, for lane l, the
+// term of row l to 0.0 and then the term of row l + 64, and reduces by the
+// xor butterfly: the four-wave form's strided order.  One wave issues at
+// most one instruction every fourth cycle, so a second wave for rows 64..89
+// runs on another SIMD instead of lengthening the first wave's stream.
+constexpr int kR1 = 2;  // entries per lane of the setup vectors (k < 128)
 
-// Row sums of A v, (S0 + S1) + (S2 + S3) with S_g the ascending sum over
-// c = g mod 4 of A(r, c) v_c; A(r, c) = a[c * lda + r]; v in LDS.
-__device__ __forceinline__ void matvec1(const double* a, int lda, int n, const double* v, double (&out)[kR1]) {
-  const int lane = threadIdx.x & 63;
-  double S[kR1][4];
+// Row r of A v: (S0 + S1) + (S2 + S3), S_g the ascending sum over c = g
+// mod 4 of A(r, c) v_c.  A is symmetric (k_submatrix writes both triangles
+// from one value, the border is copied), so row r is read along its
+// contiguous storage a[r * lda + c], eight columns per batch; v in LDS.
+__device__ __forceinline__ double matvec_row(const double* a, int lda, int n, int r, const double* v) {
+  const double* rowp = a + (r < n ? r : n - 1) * lda;
+  double S[4] = {0.0, 0.0, 0.0, 0.0};
+  int c0 = 0;
+  for (; c0 + 8 <= n; c0 += 8) {
+    double vv[8], aa[8];
 #pragma unroll
-  for (int t = 0; t < kR1; ++t)
-#pragma unroll
-    for (int g = 0; g < 4; ++g) S[t][g] = 0.0;
-  int rr[kR1];
-#pragma unroll
-  for (int t = 0; t < kR1; ++t) {
-    const int r = lane + 64 * t;
-    rr[t] = r < n ? r : n - 1;  // loads stay in A; rows beyond n are not used
-  }
-  for (int c0 = 0; c0 < n; c0 += 4) {
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int c = c0 + g;
-      if (c < n) {
-        const double vc = v[c];
-        const double* col = a + (size_t)c * lda;
-#pragma unroll
-        for (int t = 0; t < kR1; ++t) S[t][g] = S[t][g] + col[rr[t]] * vc;
-      }
+    for (int u = 0; u < 8; ++u) {
+      vv[u] = v[c0 + u];
+      aa[u] = rowp[c0 + u];
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) S[u & 3] = S[u & 3] + aa[u] * vv[u];
+  }
+  if (c0 + 4 <= n) {
+    double vv[4], aa[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      vv[u] = v[c0 + u];
+      aa[u] = rowp[c0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) S[u] = S[u] + aa[u] * vv[u];
+    c0 += 4;
   }
 #pragma unroll
-  for (int t = 0; t < kR1; ++t) out[t] = (S[t][0] + S[t][1]) + (S[t][2] + S[t][3]);
+  for (int u = 0; u < 3; ++u)
+    if (c0 + u < n) S[u] = S[u] + rowp[c0 + u] * v[c0 + u];
+  return (S[0] + S[1]) + (S[2] + S[3]);
 }
 
-// lanczos_lowest for A held in LDS (column-major, n x n), one wave.
+// Sum over all rows of a per-row term p (0.0 where the lane's row does not
+// exist): (0.0 + p(row l)) + p(row l + 64) per lane l, then the butterfly;
+// every wave returns the same value.  buf alternates between consecutive
+// calls so that a barrier always separates reuse of one buffer.
+__device__ __forceinline__ double rows_sum(EigLds& L, double p, int nw, int buf) {
+  double s = 0.0 + p;
+  if (nw == 2) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    L.red[buf][w][lane] = p;
+    __syncthreads();
+    s = (0.0 + L.red[buf][0][lane]) + L.red[buf][1][lane];
+  }
+  return bfly_sum(s);
+}
+
+// lanczos_lowest for A held in LDS (n x n, lda n), row-owner form.
 __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, double perturb, double* Q) {
-  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6, bs = blockDim.x;
+  const int r = lane + 64 * w;
+  const bool own = r < n;
+  const bool wave_rows = 64 * w < n;  // wave-uniform: this wave owns a row
   const int mcap = lz_capacity(n);
-  double QC[kR1], QP[kR1], W[kR1];
+  double QC = 0.0, QP = 0.0, W = 0.0;
   {
-    double p0 = 0.0;
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      if (r < n) p0 = p0 + L.v0[r] * L.v0[r];
-    }
-    const double s0 = __builtin_sqrt(bfly_sum(p0));
-    double p1 = 0.0;
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      QP[t] = 0.0;
-      QC[t] = 0.0;
-      if (r < n) {
-        const double q = (s0 > 0.0 ? L.v0[r] / s0 : 0.0) + perturb * lz_hash_unit(r);
-        QC[t] = q;
-        p1 = p1 + q * q;
-      }
-    }
-    const double s1 = __builtin_sqrt(bfly_sum(p1));
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      if (r < n) {
-        QC[t] = QC[t] / s1;
-        Q[r] = QC[t];
-      }
+    const double v = own ? L.v0[r] : 0.0;
+    const double s0 = __builtin_sqrt(rows_sum(L, own ? v * v : 0.0, nw, 0));
+    double q = 0.0;
+    if (own) q = (s0 > 0.0 ? v / s0 : 0.0) + perturb * lz_hash_unit(r);
+    const double s1 = __builtin_sqrt(rows_sum(L, own ? q * q : 0.0, nw, 1));
+    if (own) {
+      QC = q / s1;
+      Q[r] = QC;
     }
   }
   __syncthreads();
@@ -812,76 +937,64 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
   double hint = -__builtin_huge_val(), prev = 0.0;
   int m = 0;
   for (int j = 0; j < n; ++j) {
-    double Av[kR1];
-    matvec1(A, n, n, Q + (size_t)j * n, Av);
-    double pa = 0.0;
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      W[t] = 0.0;
-      if (r < n) {
-        double sv = Av[t];
-        if (j > 0) sv = sv - bprev * QP[t];
-        W[t] = sv;
-        pa = pa + QC[t] * sv;
-      }
+    const double av = matvec_row(A, n, n, r, Q + j * n);
+    double sv = 0.0;
+    if (own) {
+      sv = av;
+      if (j > 0) sv = sv - bprev * QP;
     }
-    const double alpha = bfly_sum(pa);
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      W[t] = W[t] - alpha * QC[t];
-      if (r < n) L.wv[r] = W[t];
-    }
+    const double alpha = rows_sum(L, own ? QC * sv : 0.0, nw, 0);
+    W = own ? sv - alpha * QC : 0.0;
+    if (own) L.wv[r] = W;
     __syncthreads();
-    // Gram-Schmidt coefficients h_i = q_i.w, lane 4i' + g forming S_g over
-    // the rows r = g mod 4 (16 coefficients per pass), as the four-wave form.
-    for (int i0 = 0; i0 <= j; i0 += 16) {
-      const int i = i0 + (lane >> 2), g = lane & 3;
-      double S = 0.0;
-      if (i <= j) {
-        const double* qi = Q + (size_t)i * n;
-        int r = g;
-        for (; r + 12 < n; r += 16) {
-          double qa[4], wa[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            qa[u] = qi[r + 4 * u];
-            wa[u] = L.wv[r + 4 * u];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) S = S + qa[u] * wa[u];
-        }
-        for (; r < n; r += 4) S = S + qi[r] * L.wv[r];
-      }
-      const double s01 = S + lane_xor<1>(S);
-      const double h = s01 + lane_xor<2>(s01);
-      if (g == 0 && i <= j) L.hbuf[i] = h;
-    }
-    __syncthreads();
-    double pb = 0.0;
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      if (r < n) {
-        double acc = W[t];
-        int i = 0;
-        for (; i + 7 <= j; i += 8) {
-          double qa[8], ha[8];
+    // Gram-Schmidt coefficients h_i = q_i.w: lane 4i' + g forms S_g over
+    // the rows r = g mod 4 (16 coefficients per pass, passes alternating
+    // between the waves), combined as (S0 + S1) + (S2 + S3).
+    {
+      const int g = lane & 3;
+      for (int i0 = 16 * w; i0 <= j; i0 += 16 * nw) {
+        const int i = i0 + (lane >> 2);
+        const double* qi = Q + (i <= j ? i : j) * n + g;
+        const double* wp = L.wv + g;
+        double S = 0.0;
+        int b = 0;
+        for (; b + 32 <= n; b += 32) {
+          double qa[8], wa[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
-            qa[u] = Q[(size_t)(i + u) * n + r];
-            ha[u] = L.hbuf[i + u];
+            qa[u] = qi[b + 4 * u];
+            wa[u] = wp[b + 4 * u];
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+          for (int u = 0; u < 8; ++u) S = S + qa[u] * wa[u];
         }
-        for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
-        W[t] = acc;
-        pb = pb + acc * acc;
+        for (; b < n; b += 4)
+          if (b + g < n) S = S + qi[b] * wp[b];
+        const double s01 = S + lane_xor<1>(S);
+        const double h = s01 + lane_xor<2>(s01);
+        if (g == 0 && i <= j) L.hbuf[i] = h;
       }
     }
-    const double beta = __builtin_sqrt(bfly_sum(pb));
+    __syncthreads();
+    // w -= sum_i h_i q_i, ascending i, eight terms per batch
+    if (wave_rows) {
+      const double* qp = Q + (own ? r : n - 1);
+      double acc = W;
+      int i = 0;
+      for (; i + 8 <= j + 1; i += 8) {
+        double qa[8], ha[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          qa[u] = qp[(i + u) * n];
+          ha[u] = L.hbuf[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+      }
+      for (; i <= j; ++i) acc = acc - L.hbuf[i] * qp[i * n];
+      if (own) W = acc;
+    }
+    const double beta = __builtin_sqrt(rows_sum(L, own ? W * W : 0.0, nw, 1));
     if (j > 0) {
       double rj = __builtin_fabs(aprev);
       if (j > 1) rj = rj + __builtin_fabs(bprev2);
@@ -892,7 +1005,7 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
     if (j > 0) last = last + __builtin_fabs(bprev);
     tn = fmax(tn_done, last);
     m = j + 1;
-    if (lane == 0) {
+    if (tid == 0) {
       L.alpha[j] = alpha;
       L.beta[j] = beta;
       L.e2[j] = beta * beta;
@@ -901,12 +1014,19 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
     if (m == n || breakdown || m == mcap || (m >= first_check && ((m - first_check) & 1) == 0)) {
       __syncthreads();
       const double theta = wave_lowest(m, L.alpha, L.beta, L.e2, hint > -__builtin_huge_val(), hint, prev);
-      if (lane == 0) {
+      // eigenvector of T_m and the convergence test: wave 0 for m <= 64,
+      // thread 0 above (the same arithmetic)
+      double ss = 0.0, ylast = 0.0;
+      if (m <= 64) {
+        if (w == 0) tri_vector_w(L, m, theta, ss, ylast);
+      } else if (tid == 0) {
         tri_vector(L, m, theta);
-        double ss = 0.0;
         for (int i = 0; i < m; ++i) ss = ss + L.y[i] * L.y[i];
+        ylast = L.y[m - 1];
+      }
+      if (tid == 0) {
         const double rn = __builtin_sqrt(ss);
-        const double est = beta * __builtin_fabs(L.y[m - 1]);
+        const double est = beta * __builtin_fabs(ylast);
         const bool cv = m == n || breakdown || est <= (kLzTol * tn) * rn;
         L.iv[0] = (cv || m == mcap) ? 1 : 0;
         L.sc[0] = theta;
@@ -918,54 +1038,54 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
       prev = L.sc[3];
       if (L.iv[0]) break;
     }
-#pragma unroll
-    for (int t = 0; t < kR1; ++t) {
-      const int r = lane + 64 * t;
-      QP[t] = QC[t];
-      if (r < n) {
-        const double q = W[t] / beta;
-        QC[t] = q;
-        Q[(size_t)(j + 1) * n + r] = q;
-      }
+    QP = QC;
+    if (own) {
+      QC = W / beta;
+      Q[(j + 1) * n + r] = QC;
     }
     __syncthreads();
     aprev = alpha;
     bprev2 = bprev;
     bprev = beta;
   }
-  // Ritz vector v = sum_i y_i q_i, normalised in double-double.
-  double V[kR1];
-  dd pn{0.0, 0.0};
+  // Ritz vector v = sum_i y_i q_i, normalised in double-double: per lane l,
+  // dd(0) + (row l)^2 + (row l + 64)^2, then the butterfly.
+  double V = 0.0;
+  dd tp{0.0, 0.0};
+  if (wave_rows) {
+    const double* qp = Q + (own ? r : n - 1);
+    double acc = 0.0;
+    int i = 0;
+    for (; i + 8 <= m; i += 8) {
+      double qa[8], ya[8];
 #pragma unroll
-  for (int t = 0; t < kR1; ++t) {
-    const int r = lane + 64 * t;
-    V[t] = 0.0;
-    if (r < n) {
-      double acc = 0.0;
-      int i = 0;
-      for (; i + 7 < m; i += 8) {
-        double qa[8], ya[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          qa[u] = Q[(size_t)(i + u) * n + r];
-          ya[u] = L.y[i + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+      for (int u = 0; u < 8; ++u) {
+        qa[u] = qp[(i + u) * n];
+        ya[u] = L.y[i + u];
       }
-      for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
-      V[t] = acc;
-      pn = dd_add(pn, two_prod(acc, acc));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+    }
+    for (; i < m; ++i) acc = acc + L.y[i] * qp[i * n];
+    if (own) {
+      V = acc;
+      tp = two_prod(acc, acc);
     }
   }
-  const dd nrm = dd_sqrt(bfly_sum_dd(pn));
-#pragma unroll
-  for (int t = 0; t < kR1; ++t) {
-    const int r = lane + 64 * t;
-    if (r < n) L.q[r] = dd_div(mkdd(V[t]), nrm);
+  dd pn{0.0, 0.0};
+  if (nw == 2) {
+    L.red[0][w][lane] = tp.hi;
+    L.red[1][w][lane] = tp.lo;
+    __syncthreads();
+    if (lane < n) pn = dd_add(pn, dd{L.red[0][0][lane], L.red[1][0][lane]});
+    if (lane + 64 < n) pn = dd_add(pn, dd{L.red[0][1][lane], L.red[1][1][lane]});
+  } else if (own) {
+    pn = dd_add(pn, tp);
   }
+  const dd nrm = dd_sqrt(bfly_sum_dd(pn));
+  if (own) L.q[r] = dd_div(mkdd(V), nrm);
   __syncthreads();
-  if (lane == 0) {
+  if (tid == 0) {
     int lead = 0;
     while (lead < n - 1 && dd_zero_p(L.q[lead])) ++lead;
     L.iv[1] = dd_neg_p(L.q[lead]) ? 1 : 0;
@@ -974,13 +1094,13 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
   }
   __syncthreads();
   if (L.iv[1])
-    for (int i = lane; i < n; i += 64) L.q[i] = dd_neg(L.q[i]);
+    for (int i = tid; i < n; i += bs) L.q[i] = dd_neg(L.q[i]);
   __syncthreads();
 }
 
 // The update of src/solver.h:510-538 from the accepted Ritz vector (L.q,
 // L.ev) and its double-double Rayleigh quotient rq, then update_x: shared by
-// the four-wave and the one-wave solvers (block size bs, any multiple of 64
+// the four-wave and the row-owner solvers (block size bs, any multiple of 64
 // up to kWG).
 __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n, bool normal, double y1, dd rq,
                                            double res2, double* dx, double* tabx, int slot_stride,
@@ -1198,75 +1318,101 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   eig_finish(st, L, k, n, normal, y1, rq, res2, dx, tabx, slot_stride, pslot, mode);
 }
 
-// k_eig on one wave for n <= kEigLdsN (k + 1 <= 90): the same setup,
-// solve, restart rule and update in the same arithmetic order.
-__global__ __launch_bounds__(64) void k_eig1(DevState* st, int k, int ldh, const double* Hs, const double* px,
-                                             const double* pz, double* dx, double* tabx, int slot_stride,
-                                             const uint32_t* pslot, int mode) {
+// k_eig in the row-owner form for n <= kEigLdsN (k + 1 <= 90), one wave
+// per 64 rows: the same setup, solve, restart rule and update in the same
+// arithmetic order.
+__global__ __launch_bounds__(128) void k_eig1(DevState* st, int k, int ldh, const double* Hs, const double* px,
+                                              const double* pz, double* dx, double* tabx, int slot_stride,
+                                              const uint32_t* pslot, int mode) {
   __shared__ EigLds L;
   __shared__ double Qlds[kQBudget];  // A, then the Lanczos basis
   if (st->error) return;
-  const int tid = threadIdx.x, bs = blockDim.x;
-  for (int i = tid; i < k; i += bs) {
-    L.x[i] = px[i];
-    L.z[i] = pz[i];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double xTx = 0.0;
-    for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
-    dd xx{st->xx_hi, st->xx_lo};
-    const double yy = dd_add_d(xx, -xTx).hi;
-    L.sc[4] = yy;
-    L.sc[5] = __builtin_sqrt(yy);
-  }
-  __syncthreads();
-  const double yy = L.sc[4], y1 = L.sc[5];
-  const bool normal = __builtin_fabs(yy) > 1e-14;
-  const int n = normal ? k + 1 : k;
-  const int o = normal ? 1 : 0;
-  {
-    // H x, H read from Hs in place (both triangles)
-    double hx[kR1];
-    matvec1(Hs, ldh, k, L.x, hx);
+  const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63, w = tid >> 6;
+  // Every wave holds x, z of entries lane and lane + 64 (k <= 89).
+  double xr[kR1], zr[kR1];
 #pragma unroll
-    for (int t = 0; t < kR1; ++t)
-      if (tid + 64 * t < k) L.Hx[tid + 64 * t] = hx[t];
-  }
-  __syncthreads();
-  if (normal) {
-    if (tid == 0) {
-      double xTz = 0.0, xTHx = 0.0;
-      for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
-      for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
-      dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
-      dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
-      L.sc[6] = dd_div_d(a00, yy).hi;
-      L.v0[0] = y1;
+  for (int t = 0; t < kR1; ++t) {
+    const int i = lane + 64 * t;
+    xr[t] = i < k ? px[i] : 0.0;
+    zr[t] = i < k ? pz[i] : 0.0;
+    if (w == 0 && i < k) {
+      L.x[i] = xr[t];
+      L.z[i] = zr[t];
     }
-    for (int i = tid; i < k; i += bs) {
-      L.b0[i] = (L.z[i] - L.Hx[i]) / y1;
-      L.v0[i + 1] = L.x[i];
-    }
-  } else {
-    for (int i = tid; i < k; i += bs) L.v0[i] = 0.0;
   }
-  __syncthreads();
-  const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
+  // H_sub into LDS at its place A(r + 1, c + 1) of the bordered matrix
+  // (lda k + 1), sixteen loads in flight per lane.
   {
-    constexpr int U = 8;
-    for (int i0 = tid; i0 < n * n; i0 += U * bs) {
+    constexpr int U = 16;
+    const int kk = k * k, na = k + 1;
+    for (int i0 = tid; i0 < kk; i0 += U * bs) {
       double v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
+        const int idx = i0 + u * bs < kk ? i0 + u * bs : kk - 1;
+        const int c = idx / k, r = idx - c * k;
+        v[u] = Hs[(size_t)c * ldh + r];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         const int idx = i0 + u * bs;
-        const int c = idx / n, r = idx - c * n;
-        v[u] = idx < n * n ? Ms(r, c) : 0.0;
+        if (idx < kk) {
+          const int c = idx / k, r = idx - c * k;
+          Qlds[(c + 1) * na + r + 1] = v[u];
+        }
+      }
+    }
+  }
+  // x^T x as the ascending sum (every wave, entries broadcast by readlane)
+  double xTx = 0.0;
+  bcast_each2(xr, xr, k, [&](double xi, double) { xTx = xTx + xi * xi; });
+  const double yy = dd_add_d(dd{st->xx_hi, st->xx_lo}, -xTx).hi;
+  const double y1 = __builtin_sqrt(yy);
+  const bool normal = __builtin_fabs(yy) > 1e-14;
+  const int n = normal ? k + 1 : k;
+  __syncthreads();
+  if (normal) {
+    // H x from the LDS copy, then the border alpha = A(0, 0), b = A(1:, 0)
+    const int r = lane + 64 * w;
+    const double hxr = matvec_row(Qlds + n + 1, n, k, r, L.x);
+    if (r < k) L.Hx[r] = hxr;
+    __syncthreads();
+    double hx[kR1];
+#pragma unroll
+    for (int t = 0; t < kR1; ++t) hx[t] = lane + 64 * t < k ? L.Hx[lane + 64 * t] : 0.0;
+    double xTz = 0.0, xTHx = 0.0;
+    bcast_each2(xr, zr, k, [&](double xi, double zi) { xTz = xTz + xi * zi; });
+    bcast_each2(xr, hx, k, [&](double xi, double hi) { xTHx = xTHx + xi * hi; });
+    const dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
+    const dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+    const double a0 = dd_div_d(a00, yy).hi;
+    if (r < k) {
+      const double b = ((w == 0 ? zr[0] : zr[1]) - hxr) / y1;
+      Qlds[r + 1] = b;
+      Qlds[(r + 1) * n] = b;
+      L.v0[r + 1] = w == 0 ? xr[0] : xr[1];
+    }
+    if (tid == 0) {
+      Qlds[0] = a0;
+      L.v0[0] = y1;
+    }
+  } else {
+    // x is the reference determinant alone (first iteration): A = H_sub,
+    // lda k, reloaded from Hs; start from the pseudo-random vector.
+    constexpr int U = 8;
+    for (int i0 = tid; i0 < k * k; i0 += U * bs) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = i0 + u * bs < k * k ? i0 + u * bs : k * k - 1;
+        const int c = idx / k, r = idx - c * k;
+        v[u] = Hs[(size_t)c * ldh + r];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (i0 + u * bs < n * n) Qlds[i0 + u * bs] = v[u];
+        if (i0 + u * bs < k * k) Qlds[i0 + u * bs] = v[u];
     }
+    for (int i = tid; i < k; i += bs) L.v0[i] = 0.0;
   }
   __syncthreads();
   const int first0 = st->lz_prev - 1 > 4 ? st->lz_prev - 1 : 4;
@@ -1283,11 +1429,9 @@ __global__ __launch_bounds__(64) void k_eig1(DevState* st, int k, int ldh, const
     }
     __syncthreads();
     {
-      double ae[kR1];
-      matvec1(Qlds, n, n, L.ev, ae);
-#pragma unroll
-      for (int t = 0; t < kR1; ++t)
-        if (tid + 64 * t < n) L.Ae[tid + 64 * t] = ae[t];
+      const int r = lane + 64 * w;
+      const double ae = matvec_row(Qlds, n, n, r, L.ev);
+      if (r < n) L.Ae[r] = ae;
     }
     __syncthreads();
     const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
@@ -1304,6 +1448,7 @@ __global__ __launch_bounds__(64) void k_eig1(DevState* st, int k, int ldh, const
       if (tid == 0) st->eig_unconverged += 1;
       break;
     }
+    __syncthreads();
     for (int i = tid; i < n; i += bs) L.v0[i] = L.ev[i];
     __syncthreads();
     first = 4;
@@ -1371,11 +1516,11 @@ void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, 
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
-  // Up to n = k + 1 = kEigLdsN one wave; above, four (the four-way partial
-  // sums assume exactly four waves there).
+  // Up to n = k + 1 = kEigLdsN the row-owner form, one wave per 64 rows;
+  // above, four waves (the four-way partial sums assume exactly four there).
   if (k + 1 <= kEigLdsN)
-    hipLaunchKernelGGL(k_eig1, dim3(1), dim3(64), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride, pslot,
-                       mode);
+    hipLaunchKernelGGL(k_eig1, dim3(1), dim3(k + 1 <= 64 ? 64 : 128), 0, s, st, k, ldh, Hs, px, pz, dx, tabx,
+                       slot_stride, pslot, mode);
   else
     hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride,
                        pslot, mode);
