@@ -159,9 +159,14 @@ def cpu_baseline(fcidump, k, M, z_thr, max_size, seconds, threads):
     return k * n / dt, n, dt
 
 
-def timed_rate(ctx, k, warmup, steps, tune_rounds, graph):
-    """Autotune, warm up, then time `steps` iterations (graph replay)."""
-    ch = ctx.autotune(tune_rounds) if tune_rounds else None
+def timed_rate(ctx, k, warmup, steps, tune_rounds, graph, launch=None):
+    """Autotune (or pin the launch forms given as (elements per thread,
+    tree pick)), warm up, then time `steps` iterations (graph replay)."""
+    if launch:
+        ctx.set_launch(launch[0], launch[1])
+        ch = {"elements_per_thread": launch[0], "tree_pick": launch[1], "pinned": True}
+    else:
+        ch = ctx.autotune(tune_rounds) if tune_rounds else None
     ctx.set_graph(graph)
     ctx.step(warmup)
     ctx.report()
@@ -319,8 +324,9 @@ def c5_leg(args):
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": elems * bpe / sc / 1e9 / HBM_PEAK_GBS,
                        "traffic": measured_traffic("c5", kk, "mixed"), "bytes_per_element": bpe,
                        "elements_per_launch": elems, "avg_launch_us": 1e6 * sc}
-    # two 64-bit atomic adds per element (exact z) plus one CAS per new key
-    res["scatter_atomic_ops_per_s"] = (2 * elems + inserts_per_it) / sc
+    # one 64-bit atomic add per element (the low word of the exact z; a
+    # contribution of 2^-11 or more adds to both words) plus one CAS per new key
+    res["scatter_atomic_ops_per_s"] = (elems + inserts_per_it) / sc
     res["e_mixed"] = r2["energy"]
     res["e_fp64_of_x"] = c.energy_fp64()
     res["mixed_abs_error_vs_fp64_of_x"] = abs(res["e_mixed"] - res["e_fp64_of_x"])
@@ -374,6 +380,9 @@ def main():
     ap.add_argument("--ttt-extra-k", default="256",
                     help="further mixed num_of_coordinate values timed to the threshold (empty: none)")
     ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
+    ap.add_argument("--launch", default="",
+                    help="pin the main run's launch forms instead of tuning: 'EPT,TREE', e.g. '8,1' "
+                         "(profiling runs, so that every iteration runs the same kernels)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -416,7 +425,11 @@ def main():
     log(f"access ceiling on the {max_mem:g} GB table: " + json.dumps(ceiling))
     # 1. throughput of the mixed + batched form
     mb.init()
-    value, dt, r1, choice = timed_rate(mb, k, args.warmup, args.steps, args.tune_rounds, args.graph)
+    pin = None
+    if args.launch:
+        e, t = args.launch.split(",")
+        pin = (int(e), bool(int(t)))
+    value, dt, r1, choice = timed_rate(mb, k, args.warmup, args.steps, args.tune_rounds, args.graph, pin)
     ms_per_step = 1e3 * dt / args.steps
     log(f"autotune: {choice}")
     log(f"throughput: {args.steps} iterations in {dt:.3f} s -> {value:.1f} coord updates/s, "
