@@ -380,6 +380,9 @@ def main():
     ap.add_argument("--ttt-extra-k", default="256",
                     help="further mixed num_of_coordinate values timed to the threshold (empty: none)")
     ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
+    ap.add_argument("--footprint-sweep", default="",
+                    help="only measure the random-slot access rates on tables of these sizes in GB "
+                         "(e.g. '0.5,2,8,32,64') and print one JSON line per size")
     ap.add_argument("--launch", default="",
                     help="pin the main run's launch forms instead of tuning: 'EPT,TREE', e.g. '8,1' "
                          "(profiling runs, so that every iteration runs the same kernels)")
@@ -405,6 +408,26 @@ def main():
         print(json.dumps({"calibration_copy_bytes": n * 8}))
         return
     from mcdfci_amd import Cdfci, PREC_FP64, PREC_MIXED, synth
+
+    if args.footprint_sweep:
+        # Random-slot operation rates against the table's footprint: which
+        # part of the scatter's and pass 2's ceiling is HBM and which is the
+        # caches (the 64 GB table of the norb-24 workload is 0.6-2 % full).
+        tmp = tempfile.mkdtemp(prefix="mcdfci_bench_")
+        info = make_workload(args.workload, tmp)
+        n_ops = 1 << 25
+        modes = ((0, "insert"), (1, "update_1xi64"), (4, "update_2xi64"), (3, "update_1xf64"), (2, "read"))
+        for gb in [float(v) for v in args.footprint_sweep.split(",") if v.strip()]:
+            c = Cdfci(info["fcidump"], num_of_coordinate=8, estimate=8, z_threshold=1e-6, max_memory=gb,
+                      threshold=0.0)
+            slots = c.report()["table_slots"]
+            res = {"max_memory_gb": gb, "table_slots": slots,
+                   "table_gib": slots * bytes_per_element(c.nwords) / 2 ** 30, "ops_per_dispatch": n_ops}
+            for mode, name in modes:
+                res[name + "_gops"] = n_ops / c.access_ceiling(mode, n_ops) / 1e9
+            c.close()
+            print(json.dumps(res), flush=True)
+        return
 
     spec = synth.CONFIGS[args.workload]
     sol = spec.solver
