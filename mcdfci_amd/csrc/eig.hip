@@ -1,19 +1,23 @@
 // Subspace update of the CDFCI iteration (src/solver.h:333-609) and
-// update_x (src/wavefunction.h:425-451) in one workgroup of 256 threads.
+// update_x (src/wavefunction.h:425-451) in one workgroup.
 //
 // The (k+1)-dimensional problem A = [alpha b^T; b H_sub] is solved for its
 // lowest eigenpair by Lanczos from the current state (sqrt(yy), x_I),
 // normalised and perturbed by a fixed 2^-16 pseudo-random vector:
 //   w = A q_j - beta_{j-1} q_{j-1};  alpha_j = q_j.w;  w -= alpha_j q_j;
 //   w -= sum_i (q_i.w) q_i;  beta_j = |w|;  q_{j+1} = w / beta_j.
-// A is read in place: from LDS when n <= kEigLdsN, otherwise straight from
-// the symmetric H_sub in HBM/L2 (k_submatrix fills both triangles) with the
-// border alpha, b in LDS.  Each wave takes the columns c = w mod 4 of every
-// row (four interleaved partial sums, combined (S0 + S1) + (S2 + S3)); after
-// the barrier every wave combines all rows redundantly, so the dot products
-// (64-lane order: strided partials, xor butterfly) and the next Lanczos
-// vector are known to all waves, and each wave keeps its own LDS copy of
-// the q_j entries it multiplies.
+// Row sums of A v are four interleaved partial sums over the column classes
+// c = g mod 4 (ascending), combined (S0 + S1) + (S2 + S3); dot products are
+// 64 strided per-lane partials (rows l, l + 64, ... added to 0.0 in order)
+// reduced by an xor butterfly.  Two forms compute exactly these sums:
+//  * n <= kEigLdsN (k_eig1, below): A in LDS, one wave per 64 rows, each
+//    lane owning one row;
+//  * larger n (k_eig): A read in place from the symmetric H_sub in HBM/L2
+//    (k_submatrix fills both triangles) with the border alpha, b in LDS;
+//    eight waves, wave w forming the class w mod 4 sums of row block w / 4,
+//    then waves 0-3 combine all rows redundantly, so the dot products and
+//    the next Lanczos vector are known to each of them; the other waves
+//    join the products and the Gram-Schmidt coefficients.
 // After the alpha step one classical Gram-Schmidt pass against all previous
 // Lanczos vectors (held in LDS; thread i forms q_i.w as four interleaved
 // sums over the rows) keeps them orthogonal, so a convergence test made
@@ -23,8 +27,8 @@
 // one less than the previous solve's count, then every second step) wave 0
 // finds the lowest eigenvalue of the Lanczos matrix T_m by multisection on
 // Sturm counts (two counts per lane, 128 points per round, warm-started from
-// the previous check) and thread 0 its eigenvector by two inverse-iteration
-// solves; the solve stops when
+// the previous check) and its eigenvector by two inverse-iteration solves
+// (one wave for m <= 64, thread 0 above); the solve stops when
 // beta_j |y_m| <= 1e-15 |T_m|_inf |y|, at breakdown or at m = n.  The Ritz
 // vector is normalised in double-double; the solve is accepted when
 // |A v - rq v| <= 8e-15 |T_m|_inf (rq the Rayleigh quotient) and restarted
@@ -44,7 +48,8 @@ constexpr int kEigLdsN = 90;    // A lives in LDS up to this dimension
 constexpr int kQBudget = 12288;  // LDS doubles for A (n <= kEigLdsN) and the Lanczos basis
 constexpr int kNMax = kMaxK + 1;
 constexpr int kRows = (kNMax + 63) / 64;  // rows per lane
-constexpr int kEigThreads = kWG;          // four waves
+constexpr int kEigThreads = 512;          // k_eig: eight waves, two per column class
+constexpr int kRowsW = (kNMax + 127) / 128;  // rows per lane in mat_partials (two row blocks)
 constexpr double kLzTol = 1e-15;
 constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accepts a solve
 constexpr double kLzPerturb = 0x1p-16;
@@ -139,47 +144,50 @@ struct MatLds {  // symmetric, column-major in LDS
   __device__ __forceinline__ double operator()(int r, int c) const { return a[c * n + r]; }
 };
 
-// Four-way partial row sums of A v: wave w accumulates the columns c = w
-// mod 4 (ascending) of the rows lane, lane + 64, ... (held in registers)
-// and stores S_w(r) in part[w][r].  vw(i) returns v[4i + w].
+// Four-way partial row sums of A v for the class form (k_eig, eight
+// waves, two per class: wave w takes class w mod 4 and the row block w / 4) accumulates the columns c = g mod 4 (ascending) of its rows
+// lane + 64 (h + H t), held in registers, and stores S_g(r) in part[g][r];
+// the class sums are the same whichever wave forms
+// them.  vw(i) returns v[4i + g].
 template <class Mat, class V>
 __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (*part)[kNMax]) {
   constexpr int CB = Mat::kBatch;  // columns whose loads are issued together
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int w = wv & 3, hb = wv >> 2, nhb = blockDim.x >> 8;
   const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
   // Loads are unconditional (indices clamped into A); values outside the
   // matrix are never accumulated.
-  int rcl[kRows];
+  int rcl[kRowsW];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    const int r = lane + 64 * t;
+  for (int t = 0; t < kRowsW; ++t) {
+    const int r = lane + 64 * (hb + nhb * t);
     rcl[t] = r < n ? r : n - 1;
   }
-  double acc[kRows];
+  double acc[kRowsW];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
+  for (int t = 0; t < kRowsW; ++t) acc[t] = 0.0;
   for (int i0 = 0; i0 < nc; i0 += CB) {
-    double a[CB][kRows], v[CB];
+    double a[CB][kRowsW], v[CB];
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
       const int i = i0 + b, c = w + 4 * i;
       v[b] = i < nc ? vw(i) : 0.0;
       const int cc = c < n ? c : n - 1;
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) a[b][t] = 64 * t < n ? A(rcl[t], cc) : 0.0;
+      for (int t = 0; t < kRowsW; ++t) a[b][t] = 64 * (hb + nhb * t) < n ? A(rcl[t], cc) : 0.0;
     }
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
       if (i0 + b < nc) {
 #pragma unroll
-        for (int t = 0; t < kRows; ++t)
-          if (lane + 64 * t < n) acc[t] = acc[t] + a[b][t] * v[b];
+        for (int t = 0; t < kRowsW; ++t)
+          if (lane + 64 * (hb + nhb * t) < n) acc[t] = acc[t] + a[b][t] * v[b];
       }
     }
   }
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    const int r = lane + 64 * t;
+  for (int t = 0; t < kRowsW; ++t) {
+    const int r = lane + 64 * (hb + nhb * t);
     if (r < n) part[w][r] = acc[t];
   }
 }
@@ -207,21 +215,22 @@ struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
 template <class V>
 __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, double (*part)[kNMax]) {
   constexpr int CB = 8;  // columns whose loads are issued together
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int w = wv & 3, hb = wv >> 2, nhb = blockDim.x >> 8;
   const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
   const int o = A.o, kh = n - o;    // H is kh x kh
-  int roff[kRows];
+  int roff[kRowsW];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    int rr = lane + 64 * t - o;
+  for (int t = 0; t < kRowsW; ++t) {
+    int rr = lane + 64 * (hb + nhb * t) - o;
     rr = rr < 0 ? 0 : (rr > kh - 1 ? kh - 1 : rr);
     roff[t] = rr;
   }
-  double acc[kRows];
+  double acc[kRowsW];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) acc[t] = 0.0;
+  for (int t = 0; t < kRowsW; ++t) acc[t] = 0.0;
   for (int i0 = 0; i0 < nc; i0 += CB) {
-    double a[CB][kRows], v[CB];
+    double a[CB][kRowsW], v[CB];
 #pragma unroll
     for (int b = 0; b < CB; ++b) {
       const int i = i0 + b, c = w + 4 * i;
@@ -230,7 +239,7 @@ __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, doubl
       cc = cc < 0 ? 0 : (cc > kh - 1 ? kh - 1 : cc);
       const double* col = A.hs + (size_t)cc * A.ldh;
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) a[b][t] = col[roff[t]];
+      for (int t = 0; t < kRowsW; ++t) a[b][t] = col[roff[t]];
     }
     if (o) {
 #pragma unroll
@@ -238,12 +247,12 @@ __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, doubl
         const int c = w + 4 * (i0 + b);
         if (c == 0) {
 #pragma unroll
-          for (int t = 0; t < kRows; ++t) {
-            const int r = lane + 64 * t;
+          for (int t = 0; t < kRowsW; ++t) {
+            const int r = lane + 64 * (hb + nhb * t);
             if (r < n) a[b][t] = r == 0 ? A.a00 : A.b0[r - 1];
           }
-        } else if (lane == 0 && c < n) {
-          a[b][0] = A.b0[c - 1];
+        } else if (lane == 0 && hb == 0 && c < n) {
+          a[b][0] = A.b0[c - 1];  // row 0: lane 0, block 0, t = 0
         }
       }
     }
@@ -251,14 +260,14 @@ __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, doubl
     for (int b = 0; b < CB; ++b) {
       if (i0 + b < nc) {
 #pragma unroll
-        for (int t = 0; t < kRows; ++t)
-          if (lane + 64 * t < n) acc[t] = acc[t] + a[b][t] * v[b];
+        for (int t = 0; t < kRowsW; ++t)
+          if (lane + 64 * (hb + nhb * t) < n) acc[t] = acc[t] + a[b][t] * v[b];
       }
     }
   }
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    const int r = lane + 64 * t;
+  for (int t = 0; t < kRowsW; ++t) {
+    const int r = lane + 64 * (hb + nhb * t);
     if (r < n) part[w][r] = acc[t];
   }
 }
@@ -639,9 +648,12 @@ __device__ void tri_vector_w(EigLds& L, int m, double lam, double& ss, double& y
 template <class Mat>
 __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, double perturb, double* Q) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  // Waves 0-3 hold every row and form the step; the others (if any) join
+  // the products A q_j and the Gram-Schmidt coefficients only.
+  const bool cw = w < 4;
   const int mcap = lz_capacity(n);
   double QC[kRows], QP[kRows], W[kRows];
-  {
+  if (cw) {
     double p0 = 0.0;
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
@@ -672,36 +684,40 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       }
     }
   }
+  __syncthreads();  // q_0 (L.qw) visible to every wave
   double aprev = 0.0, bprev = 0.0, bprev2 = 0.0, tn_done = 0.0, tn = 0.0;
   double hint = -__builtin_huge_val(), prev = 0.0;  // warm start of the next check (every thread)
   int m = 0;
   for (int j = 0; j < n; ++j) {
-    mat_partials(A, n, [&](int i) { return L.qw[w][i]; }, L.part[j & 1]);
+    mat_partials(A, n, [&](int i) { return L.qw[w & 3][i]; }, L.part[j & 1]);
     __syncthreads();
-    double(*P)[kNMax] = L.part[j & 1];
-    double pa = 0.0;
+    double alpha = 0.0;
+    if (cw) {
+      double(*P)[kNMax] = L.part[j & 1];
+      double pa = 0.0;
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int r = lane + 64 * t;
-      W[t] = 0.0;
-      if (r < n) {
-        double s = (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
-        if (j > 0) s = s - bprev * QP[t];
-        W[t] = s;
-        pa = pa + QC[t] * s;
+      for (int t = 0; t < kRows; ++t) {
+        const int r = lane + 64 * t;
+        W[t] = 0.0;
+        if (r < n) {
+          double s = (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
+          if (j > 0) s = s - bprev * QP[t];
+          W[t] = s;
+          pa = pa + QC[t] * s;
+        }
       }
-    }
-    const double alpha = bfly_sum(pa);
+      alpha = bfly_sum(pa);
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int r = lane + 64 * t;
-      W[t] = W[t] - alpha * QC[t];
-      if (r < n && w == 0) L.wv[r] = W[t];
+      for (int t = 0; t < kRows; ++t) {
+        const int r = lane + 64 * t;
+        W[t] = W[t] - alpha * QC[t];
+        if (r < n && w == 0) L.wv[r] = W[t];
+      }
     }
     __syncthreads();
     // Full reorthogonalisation, one classical Gram-Schmidt pass:
     // h_i = q_i.w as four interleaved sums S_g over the rows r = g mod 4
-    // (ascending), combined (S0 + S1) + (S2 + S3); then every wave subtracts
+    // (ascending), combined (S0 + S1) + (S2 + S3); then waves 0-3 subtract
     // sum_i h_i q_i in ascending i.  Thread 4i + g forms S_g and the four
     // lanes of i combine by two exchanges.
     for (int i0 = 0; i0 <= j; i0 += (int)blockDim.x >> 2) {
@@ -727,55 +743,72 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       if (g == 0 && i <= j) L.hbuf[i] = h;
     }
     __syncthreads();
-    double pb = 0.0;
+    if (cw) {
+      double pb = 0.0;
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int r = lane + 64 * t;
-      if (r < n) {
-        double acc = W[t];
-        int i = 0;
-        for (; i + 7 <= j; i += 8) {
-          double qa[8], ha[8];
+      for (int t = 0; t < kRows; ++t) {
+        const int r = lane + 64 * t;
+        if (r < n) {
+          double acc = W[t];
+          int i = 0;
+          for (; i + 7 <= j; i += 8) {
+            double qa[8], ha[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            qa[u] = Q[(size_t)(i + u) * n + r];
-            ha[u] = L.hbuf[i + u];
+            for (int u = 0; u < 8; ++u) {
+              qa[u] = Q[(size_t)(i + u) * n + r];
+              ha[u] = L.hbuf[i + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
           }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+          for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
+          W[t] = acc;
+          pb = pb + acc * acc;
         }
-        for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
-        W[t] = acc;
-        pb = pb + acc * acc;
       }
+      const double beta = __builtin_sqrt(bfly_sum(pb));
+      if (j > 0) {
+        double rj = __builtin_fabs(aprev);
+        if (j > 1) rj = rj + __builtin_fabs(bprev2);
+        rj = rj + __builtin_fabs(bprev);
+        tn_done = fmax(tn_done, rj);
+      }
+      double last = __builtin_fabs(alpha);
+      if (j > 0) last = last + __builtin_fabs(bprev);
+      tn = fmax(tn_done, last);
+      if (tid == 0) {
+        const int mm = j + 1;
+        const bool breakdown = !(beta > kLzTol * tn);
+        L.alpha[j] = alpha;
+        L.beta[j] = beta;
+        L.e2[j] = beta * beta;
+        L.iv[3] = (mm == n || breakdown || mm == mcap || (mm >= first_check && ((mm - first_check) & 1) == 0)) ? 1 : 0;
+        L.iv[4] = breakdown ? 1 : 0;
+      }
+      aprev = alpha;
+      bprev2 = bprev;
+      bprev = beta;
     }
-    const double beta = __builtin_sqrt(bfly_sum(pb));
-    if (j > 0) {
-      double rj = __builtin_fabs(aprev);
-      if (j > 1) rj = rj + __builtin_fabs(bprev2);
-      rj = rj + __builtin_fabs(bprev);
-      tn_done = fmax(tn_done, rj);
-    }
-    double last = __builtin_fabs(alpha);
-    if (j > 0) last = last + __builtin_fabs(bprev);
-    tn = fmax(tn_done, last);
     m = j + 1;
-    if (tid == 0) {
-      L.alpha[j] = alpha;
-      L.beta[j] = beta;
-      L.e2[j] = beta * beta;
-    }
-    const bool breakdown = !(beta > kLzTol * tn);
-    if (m == n || breakdown || m == mcap || (m >= first_check && ((m - first_check) & 1) == 0)) {
-      __syncthreads();  // this step's alpha, beta visible to wave 0
+    __syncthreads();  // this step's alpha, beta and the check decision
+    if (L.iv[3]) {
       double theta = 0.0;
       if (w == 0) theta = wave_lowest(m, L.alpha, L.beta, L.e2, hint > -__builtin_huge_val(), hint, prev);
-      if (tid == 0) {
+      // eigenvector of T_m: wave 0 for m <= 64, thread 0 above (the same
+      // arithmetic)
+      double ss = 0.0, ylast = 0.0;
+      if (m <= 64) {
+        if (w == 0) tri_vector_w(L, m, theta, ss, ylast);
+      } else if (tid == 0) {
         tri_vector(L, m, theta);
-        double ss = 0.0;
         for (int i = 0; i < m; ++i) ss = ss + L.y[i] * L.y[i];
+        ylast = L.y[m - 1];
+      }
+      if (tid == 0) {
+        const double beta = L.beta[j];
+        const bool breakdown = L.iv[4] != 0;
         const double rn = __builtin_sqrt(ss);
-        const double est = beta * __builtin_fabs(L.y[m - 1]);
+        const double est = beta * __builtin_fabs(ylast);
         const bool cv = m == n || breakdown || est <= (kLzTol * tn) * rn;
         L.iv[0] = (cv || m == mcap) ? 1 : 0;
         L.sc[0] = theta;
@@ -789,52 +822,55 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       prev = L.sc[3];
       if (L.iv[0]) break;
     }
+    if (cw) {
+      const double beta = bprev;
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) {
+        const int r = lane + 64 * t;
+        QP[t] = QC[t];
+        if (r < n) {
+          const double q = W[t] / beta;
+          QC[t] = q;
+          if ((lane & 3) == w) L.qw[w][r >> 2] = q;
+          if ((t & 3) == w) Q[(size_t)(j + 1) * n + r] = q;
+        }
+      }
+    }
+    __syncthreads();  // q_{j+1} (L.qw) visible to every wave
+  }
+  // Ritz vector v = sum_i y_i q_i (waves 0-3, all rows), normalised in
+  // double-double.
+  if (cw) {
+    double V[kRows];
+    dd pn{0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
       const int r = lane + 64 * t;
-      QP[t] = QC[t];
+      V[t] = 0.0;
       if (r < n) {
-        const double q = W[t] / beta;
-        QC[t] = q;
-        if ((lane & 3) == w) L.qw[w][r >> 2] = q;
-        if ((t & 3) == w) Q[(size_t)(j + 1) * n + r] = q;
-      }
-    }
-    aprev = alpha;
-    bprev2 = bprev;
-    bprev = beta;
-  }
-  // Ritz vector v = sum_i y_i q_i (every wave, all rows), normalised in
-  // double-double.
-  double V[kRows];
-  dd pn{0.0, 0.0};
+        double acc = 0.0;
+        int i = 0;
+        for (; i + 7 < m; i += 8) {
+          double qa[8], ya[8];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    const int r = lane + 64 * t;
-    V[t] = 0.0;
-    if (r < n) {
-      double acc = 0.0;
-      int i = 0;
-      for (; i + 7 < m; i += 8) {
-        double qa[8], ya[8];
+          for (int u = 0; u < 8; ++u) {
+            qa[u] = Q[(size_t)(i + u) * n + r];
+            ya[u] = L.y[i + u];
+          }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          qa[u] = Q[(size_t)(i + u) * n + r];
-          ya[u] = L.y[i + u];
+          for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+        for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
+        V[t] = acc;
+        pn = dd_add(pn, two_prod(acc, acc));
       }
-      for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
-      V[t] = acc;
-      pn = dd_add(pn, two_prod(acc, acc));
     }
-  }
-  const dd nrm = dd_sqrt(bfly_sum_dd(pn));
+    const dd nrm = dd_sqrt(bfly_sum_dd(pn));
 #pragma unroll
-  for (int t = 0; t < kRows; ++t) {
-    const int r = lane + 64 * t;
-    if (r < n && w == 0) L.q[r] = dd_div(mkdd(V[t]), nrm);
+    for (int t = 0; t < kRows; ++t) {
+      const int r = lane + 64 * t;
+      if (r < n && w == 0) L.q[r] = dd_div(mkdd(V[t]), nrm);
+    }
   }
   __syncthreads();
   if (tid == 0) {
@@ -854,11 +890,11 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
 // For n <= kEigLdsN the same solve runs with one row per lane: wave w of
 // nw = ceil(n / 64) (one or two) owns rows 64 w + lane and keeps, for its
 // row, the four interleaved partial sums S_g (columns c = g mod 4,
-// ascending) that the four-wave form spreads over its waves, so every row
+// ascending) that the class form spreads over its waves, so every row
 // sum, dot product and Gram-Schmidt coefficient is formed in the same order
 // and the results are bit-identical.  A dot product adds, for lane l, the
 // term of row l to 0.0 and then the term of row l + 64, and reduces by the
-// xor butterfly: the four-wave form's strided order.  One wave issues at
+// xor butterfly: the class form's strided order.  One wave issues at
 // most one instruction every fourth cycle, so a second wave for rows 64..89
 // runs on another SIMD instead of lengthening the first wave's stream.
 constexpr int kR1 = 2;  // entries per lane of the setup vectors (k < 128)
@@ -1100,8 +1136,8 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
 
 // The update of src/solver.h:510-538 from the accepted Ritz vector (L.q,
 // L.ev) and its double-double Rayleigh quotient rq, then update_x: shared by
-// the four-wave and the row-owner solvers (block size bs, any multiple of 64
-// up to kWG).
+// the eight-wave and the row-owner solvers (block size bs, any multiple of
+// 64 up to kEigThreads).
 __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n, bool normal, double y1, dd rq,
                                            double res2, double* dx, double* tabx, int slot_stride,
                                            const uint32_t* pslot, int mode) {
@@ -1155,8 +1191,8 @@ __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n
   // update_x (src/wavefunction.h:425-451): x_t += dx_t; x.z += z_t * dx_t.
   // Picked determinants are distinct, so every thread owns its slots; the
   // x.z increments are exact fixed-point integers (order-free sum).
-  __shared__ long long nx_w[kWG / 64];
-  __shared__ i128 xz_w[kWG / 64];
+  __shared__ long long nx_w[kEigThreads / 64];
+  __shared__ i128 xz_w[kEigThreads / 64];
   long long nx = 0;
   i128 inc = 0;
   for (int t = tid; t < k; t += bs) {
@@ -1197,7 +1233,7 @@ __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n
 // and the table: update_x through tabx, the x of slot 0, and slot_stride,
 // the slot size in doubles).  mode 1: test hook (state holds the inputs; no
 // table access).  Hs holds both triangles of H_sub.
-__global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const double* Hs,
+__global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ldh, const double* Hs,
                                              const double* px, const double* pz, double* dx, double* tabx,
                                              int slot_stride, const uint32_t* pslot, int mode) {
   __shared__ EigLds L;
@@ -1224,7 +1260,7 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
   const int o = normal ? 1 : 0;
   const bool in_lds = n <= kEigLdsN;
   // H x (H read from Hs in place, both triangles).
-  mat_partials(MatSub{Hs, ldh, nullptr, 0.0, 0}, k, [&](int i) { return L.x[4 * i + (tid >> 6)]; }, L.part[0]);
+  mat_partials(MatSub{Hs, ldh, nullptr, 0.0, 0}, k, [&](int i) { return L.x[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
   __syncthreads();
   for (int r = tid; r < k; r += bs) L.Hx[r] = dot4_combine(L.part[0], r);
   __syncthreads();
@@ -1288,9 +1324,9 @@ __global__ __launch_bounds__(kWG) void k_eig(DevState* st, int k, int ldh, const
     }
     __syncthreads();
     if (in_lds)
-      mat_partials(Ml, n, [&](int i) { return L.ev[4 * i + (tid >> 6)]; }, L.part[0]);
+      mat_partials(Ml, n, [&](int i) { return L.ev[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
     else
-      mat_partials(Ms, n, [&](int i) { return L.ev[4 * i + (tid >> 6)]; }, L.part[0]);
+      mat_partials(Ms, n, [&](int i) { return L.ev[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
     __syncthreads();
     for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L.part[0], r);
     __syncthreads();
@@ -1517,7 +1553,8 @@ void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
   // Up to n = k + 1 = kEigLdsN the row-owner form, one wave per 64 rows;
-  // above, four waves (the four-way partial sums assume exactly four there).
+  // above, the class form on eight waves (mat_partials assumes two row
+  // blocks per class there).
   if (k + 1 <= kEigLdsN)
     hipLaunchKernelGGL(k_eig1, dim3(1), dim3(k + 1 <= 64 ? 64 : 128), 0, s, st, k, ldh, Hs, px, pz, dx, tabx,
                        slot_stride, pslot, mode);
