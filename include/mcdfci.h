@@ -73,6 +73,23 @@ typedef struct {
   const double* orbital_energy;  /* norb entries, or NULL (HF = lowest orbitals) */
 } mcdfci_ham;
 
+/* Spin-unrestricted Hamiltonian by spin block (what a UHF FCIDUMP holds,
+ * src/input.h:125-127,203-237, with spin-orbital 2p the alpha and 2p+1 the
+ * beta function of orbital p).  The (beta beta|alpha alpha) block is eri_ab
+ * with the pairs exchanged. */
+typedef struct {
+  int norb;                      /* orbitals per spin (half the spin-orbitals) */
+  int nelec;
+  int ms2;
+  double core;
+  const double* h_alpha;         /* norb*norb */
+  const double* h_beta;          /* norb*norb */
+  const double* eri_aa;          /* norb^4 (p_a q_a|r_a s_a) */
+  const double* eri_bb;          /* norb^4 (p_b q_b|r_b s_b) */
+  const double* eri_ab;          /* norb^4 (p_a q_a|r_b s_b) */
+  const double* orbital_energy;  /* 2*norb spin-orbital entries, or NULL */
+} mcdfci_ham_uhf;
+
 typedef struct {
   double energy;        /* xz / xx (src/wavefunction.h:195-198) */
   double dx_max;        /* max |dx| of the last iteration */
@@ -106,13 +123,19 @@ const char* mcdfci_last_error(const mcdfci_ctx* ctx);
 
 /* Replaces Fcidump + HamiltonianMolecule<N>(fci, threshold) construction
  * (src/input.h:88-152, src/hamiltonian.h:98-109) and the table sizing of
- * src/run_wrapper.h:181,203-213. */
+ * src/run_wrapper.h:181,203-213.  mcdfci_create_uhf takes unrestricted
+ * integrals (the Fcidump::read_integral_uhf path, src/input.h:203-237);
+ * mcdfci_create_from_fcidump reads either kind of file and refuses
+ * integrals that change an electron's spin (MCDFCI_E_INPUT) and an
+ * unrestricted file with an odd number of spin-orbitals. */
 int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** out);
+int mcdfci_create_uhf(const mcdfci_ham_uhf* ham, const mcdfci_opts* opts, mcdfci_ctx** out);
 int mcdfci_create_from_fcidump(const char* path, const mcdfci_opts* opts, mcdfci_ctx** out);
 int mcdfci_destroy(mcdfci_ctx* ctx);
 
-/* info[0..6] = {nwords, spin-orbitals, nelec, ms2, column enumeration size,
- * k, 1 if the FCIDUMP carried orbital energies} */
+/* info[0..7] = {nwords, spin-orbitals, nelec, ms2, column enumeration size,
+ * k, 1 if the FCIDUMP carried orbital energies, 1 if the integrals are
+ * unrestricted} */
 int mcdfci_info(const mcdfci_ctx* ctx, int64_t* info);
 
 /* HamiltonianMolecule::get_hartree_fock (src/hamiltonian.h:469-492). */
@@ -239,7 +262,8 @@ int mcdfci_candidates(mcdfci_ctx* ctx, uint64_t* dets, double* x, double* z, int
  * fixed c, exact by linearity).  At an eigenvector this is the
  * Hellmann-Feynman gradient; it equals the spin-summed 1-RDM and 2-RDM
  * folded onto the unique parameters.  grad_h has norb^2 entries and
- * grad_eri norb^4; energy may be NULL. */
+ * grad_eri norb^4; energy may be NULL.  A context with unrestricted
+ * integrals returns MCDFCI_E_UNSUPPORTED (its parameters are per spin). */
 int mcdfci_energy_gradient(mcdfci_ctx* ctx, double* grad_h, double* grad_eri, double* energy);
 
 /* The FP64 Rayleigh quotient x.Hx / x.x of the current wavefunction x, with
@@ -250,7 +274,8 @@ int mcdfci_energy_gradient(mcdfci_ctx* ctx, double* grad_h, double* grad_eri, do
  * integrals; otherwise ham's h, (pq|rs) and core (same norb) are used for
  * this call, which evaluates the same x under other integrals (e.g. finite
  * differences of the energy at a fixed wavefunction).  Sums are exact fixed
- * point, so the value does not depend on thread order. */
+ * point, so the value does not depend on thread order.  ham must be NULL
+ * for a context with unrestricted integrals (MCDFCI_E_UNSUPPORTED). */
 int mcdfci_energy_fp64(mcdfci_ctx* ctx, const mcdfci_ham* ham, double* energy);
 
 #ifdef __cplusplus

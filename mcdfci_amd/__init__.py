@@ -55,6 +55,13 @@ class Ham(C.Structure):
                 ("orbital_energy", C.POINTER(C.c_double))]
 
 
+class HamUhf(C.Structure):
+    _fields_ = [("norb", C.c_int), ("nelec", C.c_int), ("ms2", C.c_int), ("core", C.c_double),
+                ("h_alpha", C.POINTER(C.c_double)), ("h_beta", C.POINTER(C.c_double)),
+                ("eri_aa", C.POINTER(C.c_double)), ("eri_bb", C.POINTER(C.c_double)),
+                ("eri_ab", C.POINTER(C.c_double)), ("orbital_energy", C.POINTER(C.c_double))]
+
+
 class Report(C.Structure):
     _fields_ = [("energy", C.c_double), ("dx_max", C.c_double), ("x_size", C.c_int64),
                 ("z_size", C.c_int64), ("cols_size", C.c_int64), ("updates", C.c_int64),
@@ -80,6 +87,7 @@ _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 SIGNATURES = {
     "mcdfci_last_error": (C.c_char_p, [C.c_void_p]),
     "mcdfci_create": (C.c_int, [C.POINTER(Ham), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
+    "mcdfci_create_uhf": (C.c_int, [C.POINTER(HamUhf), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     "mcdfci_create_from_fcidump": (C.c_int, [C.c_char_p, C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     "mcdfci_destroy": (C.c_int, [C.c_void_p]),
     "mcdfci_info": (C.c_int, [C.c_void_p, _i64p]),
@@ -128,14 +136,22 @@ def load_library(path: str = LIB_PATH):
 class Cdfci:
     """One device solver context (one HIP device, one stream)."""
 
-    def __init__(self, fcidump=None, *, ham=None, num_of_coordinate=1, estimate=100, z_threshold=0.0,
+    def __init__(self, fcidump=None, *, ham=None, ham_uhf=None, num_of_coordinate=1, estimate=100, z_threshold=0.0,
                  max_memory=1.0, threshold=1e-13, precision=PREC_FP64, device=0, coordinate_update=UPDATE_EIG):
         self.lib = load_library()
         o = Opts(int(num_of_coordinate), int(estimate), float(z_threshold), float(max_memory),
                  float(threshold), int(precision), int(device), int(coordinate_update))
         h = C.c_void_p()
+        dp = C.POINTER(C.c_double)
         if fcidump is not None:
             rc = self.lib.mcdfci_create_from_fcidump(os.fsencode(fcidump), C.byref(o), C.byref(h))
+        elif ham_uhf is not None:
+            # (norb, nelec, ms2, core, h_alpha, h_beta, eri_aa, eri_bb, eri_ab)
+            norb, nelec, ms2, core = ham_uhf[:4]
+            self._uhf = [np.ascontiguousarray(a, dtype=np.float64) for a in ham_uhf[4:9]]
+            hs = HamUhf(int(norb), int(nelec), int(ms2), float(core),
+                        *[a.ctypes.data_as(dp) for a in self._uhf], None)
+            rc = self.lib.mcdfci_create_uhf(C.byref(hs), C.byref(o), C.byref(h))
         else:
             norb, nelec, ms2, core, h1, eri = ham
             self._h1 = np.ascontiguousarray(h1, dtype=np.float64)
@@ -149,8 +165,9 @@ class Cdfci:
         self.h = h
         info = np.zeros(8, np.int64)
         self._check(self.lib.mcdfci_info(self.h, info))
-        self.nwords, self.nso, self.nelec, self.ms2, self.n_enum, self.k, self.has_orbital_energy = (
-            int(v) for v in info[:7])
+        self.nwords, self.nso, self.nelec, self.ms2, self.n_enum, self.k, self.has_orbital_energy, uhf = (
+            int(v) for v in info[:8])
+        self.uhf = bool(uhf)
         self.norb = self.nso // 2
         self.estimate = int(estimate)
 

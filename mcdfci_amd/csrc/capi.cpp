@@ -226,9 +226,14 @@ int check_det(mcdfci_ctx* c, const D2& det, const char* what) {
   return MCDFCI_OK;
 }
 
-// The fp64 tables derived from a spin-restricted Hamiltonian: single-
-// excitation sums S_same/S_opp [p][r][k], the single-excitation filter and
-// the per-spin-orbital diagonal terms.
+// The fp64 tables derived from the integrals: single-excitation sums
+// S_same/S_opp [p][r][k], the single-excitation filter [p][r] (one block
+// per spin when the integrals are unrestricted, else one shared block) and
+// the per-spin-orbital diagonal terms.  For an electron of spin s moving
+// p -> r past an occupied orbital k (src/hamiltonian.h:176-219):
+//   S_same[s][p][r][k] = (p r|k k)_ss - (p k|k r)_ss      (k of spin s)
+//   S_opp[s][p][r][k]  = (p r|k k)_{s s'}                 (k of the other spin)
+// with (p r|k k)_{ab} = eri_ab[p][r][k][k] and (p r|k k)_{ba} = eri_ab[k][k][p][r].
 struct Fp64Tables {
   std::vector<double> ss, so, d1, d2;
   std::vector<uint8_t> sok;
@@ -237,53 +242,92 @@ struct Fp64Tables {
 Fp64Tables make_tables(const SpatialHam& S, double threshold) {
   const int M = S.norb, ns = 2 * M;
   const size_t M2 = (size_t)M * M, M3 = M2 * M;
-  auto E = [&](int p, int q, int r, int s) { return S.eri[(((size_t)p * M + q) * M + r) * M + s]; };
+  const int nsp = S.uhf ? 2 : 1;
+  auto X = [&](const double* e, int p, int q, int r, int s) { return e[(((size_t)p * M + q) * M + r) * M + s]; };
   Fp64Tables T;
-  T.ss.assign(M3, 0.0);
-  T.so.assign(M3, 0.0);
+  T.ss.assign(nsp * M3, 0.0);
+  T.so.assign(nsp * M3, 0.0);
   T.d1.assign(ns, 0.0);
   T.d2.assign((size_t)ns * ns, 0.0);
-  T.sok.assign(M2, 0);
-  for (int p = 0; p < M; ++p)
-    for (int r = 0; r < M; ++r) {
-      double mx = std::fabs(S.h[p * M + r]);
-      for (int k = 0; k < M; ++k) {
-        double a = E(p, r, k, k);
-        double b = E(p, k, k, r);
-        T.ss[((size_t)p * M + r) * M + k] = a - b;
-        T.so[((size_t)p * M + r) * M + k] = a;
-        mx = std::max(mx, std::fabs(a - b));
-        mx = std::max(mx, std::fabs(a));
+  T.sok.assign(nsp * M2, 0);
+  const double* eab = S.eri_block(2);
+  for (int sp = 0; sp < nsp; ++sp) {
+    const double* hs = S.h_spin(sp);
+    const double* es = S.eri_block(sp);
+    for (int p = 0; p < M; ++p)
+      for (int r = 0; r < M; ++r) {
+        double mx = std::fabs(hs[p * M + r]);
+        for (int k = 0; k < M; ++k) {
+          double a = X(es, p, r, k, k);
+          double b = X(es, p, k, k, r);
+          double o = !S.uhf ? a : (sp == 0 ? X(eab, p, r, k, k) : X(eab, k, k, p, r));
+          T.ss[sp * M3 + ((size_t)p * M + r) * M + k] = a - b;
+          T.so[sp * M3 + ((size_t)p * M + r) * M + k] = o;
+          mx = std::max(mx, std::fabs(a - b));
+          mx = std::max(mx, std::fabs(o));
+        }
+        T.sok[sp * M2 + p * M + r] = mx > threshold ? 1 : 0;
       }
-      T.sok[p * M + r] = mx > threshold ? 1 : 0;
-    }
-  for (int i = 0; i < ns; ++i) T.d1[i] = S.h[(i >> 1) * M + (i >> 1)];
+  }
+  for (int i = 0; i < ns; ++i) T.d1[i] = S.h_spin(i & 1)[(i >> 1) * M + (i >> 1)];
   for (int i = 0; i < ns; ++i)
     for (int j = i + 1; j < ns; ++j) {
       int p = i >> 1, q = j >> 1;
-      double v = E(p, p, q, q);
-      if ((i & 1) == (j & 1)) v = v - E(p, q, q, p);
+      double v;
+      if (!S.uhf) {
+        v = X(S.eri.data(), p, p, q, q);
+        if ((i & 1) == (j & 1)) v = v - X(S.eri.data(), p, q, q, p);
+      } else if ((i & 1) == (j & 1)) {
+        const double* es = S.eri_block(i & 1);
+        v = X(es, p, p, q, q) - X(es, p, q, q, p);
+      } else {
+        v = (i & 1) == 0 ? X(eab, p, p, q, q) : X(eab, q, q, p, p);
+      }
       T.d2[(size_t)i * ns + j] = v;
     }
   return T;
 }
 
+// The integral arrays as the device holds them: the alpha-alpha,
+// beta-beta and alpha-beta ERI blocks one after another (one block when
+// the integrals are restricted), and the alpha and beta h.
+void device_integrals(const SpatialHam& S, std::vector<double>& eri, std::vector<double>& h) {
+  eri = S.eri;
+  h = S.h;
+  if (S.uhf) {
+    eri.insert(eri.end(), S.eri_bb.begin(), S.eri_bb.end());
+    eri.insert(eri.end(), S.eri_ab.begin(), S.eri_ab.end());
+    h.insert(h.end(), S.hb.begin(), S.hb.end());
+  }
+}
+
+// Block strides of the device tables: zero for restricted integrals, so
+// both spins read the one block.
+void set_strides(DevHam& H, const SpatialHam& S) {
+  const size_t M2 = (size_t)S.norb * S.norb, M3 = M2 * S.norb, M4 = M2 * M2;
+  H.h_so = S.uhf ? M2 : 0;
+  H.s_so = S.uhf ? M3 : 0;
+  H.e_bb = S.uhf ? M4 : 0;
+  H.e_ab = S.uhf ? 2 * M4 : 0;
+}
+
 int build_tables(mcdfci_ctx* c) {
   const SpatialHam& S = c->ham;
   const int M = S.norb, ns = 2 * M;
-  const size_t M2 = (size_t)M * M, M3 = M2 * M, M4 = M2 * M2;
   const Fp64Tables T = make_tables(S, c->opts.threshold);
-  std::vector<float> eri32(M4), h132(M2), ss32(M3), so32(M3);
-  for (size_t i = 0; i < M4; ++i) eri32[i] = (float)S.eri[i];
-  for (size_t i = 0; i < M2; ++i) h132[i] = (float)S.h[i];
-  for (size_t i = 0; i < M3; ++i) {
+  std::vector<double> eri, h1;
+  device_integrals(S, eri, h1);
+  std::vector<float> eri32(eri.size()), h132(h1.size()), ss32(T.ss.size()), so32(T.so.size());
+  for (size_t i = 0; i < eri.size(); ++i) eri32[i] = (float)eri[i];
+  for (size_t i = 0; i < h1.size(); ++i) h132[i] = (float)h1[i];
+  for (size_t i = 0; i < T.ss.size(); ++i) {
     ss32[i] = (float)T.ss[i];
     so32[i] = (float)T.so[i];
   }
   int rc;
-  if ((rc = upload(c, &c->d_eri, S.eri))) return rc;
+  if ((rc = upload(c, &c->d_eri, eri))) return rc;
   if ((rc = upload(c, &c->d_eri32, eri32))) return rc;
-  if ((rc = upload(c, &c->d_h1, S.h))) return rc;
+  if ((rc = upload(c, &c->d_h1, h1))) return rc;
   if ((rc = upload(c, &c->d_h132, h132))) return rc;
   if ((rc = upload(c, &c->d_ss, T.ss))) return rc;
   if ((rc = upload(c, &c->d_so, T.so))) return rc;
@@ -298,6 +342,7 @@ int build_tables(mcdfci_ctx* c) {
   H.mixed = c->opts.precision == MCDFCI_PREC_MIXED ? 1 : 0;
   H.core = S.core;
   H.thr = c->opts.threshold;
+  set_strides(H, S);
   H.eri = c->d_eri;
   H.eri32 = c->d_eri32;
   H.h1 = c->d_h1;
@@ -475,10 +520,14 @@ extern "C" {
 
 const char* mcdfci_last_error(const mcdfci_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
-int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** out) {
-  if (!ham || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
-  *out = nullptr;
-  if (ham->norb <= 0 || !ham->h || !ham->eri) return fail(nullptr, MCDFCI_E_INPUT, "invalid Hamiltonian");
+}  // extern "C"
+
+namespace {
+
+// Context creation from integrals already in SpatialHam form (restricted or
+// unrestricted); every public create entry point ends here.
+int create_ctx(SpatialHam&& ham_in, const mcdfci_opts* opts, mcdfci_ctx** out) {
+  const SpatialHam* ham = &ham_in;
   if (ham->norb > kMaxOrb2)
     return fail(nullptr, MCDFCI_E_UNSUPPORTED,
                 "the device path supports up to " + std::to_string(kMaxOrb2) + " spatial orbitals");
@@ -519,19 +568,7 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   c->device = opts->device;
   c->nw = ham->norb <= kMaxOrb ? 1 : 2;
   c->nio = 1 + (2 * ham->norb - 1) / 64;
-  SpatialHam& S = c->ham;
-  S.norb = ham->norb;
-  S.nelec = ham->nelec;
-  S.ms2 = ham->ms2;
-  S.core = ham->core;
-  const size_t n2 = (size_t)S.norb * S.norb;
-  S.h.assign(ham->h, ham->h + n2);
-  S.eri.assign(ham->eri, ham->eri + n2 * n2);
-  if (ham->orbital_energy)
-    for (int p = 0; p < S.norb; ++p) {
-      S.orb_e.push_back({2 * p, ham->orbital_energy[p]});
-      S.orb_e.push_back({2 * p + 1, ham->orbital_energy[p]});
-    }
+  c->ham = std::move(ham_in);
   int rc = MCDFCI_OK;
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) rc = fail(c, MCDFCI_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
@@ -577,26 +614,68 @@ int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** o
   return MCDFCI_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int mcdfci_create(const mcdfci_ham* ham, const mcdfci_opts* opts, mcdfci_ctx** out) {
+  if (!ham || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
+  *out = nullptr;
+  if (ham->norb <= 0 || !ham->h || !ham->eri) return fail(nullptr, MCDFCI_E_INPUT, "invalid Hamiltonian");
+  if (ham->norb > kMaxOrb2)
+    return fail(nullptr, MCDFCI_E_UNSUPPORTED,
+                "the device path supports up to " + std::to_string(kMaxOrb2) + " spatial orbitals");
+  SpatialHam S;
+  S.norb = ham->norb;
+  S.nelec = ham->nelec;
+  S.ms2 = ham->ms2;
+  S.core = ham->core;
+  const size_t n2 = (size_t)S.norb * S.norb;
+  S.h.assign(ham->h, ham->h + n2);
+  S.eri.assign(ham->eri, ham->eri + n2 * n2);
+  if (ham->orbital_energy)
+    for (int p = 0; p < S.norb; ++p) {
+      S.orb_e.push_back({2 * p, ham->orbital_energy[p]});
+      S.orb_e.push_back({2 * p + 1, ham->orbital_energy[p]});
+    }
+  return create_ctx(std::move(S), opts, out);
+}
+
+int mcdfci_create_uhf(const mcdfci_ham_uhf* ham, const mcdfci_opts* opts, mcdfci_ctx** out) {
+  if (!ham || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
+  *out = nullptr;
+  if (ham->norb <= 0 || !ham->h_alpha || !ham->h_beta || !ham->eri_aa || !ham->eri_bb || !ham->eri_ab)
+    return fail(nullptr, MCDFCI_E_INPUT, "invalid Hamiltonian");
+  if (ham->norb > kMaxOrb2)
+    return fail(nullptr, MCDFCI_E_UNSUPPORTED,
+                "the device path supports up to " + std::to_string(kMaxOrb2) + " spatial orbitals");
+  SpatialHam S;
+  S.norb = ham->norb;
+  S.nelec = ham->nelec;
+  S.ms2 = ham->ms2;
+  S.core = ham->core;
+  S.uhf = true;
+  const size_t n2 = (size_t)S.norb * S.norb, n4 = n2 * n2;
+  S.h.assign(ham->h_alpha, ham->h_alpha + n2);
+  S.hb.assign(ham->h_beta, ham->h_beta + n2);
+  S.eri.assign(ham->eri_aa, ham->eri_aa + n4);
+  S.eri_bb.assign(ham->eri_bb, ham->eri_bb + n4);
+  S.eri_ab.assign(ham->eri_ab, ham->eri_ab + n4);
+  if (ham->orbital_energy)
+    for (int i = 0; i < 2 * S.norb; ++i) S.orb_e.push_back({i, ham->orbital_energy[i]});
+  return create_ctx(std::move(S), opts, out);
+}
+
 int mcdfci_create_from_fcidump(const char* path, const mcdfci_opts* opts, mcdfci_ctx** out) {
-  if (!path) return fail(nullptr, MCDFCI_E_INPUT, "null path");
+  if (!path || !opts || !out) return fail(nullptr, MCDFCI_E_INPUT, "null argument");
+  *out = nullptr;
   SpatialHam S;
   try {
     S = read_fcidump(path, nullptr);
   } catch (const std::exception& ex) {
     return fail(nullptr, MCDFCI_E_INPUT, ex.what());
   }
-  std::vector<double> oe;
-  mcdfci_ham h{};
-  h.norb = S.norb;
-  h.nelec = S.nelec;
-  h.ms2 = S.ms2;
-  h.core = S.core;
-  h.h = S.h.data();
-  h.eri = S.eri.data();
-  h.orbital_energy = nullptr;
-  int rc = mcdfci_create(&h, opts, out);
-  if (rc == MCDFCI_OK) (*out)->ham.orb_e = S.orb_e;  // keep per-spin-orbital energies as read
-  return rc;
+  return create_ctx(std::move(S), opts, out);
 }
 
 int mcdfci_destroy(mcdfci_ctx* c) {
@@ -622,6 +701,7 @@ int mcdfci_info(const mcdfci_ctx* c, int64_t* info) {
   info[4] = c->dh.n_enum;
   info[5] = c->k;
   info[6] = c->ham.orb_e.empty() ? 0 : 1;
+  info[7] = c->ham.uhf ? 1 : 0;
   return MCDFCI_OK;
 }
 
@@ -750,6 +830,11 @@ uint64_t ham_hash(const SpatialHam& S) {
   feed(S.h.data(), S.h.size() * sizeof(double));
   feed(S.eri.data(), S.eri.size() * sizeof(double));
   feed(&S.core, sizeof(S.core));
+  if (S.uhf) {
+    feed(S.hb.data(), S.hb.size() * sizeof(double));
+    feed(S.eri_bb.data(), S.eri_bb.size() * sizeof(double));
+    feed(S.eri_ab.data(), S.eri_ab.size() * sizeof(double));
+  }
   return h;
 }
 
@@ -1345,6 +1430,8 @@ int mcdfci_energy_fp64(mcdfci_ctx* c, const mcdfci_ham* ham, double* energy) {
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
   if (ham && (ham->norb != c->ham.norb || !ham->h || !ham->eri))
     return fail(c, MCDFCI_E_INPUT, "the integrals must have the context's NORB");
+  if (ham && c->ham.uhf)
+    return fail(c, MCDFCI_E_UNSUPPORTED, "caller integrals are spin-restricted; the context's are unrestricted");
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   DevState s;
@@ -1444,6 +1531,8 @@ int mcdfci_energy_fp64(mcdfci_ctx* c, const mcdfci_ham* ham, double* energy) {
 int mcdfci_energy_gradient(mcdfci_ctx* c, double* grad_h, double* grad_eri, double* energy) {
   if (!c || !grad_h || !grad_eri) return fail(c, MCDFCI_E_INPUT, "null argument");
   if (!c->ready) return fail(c, MCDFCI_E_INPUT, "mcdfci_init has not been called");
+  if (c->ham.uhf)
+    return fail(c, MCDFCI_E_UNSUPPORTED, "the energy gradient is defined for spin-restricted integrals");
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   DevState s;

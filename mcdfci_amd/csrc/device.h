@@ -84,6 +84,11 @@ struct DevHam {
   const uint8_t* single_ok;  // [p][r]
   const double* d1;          // [spin-orbital] h_ii
   const double* d2;          // [i*ns + j] <ij||ij>, i < j
+  // Spin-block strides (elements) for unrestricted integrals, 0 when one
+  // block serves both spins: h1/h132/single_ok of spin s start at s*h_so,
+  // S_same/S_opp of spin s at s*s_so; eri/eri32 hold the alpha-alpha block,
+  // the beta-beta block at e_bb and the alpha-beta block at e_ab.
+  size_t h_so, s_so, e_bb, e_ab;
 };
 
 // Device-resident solver state (scalars and counters).

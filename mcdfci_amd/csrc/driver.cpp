@@ -163,7 +163,7 @@ double run(Value& option) {
   std::cout << "Number of elections: " << info[2] << std::endl;
   std::cout << "Number of spin-orbitals: " << nso << std::endl;
   std::cout << "Ms2: " << info[3] << std::endl;
-  std::cout << "Hartree Fock: restricted" << std::endl << std::endl;
+  std::cout << "Hartree Fock: " << (info[7] ? "unrestricted" : "restricted") << std::endl << std::endl;
   std::cout << "Machine information" << std::endl;
   std::cout << "-------------------" << std::endl;
   std::cout << "sizeof(size_t): 8 bytes or 64 bits" << std::endl;

@@ -67,14 +67,16 @@ __device__ __forceinline__ dd load_xss(const DevState* st) {
 }
 
 // Single excitation value h_pr + sum_{k in occ} <ik||ak> (src/hamiltonian.h:343-347),
-// summed over the occupied spin-orbitals in ascending order.
+// summed over the occupied spin-orbitals in ascending order; the tables of
+// the moving electron's spin.
 template <bool MIXED>
 __device__ __forceinline__ double single_value(const DevHam& H, const uint8_t* occ, int ne, int p,
                                                int r, int spin) {
   const int M = H.M;
-  const size_t base = ((size_t)p * M + r) * M;
+  const size_t base = ((size_t)p * M + r) * M + spin * H.s_so;
+  const size_t hi = spin * H.h_so + p * M + r;
   if (MIXED) {
-    float v = H.h132[p * M + r];
+    float v = H.h132[hi];
     for (int x = 0; x < ne; ++x) {
       int k = occ[x];
       float t = ((k & 1) == spin) ? H.s_same32[base + (k >> 1)] : H.s_opp32[base + (k >> 1)];
@@ -82,7 +84,7 @@ __device__ __forceinline__ double single_value(const DevHam& H, const uint8_t* o
     }
     return (double)v;
   } else {
-    double v = H.h1[p * M + r];
+    double v = H.h1[hi];
     for (int x = 0; x < ne; ++x) {
       int k = occ[x];
       double t = ((k & 1) == spin) ? H.s_same[base + (k >> 1)] : H.s_opp[base + (k >> 1)];
@@ -92,18 +94,21 @@ __device__ __forceinline__ double single_value(const DevHam& H, const uint8_t* o
   }
 }
 
+// (p_a r_a|q_b s_b): the alpha-beta block.
 template <bool MIXED>
 __device__ __forceinline__ double eri_val(const DevHam& H, int p, int r, int q, int s) {
   const int M = H.M;
-  size_t i = (((size_t)p * M + r) * M + q) * M + s;
+  size_t i = (((size_t)p * M + r) * M + q) * M + s + H.e_ab;
   return MIXED ? (double)H.eri32[i] : H.eri[i];
 }
 
+// <pq||rs> of one spin: (pr|qs) - (ps|qr) in that spin's block.
 template <bool MIXED>
-__device__ __forceinline__ double same_spin_val(const DevHam& H, int p, int r, int q, int s) {
+__device__ __forceinline__ double same_spin_val(const DevHam& H, int spin, int p, int r, int q, int s) {
   const int M = H.M;
-  size_t i1 = (((size_t)p * M + r) * M + q) * M + s;
-  size_t i2 = (((size_t)p * M + s) * M + q) * M + r;
+  const size_t o = spin ? H.e_bb : 0;
+  size_t i1 = (((size_t)p * M + r) * M + q) * M + s + o;
+  size_t i2 = (((size_t)p * M + s) * M + q) * M + r + o;
   if (MIXED) return (double)(H.eri32[i1] - H.eri32[i2]);
   return H.eri[i1] - H.eri[i2];
 }
@@ -133,7 +138,7 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtxW<W>& C, i
     }
     int p = spin ? C.ob[o] : C.oa[o];
     int r = spin ? C.vb[u] : C.va[u];
-    if (!H.single_ok[p * H.M + r]) return false;
+    if (!H.single_ok[spin * H.h_so + p * H.M + r]) return false;
     v = single_value<MIXED>(H, C.occ, C.ne, p, r, spin);
     i = 2 * p + spin;
     a = 2 * r + spin;
@@ -163,7 +168,7 @@ __device__ __forceinline__ bool gen_elem(const DevHam& H, const ColCtxW<W>& C, i
     const uint8_t* ol = spin ? C.ob : C.oa;
     const uint8_t* vl = spin ? C.vb : C.va;
     int p = ol[po & 0xff], q = ol[po >> 8], r = vl[pv & 0xff], s = vl[pv >> 8];
-    v = same_spin_val<MIXED>(H, p, r, q, s);
+    v = same_spin_val<MIXED>(H, spin, p, r, q, s);
     if (!(__builtin_fabs(v) > H.thr)) return false;
     i = 2 * p + spin;
     j = 2 * q + spin;
@@ -216,9 +221,9 @@ __device__ double off_diagonal_of(const DevHam& H, const D2& D1, const D2& D2b) 
     int a = dffs<W>(dand(x, D2b));
     int spin = i & 1;
     int p = i >> 1, r = a >> 1;
-    if (!H.single_ok[p * H.M + r]) return 0.0;
-    const size_t base = ((size_t)p * H.M + r) * H.M;
-    double v = H.h1[p * H.M + r];
+    if (!H.single_ok[spin * H.h_so + p * H.M + r]) return 0.0;
+    const size_t base = ((size_t)p * H.M + r) * H.M + spin * H.s_so;
+    double v = H.h1[spin * H.h_so + p * H.M + r];
     D2 o = D1;
     for (int k = dffs<W>(o); k >= 0; k = dffs<W>(o)) {
       dpop_low<W>(o);
@@ -243,8 +248,7 @@ __device__ double off_diagonal_of(const DevHam& H, const D2& D1, const D2& D2b) 
     if ((i & 1) == (j & 1)) {
       a = a0;
       b = a1;
-      v = H.eri[(((size_t)(i >> 1) * H.M + (a >> 1)) * H.M + (j >> 1)) * H.M + (b >> 1)] -
-          H.eri[(((size_t)(i >> 1) * H.M + (b >> 1)) * H.M + (j >> 1)) * H.M + (a >> 1)];
+      v = same_spin_val<false>(H, i & 1, i >> 1, a >> 1, j >> 1, b >> 1);
     } else {
       if ((a0 & 1) == (i & 1)) {
         a = a0;
@@ -253,7 +257,10 @@ __device__ double off_diagonal_of(const DevHam& H, const D2& D1, const D2& D2b) 
         a = a1;
         b = a0;
       }
-      v = H.eri[(((size_t)(i >> 1) * H.M + (a >> 1)) * H.M + (j >> 1)) * H.M + (b >> 1)];
+      // (i a|j b) with i, a of one spin and j, b of the other: the
+      // alpha-beta block with the alpha pair first
+      v = ((i & 1) == 0 || H.e_ab == 0) ? eri_val<false>(H, i >> 1, a >> 1, j >> 1, b >> 1)
+                                        : eri_val<false>(H, j >> 1, b >> 1, i >> 1, a >> 1);
     }
     if ((i & 1) != (a & 1) || (j & 1) != (b & 1)) return 0.0;
     if (!(__builtin_fabs(v) > H.thr)) return 0.0;

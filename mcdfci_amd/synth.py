@@ -21,6 +21,19 @@ PCG64, in exactly this order):
 3. ``(pq|rs) = sum_L B^L_pq B^L_rs`` (8-fold symmetric, positive
    semidefinite as a matrix over pairs).
 
+Unrestricted variant (``uhf=True``): the draws above give the alpha
+integrals; then, from the same generator, ``norb*(norb+1)/2`` draws of
+``N(0, 1)`` over ``p <= q`` scaled by ``uhf_delta`` give a symmetric
+``D`` with ``h_beta = h_alpha + D``, and for each factor ``L`` another
+``norb*(norb+1)/2`` draws scaled by ``uhf_delta * eri_std / (1 + |p - q|)``
+give ``B^L_beta = B^L + E^L``.  The blocks are
+``(pq|rs)_{st} = sum_L B^L_{s,pq} B^L_{t,rs}`` for spins s, t, which keeps the
+spin-orbital tensor positive semidefinite.  Such a Hamiltonian is written in
+the reference's unrestricted dialect (src/input.h:125-127,203-237): NORB
+counts spin-orbitals, ``UHF=.TRUE.``, and every index is a 1-based
+spin-orbital, ``2p + 1`` the alpha and ``2p + 2`` the beta function of
+orbital ``p``.
+
 The CDFCI update takes ``sqrt(-lambda)`` of the subspace eigenvalue
 (src/solver.h:524,587-591), so the Hamiltonian, core energy included, must
 have a negative spectrum.  ``core="auto"`` keeps the core energy 0 when the
@@ -42,6 +55,9 @@ __all__ = [
     "CONFIGS",
     "molecule_integrals",
     "hubbard_integrals",
+    "uhf_integrals",
+    "hf_energy_uhf",
+    "write_fcidump_uhf",
     "hf_energy_closed_shell",
     "write_fcidump",
     "write_input_json",
@@ -66,6 +82,9 @@ class SyntheticSpec:
     eri_std: float = 0.3
     core: object = "auto"          # float or "auto"
     kind: str = "molecule"         # "molecule" or "hubbard"
+    uhf: bool = False              # write the unrestricted dialect (spin-resolved integrals)
+    uhf_delta: float = 0.05        # size of the beta - alpha differences (uhf only)
+    rhf_as_uhf: bool = False       # restricted integrals written in the unrestricted dialect
     hubbard_t: float = 1.0
     hubbard_u: float = 4.0
     solver: dict = field(default_factory=dict)
@@ -93,6 +112,21 @@ CONFIGS = {
                         solver=dict(num_iterations=100000, report_interval=1000,
                                     num_of_coordinate=64, estimate=64,
                                     z_threshold=1e-6, max_memory=64.0)),
+    # unrestricted (spin-resolved) integrals: open shell with 7 orbitals,
+    # closed shell with 12, and the norb=7 molecule written in the
+    # unrestricted dialect (every block equal)
+    "u1": SyntheticSpec("u1", norb=7, nelec=9, ms2=1, rank=14, seed=5, uhf=True,
+                        solver=dict(num_iterations=20000, report_interval=1000,
+                                    num_of_coordinate=8, estimate=8,
+                                    z_threshold=0.0, max_memory=0.005)),
+    "u3": SyntheticSpec("u3", norb=12, nelec=12, rank=24, seed=6, uhf=True,
+                        solver=dict(num_iterations=200000, report_interval=1000,
+                                    num_of_coordinate=8, estimate=8,
+                                    z_threshold=0.0, max_memory=1.0)),
+    "c1_as_uhf": SyntheticSpec("c1_as_uhf", norb=7, nelec=10, rank=14, seed=0, core=9.0, rhf_as_uhf=True,
+                               solver=dict(num_iterations=20000, report_interval=1000,
+                                           num_of_coordinate=8, estimate=8,
+                                           z_threshold=0.0, max_memory=0.005)),
     "c5": SyntheticSpec("c5", norb=42, nelec=48, rank=84, seed=4, h_diag_a=-5.0,
                         h_diag_b=0.25,
                         solver=dict(num_iterations=1000000, report_interval=1000,
@@ -127,6 +161,46 @@ def molecule_integrals(spec: SyntheticSpec):
     return h, eri
 
 
+def uhf_integrals(spec: SyntheticSpec):
+    """Return (h_alpha, h_beta, eri_aa, eri_bb, eri_ab) in chemists' order
+    (eri_ab[p,q,r,s] = (p_a q_a|r_b s_b)); the alpha part equals
+    molecule_integrals(spec)."""
+    n = spec.norb
+    rng = np.random.default_rng(spec.seed)
+    h = np.zeros((n, n))
+    iu = np.triu_indices(n, 1)
+    off = rng.normal(0.0, 1.0, size=len(iu[0])) * (spec.h_offdiag_std * spec.h_offdiag_scale)
+    h[iu] = off
+    h = h + h.T
+    h[np.arange(n), np.arange(n)] = spec.h_diag_a + spec.h_diag_b * np.arange(n)
+    iu0 = np.triu_indices(n, 0)
+    std = spec.eri_std / (1.0 + np.abs(iu0[0] - iu0[1]))
+    B = np.zeros((spec.rank, n, n))
+    for L in range(spec.rank):
+        v = rng.normal(0.0, 1.0, size=len(iu0[0])) * std
+        B[L][iu0] = v
+        B[L] = B[L] + np.triu(B[L], 1).T
+    D = np.zeros((n, n))
+    D[iu0] = rng.normal(0.0, 1.0, size=len(iu0[0])) * spec.uhf_delta
+    D = D + np.triu(D, 1).T
+    hb = h + D
+    Bb = np.zeros_like(B)
+    for L in range(spec.rank):
+        E = np.zeros((n, n))
+        E[iu0] = rng.normal(0.0, 1.0, size=len(iu0[0])) * (spec.uhf_delta * std)
+        Bb[L] = B[L] + E + np.triu(E, 1).T
+
+    def block(X, Y):
+        e = np.einsum("lpq,lrs->pqrs", X, Y, optimize=True)
+        e = 0.5 * (e + e.transpose(1, 0, 2, 3))
+        return 0.5 * (e + e.transpose(0, 1, 3, 2))
+
+    eaa, ebb, eab = block(B, B), block(Bb, Bb), block(B, Bb)
+    eaa = 0.5 * (eaa + eaa.transpose(2, 3, 0, 1))
+    ebb = 0.5 * (ebb + ebb.transpose(2, 3, 0, 1))
+    return h, hb, eaa, ebb, eab
+
+
 def hubbard_integrals(spec: SyntheticSpec):
     """Periodic 1D Hubbard ring in the site basis: h_{i,i+-1} = -t, (ii|ii) = U."""
     n = spec.norb
@@ -154,6 +228,23 @@ def hf_energy_closed_shell(h, eri, nalpha, nbeta, core=0.0):
     for p in ea:
         for q in eb:
             e += eri[p, p, q, q]
+    return float(e)
+
+
+def hf_energy_uhf(ha, hb, eaa, ebb, eab, nalpha, nbeta, core=0.0):
+    """Energy of the determinant with the lowest nalpha/nbeta orbitals of
+    each spin, from spin-resolved integrals."""
+    ea, eb = list(range(nalpha)), list(range(nbeta))
+    e = core + sum(ha[p, p] for p in ea) + sum(hb[p, p] for p in eb)
+    for i, p in enumerate(ea):
+        for q in ea[i + 1:]:
+            e += eaa[p, p, q, q] - eaa[p, q, q, p]
+    for i, p in enumerate(eb):
+        for q in eb[i + 1:]:
+            e += ebb[p, p, q, q] - ebb[p, q, q, p]
+    for p in ea:
+        for q in eb:
+            e += eab[p, p, q, q]
     return float(e)
 
 
@@ -197,6 +288,48 @@ def write_fcidump(path, h, eri, nelec, ms2, core, tol=0.0):
     return hashlib.sha256(text.encode()).hexdigest()
 
 
+def write_fcidump_uhf(path, ha, hb, eaa, ebb, eab, nelec, ms2, core, tol=0.0):
+    """Write spin-resolved integrals in the unrestricted dialect: NORB =
+    2 n spin-orbitals, 1-based spin-orbital indices (2p + 1 alpha, 2p + 2
+    beta); same-spin blocks as unique 8-fold (ia|jb), the alpha-beta block
+    as (i_a a_a|j_b b_b) with i >= a and j >= b."""
+    n = ha.shape[0]
+    al = lambda p: 2 * p + 1  # noqa: E731
+    be = lambda p: 2 * p + 2  # noqa: E731
+    lines = [" &FCI NORB=%d,NELEC=%d,MS2=%d," % (2 * n, nelec, ms2),
+             "  ORBSYM=" + ",".join(["1"] * (2 * n)) + ",", "  ISYM=1,", "  UHF=.TRUE.,", " &END"]
+    fmt = "%25.17E %4d %4d %4d %4d"
+    for eri, f in ((eaa, al), (ebb, be)):
+        for i in range(n):
+            for a in range(i + 1):
+                ia = i * (i + 1) // 2 + a
+                for j in range(n):
+                    for b in range(j + 1):
+                        if j * (j + 1) // 2 + b > ia:
+                            continue
+                        v = eri[i, a, j, b]
+                        if abs(v) > tol:
+                            lines.append(fmt % (v, f(i), f(a), f(j), f(b)))
+    for i in range(n):
+        for a in range(i + 1):
+            for j in range(n):
+                for b in range(j + 1):
+                    v = eab[i, a, j, b]
+                    if abs(v) > tol:
+                        lines.append(fmt % (v, al(i), al(a), be(j), be(b)))
+    for hs, f in ((ha, al), (hb, be)):
+        for i in range(n):
+            for a in range(i + 1):
+                v = hs[i, a]
+                if abs(v) > tol:
+                    lines.append(fmt % (v, f(i), f(a), 0, 0))
+    lines.append(fmt % (core, 0, 0, 0, 0))
+    text = "\n".join(lines) + "\n"
+    with open(path, "w") as fh:
+        fh.write(text)
+    return hashlib.sha256(text.encode()).hexdigest()
+
+
 def write_input_json(path, fcidump_path, solver: dict, threshold=0.0):
     import json
     cd = {
@@ -215,21 +348,34 @@ def generate(name_or_spec, outdir, **solver_overrides):
     """Write FCIDUMP + input.json for a named workload; return a summary dict."""
     spec = CONFIGS[name_or_spec] if isinstance(name_or_spec, str) else name_or_spec
     os.makedirs(outdir, exist_ok=True)
-    if spec.kind == "hubbard":
-        h, eri = hubbard_integrals(spec)
-    else:
-        h, eri = molecule_integrals(spec)
     na = (spec.nelec + spec.ms2) // 2
     nb = (spec.nelec - spec.ms2) // 2
-    core = _auto_core(h, eri, na, nb) if spec.core == "auto" else float(spec.core)
     fcid = os.path.join(outdir, "FCIDUMP")
-    sha = write_fcidump(fcid, h, eri, spec.nelec, spec.ms2, core)
+    if spec.uhf:
+        blocks = uhf_integrals(spec)
+        e0 = hf_energy_uhf(*blocks, na, nb, 0.0)
+        core = (0.0 if e0 < -5.0 else -float(math.ceil(e0 + 10.0))) if spec.core == "auto" else float(spec.core)
+        sha = write_fcidump_uhf(fcid, *blocks, spec.nelec, spec.ms2, core)
+        h, eri, e_hf = None, None, hf_energy_uhf(*blocks, na, nb, core)
+    else:
+        if spec.kind == "hubbard":
+            h, eri = hubbard_integrals(spec)
+        else:
+            h, eri = molecule_integrals(spec)
+        core = _auto_core(h, eri, na, nb) if spec.core == "auto" else float(spec.core)
+        if spec.rhf_as_uhf:
+            blocks = (h, h, eri, eri, eri)
+            sha = write_fcidump_uhf(fcid, *blocks, spec.nelec, spec.ms2, core)
+        else:
+            blocks = None
+            sha = write_fcidump(fcid, h, eri, spec.nelec, spec.ms2, core)
+        e_hf = hf_energy_closed_shell(h, eri, na, nb, core)
     solver = dict(spec.solver)
     solver.update(solver_overrides)
     inp = os.path.join(outdir, "input.json")
     write_input_json(inp, fcid, solver)
     return {"name": spec.name, "fcidump": fcid, "input": inp, "sha256": sha, "core": core,
-            "hf_energy": hf_energy_closed_shell(h, eri, na, nb, core), "h": h, "eri": eri}
+            "hf_energy": e_hf, "h": h, "eri": eri, "uhf_blocks": blocks}
 
 
 if __name__ == "__main__":  # pragma: no cover

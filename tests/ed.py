@@ -89,7 +89,20 @@ def _create(D, valid, sign, s):
 
 def hamiltonian(h, eri, core, na, nb, tol=0.0):
     """Dense H over the (na, nb) sector; returns (dets, H)."""
-    n = h.shape[0]
+    return _hamiltonian((h, h), lambda sp, st: eri, core, na, nb, tol)
+
+
+def hamiltonian_uhf(ha, hb, eaa, ebb, eab, core, na, nb, tol=0.0):
+    """Dense H from spin-resolved integrals: h of spin s, and (pq|rs) of
+    spins (s, t) = eaa, ebb, eab[p,q,r,s] (alpha pair first) and
+    eab[r,s,p,q] for (beta, alpha)."""
+    eba = np.ascontiguousarray(eab.transpose(2, 3, 0, 1))
+    blocks = {(0, 0): eaa, (1, 1): ebb, (0, 1): eab, (1, 0): eba}
+    return _hamiltonian((ha, hb), lambda sp, st: blocks[(sp, st)], core, na, nb, tol)
+
+
+def _hamiltonian(hs, block, core, na, nb, tol):
+    n = hs[0].shape[0]
     dets = basis(n, na, nb)
     dim = len(dets)
     H = np.zeros((dim, dim))
@@ -100,23 +113,25 @@ def hamiltonian(h, eri, core, na, nb, tol=0.0):
         idx = np.searchsorted(dets, Dn[valid])
         np.add.at(H, (idx, cols[valid]), coef * sign[valid])
 
-    for p in range(n):
-        for q in range(n):
-            if abs(h[p, q]) <= tol:
-                continue
-            for sp in (0, 1):
+    for sp in (0, 1):
+        h = hs[sp]
+        for p in range(n):
+            for q in range(n):
+                if abs(h[p, q]) <= tol:
+                    continue
                 D = dets.copy()
                 valid = np.ones(dim, bool)
                 sign = np.ones(dim)
                 D, valid, sign = _annihilate(D, valid, sign, 2 * q + sp)
                 D, valid, sign = _create(D, valid, sign, 2 * p + sp)
                 scatter(D, valid, sign, h[p, q])
-    for p, q, r, s in itertools.product(range(n), repeat=4):
-        v = eri[p, q, r, s]
-        if abs(v) <= tol:
-            continue
-        for sp in (0, 1):
-            for st in (0, 1):
+    for sp in (0, 1):
+        for st in (0, 1):
+            eri = block(sp, st)
+            for p, q, r, s in itertools.product(range(n), repeat=4):
+                v = eri[p, q, r, s]
+                if abs(v) <= tol:
+                    continue
                 D = dets.copy()
                 valid = np.ones(dim, bool)
                 sign = np.ones(dim)

@@ -1,7 +1,9 @@
 """The CPU oracle against the reference's own code (oracle/_ref/libref.so,
 compiled from /root/reference/src by oracle/Makefile).
 
-Pins: FCIDUMP parsing and RHF expansion (src/input.h), the excitation
+Pins: FCIDUMP parsing, the RHF expansion and the unrestricted dialect
+(src/input.h:125-127,203-285; the u1 open shell and the c1 molecule
+rewritten in spin-orbitals), the excitation
 tables, get_column / get_diagonal / get_off_diagonal / get_hartree_fock
 (src/hamiltonian.h), CDFCI::get_hamiltonian_submatrix and the gcd_grad pick
 (src/solver.h), and WaveFunctionCuckoo::update_x / update_z_and_get_sub at
@@ -15,7 +17,7 @@ from oracle_ctypes import Oracle, RefHam, ref_available, ref_lib
 
 pytestmark = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (no reference sources)")
 
-CASES = ["c1", "c3_synth", "c3_hubbard"]
+CASES = ["c1", "c3_synth", "c3_hubbard", "u1", "u3", "c1_as_uhf"]
 
 
 def _as_dict(dets, vals):
@@ -95,7 +97,8 @@ def test_pick_matches_reference(workloads):
         assert out_o.tolist() == out_r.tolist()
 
 
-@pytest.mark.parametrize("name,z_thr", [("c1", 0.0), ("c3_synth", 0.0), ("c3_synth", 1e-3)])
+@pytest.mark.parametrize("name,z_thr", [("c1", 0.0), ("c3_synth", 0.0), ("c3_synth", 1e-3), ("u1", 0.0),
+                                         ("u3", 1e-3)])
 def test_wavefunction_k1_matches_reference(workloads, name, z_thr):
     """update_x + update_z_and_get_sub with one coordinate per step."""
     path = workloads(name)["fcidump"]
