@@ -241,7 +241,17 @@ def time_to_threshold(fcid, args, z_thr, max_mem):
             if name != "plain_fp64":
                 t = rec[name]["seconds"]
                 rec[name]["speedup_vs_plain"] = (t_plain / t) if (t_plain and t) else None
+        for v in inter.values():
+            for name in runs:
+                if name != "plain_fp64":
+                    t = v.get(f"{name}_s")
+                    v[f"{name}_speedup"] = (v["plain_s"] / t) if t else None
         rec["intermediate"] = inter
+        # how far E_ref is from the lowest energy an FP64 run reached on the
+        # same table (the batched FP64 run keeps going down after the plain
+        # run's table is full)
+        rec["e_ref_minus_lowest_fp64_final"] = e_ref - min(rec[nm]["final_energy"] for nm, (_, _, kf) in runs.items()
+                                                            if nm in ("plain_fp64", "fp64_batched"))
         # energy of each form at equal wall time: the mixed run's last report
         t_last = runs["mixed"][0][-1][0]
         rec["mixed_minus_plain_at_equal_time"] = runs["mixed"][0][-1][2] - min(
@@ -261,6 +271,13 @@ def time_to_threshold(fcid, args, z_thr, max_mem):
         "repetitions": len(reps),
         "speedup_median": med([r["mixed"]["speedup_vs_plain"] for r in reps]),
         "speedup_per_rep": [r["mixed"]["speedup_vs_plain"] for r in reps],
+        # the same ratio at the energies the plain run reached at 1/4, 1/2
+        # and 3/4 of its wall time (targets before its table fills)
+        "speedup_at_intermediate_targets_median": med([r["intermediate"][f]["mixed_speedup"] for r in reps
+                                                       for f in r["intermediate"]]),
+        "speedup_at_intermediate_targets_min": min([r["intermediate"][f]["mixed_speedup"] for r in reps
+                                                    for f in r["intermediate"]
+                                                    if r["intermediate"][f]["mixed_speedup"]], default=None),
         "batching_only_fp64_k%d_speedup_median" % args.k: med([r["fp64_batched"]["speedup_vs_plain"]
                                                                 for r in reps]),
         "precision_only_speedup_median": med([

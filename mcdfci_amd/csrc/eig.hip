@@ -7,9 +7,11 @@
 //   w = A q_j - beta_{j-1} q_{j-1};  alpha_j = q_j.w;  w -= alpha_j q_j;
 //   w -= sum_i (q_i.w) q_i;  beta_j = |w|;  q_{j+1} = w / beta_j.
 // Row sums of A v are four interleaved partial sums over the column classes
-// c = g mod 4 (ascending), combined (S0 + S1) + (S2 + S3); dot products are
-// 64 strided per-lane partials (rows l, l + 64, ... added to 0.0 in order)
-// reduced by an xor butterfly.  Two forms compute exactly these sums:
+// c = g mod 4 (ascending fused multiply-add chains from 0.0), combined
+// (S0 + S1) + (S2 + S3); dot products are 64 strided per-lane partials
+// (rows l, l + 64, ... added to 0.0 in order) reduced by an xor butterfly;
+// w - beta q, w - alpha q, the Gram-Schmidt subtraction (ascending i) and
+// the Ritz vector are fused multiply-adds.  Two forms compute exactly these sums:
 //  * n <= kEigLdsN (k_eig1, below): A in LDS, one wave per 64 rows, each
 //    lane owning one row;
 //  * larger n (k_eig): A read in place from the symmetric H_sub in HBM/L2
@@ -181,7 +183,7 @@ __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (
       if (i0 + b < nc) {
 #pragma unroll
         for (int t = 0; t < kRowsW; ++t)
-          if (lane + 64 * (hb + nhb * t) < n) acc[t] = acc[t] + a[b][t] * v[b];
+          if (lane + 64 * (hb + nhb * t) < n) acc[t] = __builtin_fma(a[b][t], v[b], acc[t]);
       }
     }
   }
@@ -261,7 +263,7 @@ __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, doubl
       if (i0 + b < nc) {
 #pragma unroll
         for (int t = 0; t < kRowsW; ++t)
-          if (lane + 64 * (hb + nhb * t) < n) acc[t] = acc[t] + a[b][t] * v[b];
+          if (lane + 64 * (hb + nhb * t) < n) acc[t] = __builtin_fma(a[b][t], v[b], acc[t]);
       }
     }
   }
@@ -701,7 +703,7 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
         W[t] = 0.0;
         if (r < n) {
           double s = (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
-          if (j > 0) s = s - bprev * QP[t];
+          if (j > 0) s = __builtin_fma(-bprev, QP[t], s);
           W[t] = s;
           pa = pa + QC[t] * s;
         }
@@ -710,7 +712,7 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
 #pragma unroll
       for (int t = 0; t < kRows; ++t) {
         const int r = lane + 64 * t;
-        W[t] = W[t] - alpha * QC[t];
+        W[t] = __builtin_fma(-alpha, QC[t], W[t]);
         if (r < n && w == 0) L.wv[r] = W[t];
       }
     }
@@ -734,9 +736,9 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
             wa[u] = L.wv[r + 4 * u];
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) S = S + qa[u] * wa[u];
+          for (int u = 0; u < 4; ++u) S = __builtin_fma(qa[u], wa[u], S);
         }
-        for (; r < n; r += 4) S = S + qi[r] * L.wv[r];
+        for (; r < n; r += 4) S = __builtin_fma(qi[r], L.wv[r], S);
       }
       const double s01 = S + lane_xor<1>(S);
       const double h = s01 + lane_xor<2>(s01);
@@ -759,9 +761,9 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
               ha[u] = L.hbuf[i + u];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+            for (int u = 0; u < 8; ++u) acc = __builtin_fma(-ha[u], qa[u], acc);
           }
-          for (; i <= j; ++i) acc = acc - L.hbuf[i] * Q[(size_t)i * n + r];
+          for (; i <= j; ++i) acc = __builtin_fma(-L.hbuf[i], Q[(size_t)i * n + r], acc);
           W[t] = acc;
           pb = pb + acc * acc;
         }
@@ -858,9 +860,9 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
             ya[u] = L.y[i + u];
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+          for (int u = 0; u < 8; ++u) acc = __builtin_fma(ya[u], qa[u], acc);
         }
-        for (; i < m; ++i) acc = acc + L.y[i] * Q[(size_t)i * n + r];
+        for (; i < m; ++i) acc = __builtin_fma(L.y[i], Q[(size_t)i * n + r], acc);
         V[t] = acc;
         pn = dd_add(pn, two_prod(acc, acc));
       }
@@ -915,7 +917,7 @@ __device__ __forceinline__ double matvec_row(const double* a, int lda, int n, in
       aa[u] = rowp[c0 + u];
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) S[u & 3] = S[u & 3] + aa[u] * vv[u];
+    for (int u = 0; u < 8; ++u) S[u & 3] = __builtin_fma(aa[u], vv[u], S[u & 3]);
   }
   if (c0 + 4 <= n) {
     double vv[4], aa[4];
@@ -925,12 +927,12 @@ __device__ __forceinline__ double matvec_row(const double* a, int lda, int n, in
       aa[u] = rowp[c0 + u];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) S[u] = S[u] + aa[u] * vv[u];
+    for (int u = 0; u < 4; ++u) S[u] = __builtin_fma(aa[u], vv[u], S[u]);
     c0 += 4;
   }
 #pragma unroll
   for (int u = 0; u < 3; ++u)
-    if (c0 + u < n) S[u] = S[u] + rowp[c0 + u] * v[c0 + u];
+    if (c0 + u < n) S[u] = __builtin_fma(rowp[c0 + u], v[c0 + u], S[u]);
   return (S[0] + S[1]) + (S[2] + S[3]);
 }
 
@@ -977,10 +979,10 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
     double sv = 0.0;
     if (own) {
       sv = av;
-      if (j > 0) sv = sv - bprev * QP;
+      if (j > 0) sv = __builtin_fma(-bprev, QP, sv);
     }
     const double alpha = rows_sum(L, own ? QC * sv : 0.0, nw, 0);
-    W = own ? sv - alpha * QC : 0.0;
+    W = own ? __builtin_fma(-alpha, QC, sv) : 0.0;
     if (own) L.wv[r] = W;
     __syncthreads();
     // Gram-Schmidt coefficients h_i = q_i.w: lane 4i' + g forms S_g over
@@ -1002,10 +1004,10 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
             wa[u] = wp[b + 4 * u];
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) S = S + qa[u] * wa[u];
+          for (int u = 0; u < 8; ++u) S = __builtin_fma(qa[u], wa[u], S);
         }
         for (; b < n; b += 4)
-          if (b + g < n) S = S + qi[b] * wp[b];
+          if (b + g < n) S = __builtin_fma(qi[b], wp[b], S);
         const double s01 = S + lane_xor<1>(S);
         const double h = s01 + lane_xor<2>(s01);
         if (g == 0 && i <= j) L.hbuf[i] = h;
@@ -1025,9 +1027,9 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
           ha[u] = L.hbuf[i + u];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc - ha[u] * qa[u];
+        for (int u = 0; u < 8; ++u) acc = __builtin_fma(-ha[u], qa[u], acc);
       }
-      for (; i <= j; ++i) acc = acc - L.hbuf[i] * qp[i * n];
+      for (; i <= j; ++i) acc = __builtin_fma(-L.hbuf[i], qp[i * n], acc);
       if (own) W = acc;
     }
     const double beta = __builtin_sqrt(rows_sum(L, own ? W * W : 0.0, nw, 1));
@@ -1100,9 +1102,9 @@ __device__ void lanczos1(EigLds& L, const double* A, int n, int first_check, dou
         ya[u] = L.y[i + u];
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc = acc + ya[u] * qa[u];
+      for (int u = 0; u < 8; ++u) acc = __builtin_fma(ya[u], qa[u], acc);
     }
-    for (; i < m; ++i) acc = acc + L.y[i] * qp[i * n];
+    for (; i < m; ++i) acc = __builtin_fma(L.y[i], qp[i * n], acc);
     if (own) {
       V = acc;
       tp = two_prod(acc, acc);

@@ -866,7 +866,7 @@ void oc_eig_trace(int k, const double* Hs, const double* x, const double* z, con
   for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
   for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
-  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c) * x[c]; });
+  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
   for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
   dd xx{s[0], s[1]};
   dd xz = fixed_to_dd(to_fixed(s[4]) + to_fixed(s[5]));

@@ -15,8 +15,12 @@
 //        sum over the rows);
 //        beta_j = |w|;  q_{j+1} = w / beta_j,
 //      every matrix-vector row sum taken as four interleaved partial sums,
-//      c = j mod 4, combined as (S0 + S1) + (S2 + S3) (dot4), every dot
-//      product in the canonical 64-lane order (red64);
+//      c = j mod 4, each a chain of fused multiply-adds, combined as
+//      (S0 + S1) + (S2 + S3) (dot4), every other dot product in the
+//      canonical 64-lane order (red64); the vector updates w - beta q,
+//      w - alpha q, w - sum h_i q_i and the Ritz vector sum_i y_i q_i are
+//      fused multiply-adds too (std::fma here, the device's fma: both
+//      correctly rounded, so they give one result);
 //   2. at scheduled steps (from one less than the previous solve's
 //      count, then every second step) the lowest eigenvalue theta of the
 //      Lanczos matrix T_m by multisection on Sturm counts (128 points per
@@ -71,14 +75,15 @@ struct EigPair {
   double alpha0 = 0.0;  // first Lanczos coefficient (diagnostics)
 };
 
-// Row sum in the canonical four-way order: S_j = sum over c = j mod 4 in
-// ascending c, combined as (S0 + S1) + (S2 + S3).
-template <class F>
-static inline double dot4(int m, F term) {
+// Row sum of a(c) b(c) in the canonical four-way order: S_j = the fused
+// multiply-add chain over c = j mod 4 in ascending c (from 0.0), combined as
+// (S0 + S1) + (S2 + S3).
+template <class FA, class FB>
+static inline double dot4(int m, FA a, FB b) {
   double S[4];
   for (int j = 0; j < 4; ++j) {
     double acc = 0.0;
-    for (int c = j; c < m; c += 4) acc = acc + term(c);
+    for (int c = j; c < m; c += 4) acc = std::fma(a(c), b(c), acc);
     S[j] = acc;
   }
   return (S[0] + S[1]) + (S[2] + S[3]);
@@ -296,18 +301,19 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
   int m = 0;
   for (int j = 0; j < n; ++j) {
     const double* qj = &Q[(size_t)j * n];
-    for (int r = 0; r < n; ++r) w[r] = dot4(n, [&](int c) { return A(r, c) * qj[c]; });
+    for (int r = 0; r < n; ++r) w[r] = dot4(n, [&](int c) { return A(r, c); }, [&](int c) { return qj[c]; });
     if (j > 0)
-      for (int r = 0; r < n; ++r) w[r] = w[r] - beta[j - 1] * Q[(size_t)(j - 1) * n + r];
+      for (int r = 0; r < n; ++r) w[r] = std::fma(-beta[j - 1], Q[(size_t)(j - 1) * n + r], w[r]);
     alpha[j] = red64(n, [&](int i) { return qj[i] * w[i]; });
-    for (int r = 0; r < n; ++r) w[r] = w[r] - alpha[j] * qj[r];
+    for (int r = 0; r < n; ++r) w[r] = std::fma(-alpha[j], qj[r], w[r]);
     // Full reorthogonalisation: one classical Gram-Schmidt pass against
     // q_0..q_j (coefficients as four-way sums over the rows, then
     // subtracted in ascending order).
-    for (int i = 0; i <= j; ++i) h[i] = dot4(n, [&](int r) { return Q[(size_t)i * n + r] * w[r]; });
+    for (int i = 0; i <= j; ++i)
+      h[i] = dot4(n, [&](int r) { return Q[(size_t)i * n + r]; }, [&](int r) { return w[r]; });
     for (int r = 0; r < n; ++r) {
       double acc = w[r];
-      for (int i = 0; i <= j; ++i) acc = acc - h[i] * Q[(size_t)i * n + r];
+      for (int i = 0; i <= j; ++i) acc = std::fma(-h[i], Q[(size_t)i * n + r], acc);
       w[r] = acc;
     }
     beta[j] = std::sqrt(red64(n, [&](int i) { return w[i] * w[i]; }));
@@ -342,7 +348,7 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
   std::vector<double> v(n);
   for (int r = 0; r < n; ++r) {
     double acc = 0.0;
-    for (int i = 0; i < m; ++i) acc = acc + y[i] * Q[(size_t)i * n + r];
+    for (int i = 0; i < m; ++i) acc = std::fma(y[i], Q[(size_t)i * n + r], acc);
     v[r] = acc;
   }
   const dd nrm = dd_sqrt(red64_dd(n, [&](int i) { return two_prod(v[i], v[i]); }));
@@ -406,7 +412,7 @@ static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st)
       S.ev[i] = S.ep.q[i].hi;
       S.dl[i] = S.ep.q[i].lo;
     }
-    for (int r = 0; r < n; ++r) S.Ae[r] = dot4(n, [&](int c) { return A(r, c) * S.ev[c]; });
+    for (int r = 0; r < n; ++r) S.Ae[r] = dot4(n, [&](int c) { return A(r, c); }, [&](int c) { return S.ev[c]; });
     dd t1 = red64_dd(n, [&](int i) { return two_prod(S.ev[i], S.Ae[i]); });
     dd t2 = red64_dd(n, [&](int i) { return two_prod(S.dl[i], S.Ae[i]); });
     S.rq = dd_add(t1, dd_mul_d(t2, 2.0));
@@ -441,7 +447,7 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
   for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
   for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
-  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c) * x[c]; });
+  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
   for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
   double yy = dd_add_d(xx, -xTx).hi;
   double y1 = std::sqrt(yy);
