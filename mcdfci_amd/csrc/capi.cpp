@@ -26,12 +26,15 @@ using namespace mc;
 // Scratch of mcdfci_eig_update_noalloc, the subspace update that calls no
 // device allocator: one copy per GPU (each device holds its own at its own
 // address), kept until the process ends, sized for k <= kMaxK:
-//   [DevState, 32 doubles] [H_sub k*k] [x k] [z k] [dx k]  (packed by k).
+//   [DevState, 32 doubles] [H_sub k*k] [x k] [z k] [dx k]  (packed by k)
+//   and, at a fixed place after the largest of those, the subspace solver's
+//   exchange buffer (kXchgWords words; zero at load like the whole array).
 // Host code never passes `scratch` itself (a host-side stand-in) to a
 // kernel or a copy: it uses the per-device address from
 // hipGetSymbolAddress, taken once per device at mcdfci_create.
 constexpr size_t kNoAllocState = 32;
-constexpr size_t kNoAllocDoubles = kNoAllocState + (size_t)kMaxK * kMaxK + 3 * (size_t)kMaxK;
+constexpr size_t kNoAllocXchg = kNoAllocState + (size_t)kMaxK * kMaxK + 3 * (size_t)kMaxK;
+constexpr size_t kNoAllocDoubles = kNoAllocXchg + (size_t)kXchgWords;
 static_assert(sizeof(DevState) <= kNoAllocState * sizeof(double), "DevState outgrows its scratch slot");
 static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
@@ -110,6 +113,7 @@ struct mcdfci_ctx {
   long long* ctr = nullptr;  // striped iteration counters
   uint32_t* eslot = nullptr;  // per column element: the slot where the scatter's probe ended
   uint32_t* znorm = nullptr;  // slots whose z word B pass 2 found near its range (kZNormCap)
+  uint64_t* xbuf = nullptr;   // exchange buffer of the subspace solver's workgroups (kXchgWords)
   int znorm_bits = 61;        // fold threshold (MCDFCI_ZNORM_BITS lowers it to exercise the fold)
   size_t iter_bytes = 0;
   // launch choices (result-neutral; set by mcdfci_autotune)
@@ -430,6 +434,8 @@ int check_state_error(mcdfci_ctx* c, const DevState& s) {
       return fail(c, MCDFCI_E_TABLE_FULL, "The hash table has no free slot left.");
     case ERR_CAP:
       return fail(c, MCDFCI_E_UNSUPPORTED, "candidate list exceeds the device sort capacity.");
+    case ERR_SYNC:
+      return fail(c, MCDFCI_E_HIP, "the subspace solver's workgroups lost an exchanged value (no reply in time).");
     default:
       return fail(c, MCDFCI_E_HIP, "device error " + std::to_string(s.error));
   }
@@ -469,7 +475,7 @@ void launch_iteration(mcdfci_ctx* c) {
   if (c->opts.coordinate_update == MCDFCI_UPDATE_LS)
     launch_ls(s, c->st, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot);
   else
-    launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot, 0);
+    launch_eig(s, c->st, k, k, c->Hs, c->px, c->pz, c->dx, tab_x(c), slot_doubles(c->nw), c->pslot, 0, c->xbuf);
   if (c->timing) (void)hipEventRecord(c->ev[3], s);
   launch_update_z(c, k, 1);
   if (c->timing) {
@@ -587,6 +593,9 @@ int create_ctx(SpatialHam&& ham_in, const mcdfci_opts* opts, mcdfci_ctx** out) {
   }
   if (!rc) rc = dalloc(c, &c->st, 1);
   if (!rc) rc = dalloc(c, &c->st_scratch, 1);
+  if (!rc) rc = dalloc(c, &c->xbuf, kXchgWords);
+  if (!rc && hipMemset(c->xbuf, 0, sizeof(uint64_t) * kXchgWords) != hipSuccess)
+    rc = fail(c, MCDFCI_E_HIP, "hipMemset of the exchange buffer failed");
   if (!rc) {
     // Address of this device's `scratch`, looked up once per device (the
     // device is current here).
@@ -1271,7 +1280,7 @@ int mcdfci_eig_update(mcdfci_ctx* c, int k, const double* Hin, const double* x, 
   st.xz_hi = (int64_t)(xz >> 64);
   st.xz_lo = (uint64_t)xz;
   HIPCHECK(c, hipMemcpy(c->st_scratch, &st, sizeof(st), hipMemcpyHostToDevice));
-  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, nullptr, 0, nullptr, 1);
+  launch_eig(c->stream, c->st_scratch, k, k, h, dxx, dz, ddx, nullptr, 0, nullptr, 1, c->xbuf);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   HIPCHECK(c, hipMemcpy(&st, c->st_scratch, sizeof(st), hipMemcpyDeviceToHost));
@@ -1316,7 +1325,7 @@ int mcdfci_eig_update_noalloc(mcdfci_ctx* c, int k, const double* Hin, const dou
   HIPCHECK(c, hipMemcpyAsync(zd, z, 8 * (size_t)k, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_mirror_upper, dim3((unsigned)std::min(64, (k * k + 255) / 256)), dim3(256), 0, c->stream, h,
                      k);
-  launch_eig(c->stream, sd, k, k, h, xd, zd, dxd, nullptr, 0, nullptr, 1);
+  launch_eig(c->stream, sd, k, k, h, xd, zd, dxd, nullptr, 0, nullptr, 1, (uint64_t*)(c->noalloc + kNoAllocXchg));
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipMemcpyAsync(&st, sd, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipMemcpyAsync(dxo, dxd, 8 * (size_t)k, hipMemcpyDeviceToHost, c->stream));

@@ -115,7 +115,8 @@ struct DevState {
 
 constexpr unsigned int kZNormCap = 1 << 16;  // slots the fold list holds per iteration
 
-enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9 };
+enum { ERR_NONE = 0, ERR_TABLE_FULL = 3, ERR_NUMERIC = 5, ERR_CANDIDATES = 6, ERR_PROBE = 8, ERR_CAP = 9,
+       ERR_SYNC = 10 };
 
 struct StageEnt {
   uint64_t val;   // bits of |x*xss+z| (non-negative double)

@@ -57,6 +57,58 @@ constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accept
 constexpr double kLzPerturb = 0x1p-16;
 constexpr int kLzAttempts = 6;
 
+// ---------------------------------------------------------------- worker exchange
+// Above n = kEigLdsN the products A v run on worker workgroups (blocks 1..W
+// of k_eig), each holding kWRows rows of H_sub in LDS, while block 0 runs the
+// solve: it sends each vector and gathers the row sums.  A word crosses
+// workgroups as a relaxed device-scope atomic carrying its round's tag in its
+// upper half ((tag << 32) | one half of a double); the reader polls each word
+// until its tag arrives, so no fence orders the words.  Exchange buffer
+// (kXchgWords 64-bit words, zeroed when allocated):
+//   [0]                          launch counter (block 0 advances it at the end)
+//   cmd[p],   p = round & 1      header (tag << 32 | command << 16 | entries),
+//                                then two words per vector entry
+//   reply[p]                     two words per row sum
+//   border                       two words per b_h (the bordered matrix's
+//                                first column), once per launch
+// Round r of a launch with counter c has tag c * 1024 + r + 2 (the border c *
+// 1024 + 1), so a word left from an earlier round never matches.  A slot is
+// rewritten two rounds later only after every block has read it: a block
+// sends round r + 2 after reading round r + 1, which every other block
+// answered after reading round r.
+constexpr int kWRows = 16;       // H rows per worker (one wave: 16 rows x 4 column classes)
+constexpr int kXHead = 64;
+constexpr int kXCmd = 544;       // 1 + 2 * kNMax, padded
+constexpr int kXRep = 544;       // 2 * kNMax, padded
+constexpr int kXPoll = 1 << 22;  // polls before a word counts as lost
+static_assert(kXHead + 2 * kXCmd + 2 * kXRep + 2 * kMaxK == kXchgWords, "exchange buffer layout");
+static_assert(1 + 2 * (kMaxK + 1) <= kXCmd && 2 * (kMaxK + 1) <= kXRep, "exchange slots");
+enum { XCMD_STOP = 0, XCMD_H = 1, XCMD_A = 2 };
+
+__device__ __forceinline__ uint32_t xtag(uint32_t counter, int round) { return counter * 1024u + (uint32_t)round + 2u; }
+__device__ __forceinline__ uint32_t xtag_border(uint32_t counter) { return counter * 1024u + 1u; }
+
+__device__ __forceinline__ void xput(uint64_t* p, double v, uint32_t tag) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v), t = (uint64_t)tag << 32;
+  __hip_atomic_store(p, t | (b & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The double at p once both halves carry tag (bounded: bad = 1 when a half
+// does not arrive within kXPoll polls).
+__device__ __forceinline__ double xget(uint64_t* p, uint32_t tag, int& bad) {
+  uint64_t lo, hi;
+  int n = 0;
+  do {
+    lo = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    hi = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } while (((uint32_t)(lo >> 32) != tag || (uint32_t)(hi >> 32) != tag) && ++n < kXPoll);
+  if (n >= kXPoll) bad = 1;
+  return __longlong_as_double((long long)((lo & 0xffffffffu) | (hi << 32)));
+}
+
+__device__ __forceinline__ int eig_workers(int k) { return (k + kWRows - 1) / kWRows; }
+
 // Lanczos vectors the LDS holds for dimension n; a solve that reaches this
 // many stops and restarts from its Ritz vector.
 __device__ __forceinline__ int lz_capacity(int n) {
@@ -140,6 +192,7 @@ __device__ __forceinline__ double dot4_combine(double (*part)[kNMax], int r) {
 
 // Subspace matrix entries A(r, c).
 struct MatLds {  // symmetric, column-major in LDS
+  static constexpr bool kSums = false;
   static constexpr int kBatch = 6;  // columns whose loads mat_partials issues together
   const double* a;
   int n;
@@ -195,6 +248,7 @@ __device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (
 }
 
 struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
+  static constexpr bool kSums = false;
   const double* hs;
   int ldh;
   const double* b0;
@@ -271,6 +325,131 @@ __device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, doubl
   for (int t = 0; t < kRowsW; ++t) {
     const int r = lane + 64 * (hb + nhb * t);
     if (r < n) part[w][r] = acc[t];
+  }
+}
+
+// [a00 b^T; b H] (o = 1) or H (o = 0) with the products on the workers:
+// dist_mv gives whole row sums (kSums) instead of class partials.
+struct MatDist {
+  static constexpr bool kSums = true;
+  uint64_t* xb;
+  uint32_t counter;
+  int* round;        // LDS: the next exchange round (block 0's threads agree on it)
+  int* bad;          // LDS: an exchanged word was lost
+  int o;
+  double a00;
+  const double* b0;  // LDS
+};
+
+// out[r] = sum_c A(r, c) v[c] for r < n (v has nv = n entries; with o = 1
+// the vector is in the bordered matrix's indices), each row in the canonical
+// order: four fused multiply-add chains over the column classes c mod 4,
+// combined (S0 + S1) + (S2 + S3).  Rows of H come from the workers; row 0
+// of the bordered matrix is formed here by wave 0 while they work.  Every
+// thread of block 0 calls it; it starts and ends with a barrier.
+__device__ void dist_mv(const MatDist& D, int n, const double* v, double* out) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  __syncthreads();
+  if (*D.bad) {
+    // an earlier exchange failed: no more rounds (the solve then breaks
+    // down at once and the kernel reports the failure)
+    for (int r = tid; r < n; r += bs) out[r] = 0.0;
+    __syncthreads();
+    return;
+  }
+  const int rnd = *D.round;
+  const uint32_t tag = xtag(D.counter, rnd);
+  const int cmd = D.o ? XCMD_A : XCMD_H;
+  uint64_t* cw = D.xb + kXHead + (rnd & 1) * kXCmd;
+  uint64_t* rw = D.xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
+  for (int i = tid; i < n; i += bs) xput(cw + 1 + 2 * i, v[i], tag);
+  if (tid == 0)
+    __hip_atomic_store(cw, ((uint64_t)tag << 32) | ((uint64_t)cmd << 16) | (uint64_t)n, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  if (D.o && tid < 64) {
+    // row 0: [a00, b]; lanes 0..3 form the class chains, the quad combines
+    const int g = tid & 3;
+    double S = 0.0;
+    if (tid < 4)
+      for (int c = g; c < n; c += 4) S = __builtin_fma(c == 0 ? D.a00 : D.b0[c - 1], v[c], S);
+    const double s01 = S + lane_xor<1>(S);
+    const double s = s01 + lane_xor<2>(s01);
+    if (tid == 0) out[0] = s;
+  }
+  int bad = 0;
+  for (int r = D.o + tid; r < n; r += bs) out[r] = xget(rw + 2 * r, tag, bad);
+  if (bad) *D.bad = 1;
+  __syncthreads();
+  if (tid == 0) *D.round = rnd + 1;
+}
+
+// Blocks 1..W of k_eig: rows h0 .. h0 + kWRows - 1 of H (rows of the
+// symmetric H_sub are its columns) in LDS; answer each command with the row
+// sums of H v (XCMD_H) or of the bordered matrix's rows h + 1 (XCMD_A: the
+// class-0 chain starts with b_h v_0, H(h, c - 1) pairs with v_c), until
+// XCMD_STOP.  A lost word ends the loop (block 0 then finds its own polls
+// failing and reports the error).
+__device__ void eig_worker(uint64_t* xb, int k, int ldh, const double* Hs, double* lds, int* sh) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int h0 = (blockIdx.x - 1) * kWRows;
+  const int nr = k - h0 < kWRows ? k - h0 : kWRows;
+  double* Hl = lds;                   // [kWRows][kMaxK]
+  double* v = Hl + kWRows * kMaxK;    // [kNMax]
+  double* bl = v + kNMax;             // [kWRows]
+  const uint32_t counter = (uint32_t)__hip_atomic_load(xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = tid; i < nr * k; i += bs) {
+    const int r = i / k, c = i - r * k;
+    Hl[r * kMaxK + c] = Hs[(size_t)(h0 + r) * ldh + c];
+  }
+  int have_b = 0;
+  for (int rnd = 0;; ++rnd) {
+    const uint32_t tag = xtag(counter, rnd);
+    uint64_t* cw = xb + kXHead + (rnd & 1) * kXCmd;
+    uint64_t* rw = xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
+    if (tid == 0) {
+      uint64_t hd;
+      int p = 0;
+      do {
+        hd = __hip_atomic_load(cw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } while ((uint32_t)(hd >> 32) != tag && ++p < kXPoll);
+      sh[0] = (uint32_t)(hd >> 32) == tag ? (int)((hd >> 16) & 0xffff) : XCMD_STOP;
+      sh[1] = (int)(hd & 0xffff);
+      sh[2] = 0;
+    }
+    __syncthreads();
+    const int cmd = sh[0], nv = sh[1];
+    if (cmd == XCMD_STOP) break;
+    int bad = 0;
+    for (int i = tid; i < nv; i += bs) v[i] = xget(cw + 1 + 2 * i, tag, bad);
+    if (cmd == XCMD_A && !have_b) {
+      uint64_t* bw = xb + kXHead + 2 * kXCmd + 2 * kXRep;
+      for (int i = tid; i < nr; i += bs) bl[i] = xget(bw + 2 * (h0 + i), xtag_border(counter), bad);
+      have_b = 1;
+    }
+    if (bad) sh[2] = 1;
+    __syncthreads();
+    if (sh[2]) break;
+    if (tid < 64) {  // wave 0: lane 4 i + g forms class g of row h0 + i
+      const int i = tid >> 2, g = tid & 3;
+      double S = 0.0;
+      if (i < nr) {
+        const double* hr = Hl + i * kMaxK;
+        if (cmd == XCMD_H) {
+          for (int c = g; c < nv; c += 4) S = __builtin_fma(hr[c], v[c], S);
+        } else {
+          int c = g;
+          if (g == 0) {
+            S = __builtin_fma(bl[i], v[0], S);
+            c = 4;
+          }
+          for (; c < nv; c += 4) S = __builtin_fma(hr[c - 1], v[c], S);
+        }
+      }
+      const double s01 = S + lane_xor<1>(S);
+      const double sum = s01 + lane_xor<2>(s01);
+      if (g == 0 && i < nr) xput(rw + 2 * (h0 + i + (cmd == XCMD_A ? 1 : 0)), sum, tag);
+    }
+    __syncthreads();
   }
 }
 
@@ -691,8 +870,12 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
   double hint = -__builtin_huge_val(), prev = 0.0;  // warm start of the next check (every thread)
   int m = 0;
   for (int j = 0; j < n; ++j) {
-    mat_partials(A, n, [&](int i) { return L.qw[w & 3][i]; }, L.part[j & 1]);
-    __syncthreads();
+    if constexpr (Mat::kSums) {
+      dist_mv(A, n, Q + (size_t)j * n, L.part[j & 1][0]);
+    } else {
+      mat_partials(A, n, [&](int i) { return L.qw[w & 3][i]; }, L.part[j & 1]);
+      __syncthreads();
+    }
     double alpha = 0.0;
     if (cw) {
       double(*P)[kNMax] = L.part[j & 1];
@@ -702,7 +885,7 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
         const int r = lane + 64 * t;
         W[t] = 0.0;
         if (r < n) {
-          double s = (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
+          double s = Mat::kSums ? P[0][r] : (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
           if (j > 0) s = __builtin_fma(-bprev, QP[t], s);
           W[t] = s;
           pa = pa + QC[t] * s;
@@ -1237,11 +1420,20 @@ __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n
 // table access).  Hs holds both triangles of H_sub.
 __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ldh, const double* Hs,
                                              const double* px, const double* pz, double* dx, double* tabx,
-                                             int slot_stride, const uint32_t* pslot, int mode) {
+                                             int slot_stride, const uint32_t* pslot, int mode, uint64_t* xb) {
   __shared__ EigLds L;
   __shared__ double Qlds[kQBudget];  // A (n <= kEigLdsN), then the Lanczos basis
   if (st->error) return;
+  if (blockIdx.x > 0) {
+    eig_worker(xb, k, ldh, Hs, Qlds, L.iv);
+    return;
+  }
   const int tid = threadIdx.x, bs = blockDim.x;
+  const uint32_t xcount = (uint32_t)__hip_atomic_load(xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    L.iv[6] = 0;  // exchange round
+    L.iv[7] = 0;  // a word was lost
+  }
   for (int i = tid; i < k; i += bs) {
     L.x[i] = px[i];
     L.z[i] = pz[i];
@@ -1261,11 +1453,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   const int n = normal ? k + 1 : k;
   const int o = normal ? 1 : 0;
   const bool in_lds = n <= kEigLdsN;
-  // H x (H read from Hs in place, both triangles).
-  mat_partials(MatSub{Hs, ldh, nullptr, 0.0, 0}, k, [&](int i) { return L.x[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
-  __syncthreads();
-  for (int r = tid; r < k; r += bs) L.Hx[r] = dot4_combine(L.part[0], r);
-  __syncthreads();
+  // H x on the workers
+  dist_mv(MatDist{xb, xcount, &L.iv[6], &L.iv[7], 0, 0.0, nullptr}, k, L.x, L.Hx);
   if (normal) {
     if (tid == 0) {
       double xTz = 0.0, xTHx = 0.0;
@@ -1277,8 +1466,10 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
       L.v0[0] = y1;
     }
     for (int i = tid; i < k; i += bs) {
-      L.b0[i] = (L.z[i] - L.Hx[i]) / y1;
+      const double b = (L.z[i] - L.Hx[i]) / y1;
+      L.b0[i] = b;
       L.v0[i + 1] = L.x[i];
+      xput(xb + kXHead + 2 * kXCmd + 2 * kXRep + 2 * i, b, xtag_border(xcount));
     }
   } else {
     // x is the reference determinant alone, which a symmetry can hide from
@@ -1287,6 +1478,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   }
   __syncthreads();
   const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
+  const MatDist Md{xb, xcount, &L.iv[6], &L.iv[7], o, normal ? L.sc[6] : 0.0, L.b0};
   if (in_lds) {
     // Eight loads in flight per thread (H_sub comes from another kernel's
     // writes, so each load waits on L2 or the last-level cache).
@@ -1319,18 +1511,19 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     if (in_lds)
       lanczos_lowest(L, Ml, n, first, perturb, Qb);
     else
-      lanczos_lowest(L, Ms, n, first, perturb, Qb);
+      lanczos_lowest(L, Md, n, first, perturb, Qb);
     for (int i = tid; i < n; i += bs) {
       L.ev[i] = L.q[i].hi;
       L.dl[i] = L.q[i].lo;
     }
     __syncthreads();
-    if (in_lds)
+    if (in_lds) {
       mat_partials(Ml, n, [&](int i) { return L.ev[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
-    else
-      mat_partials(Ms, n, [&](int i) { return L.ev[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
-    __syncthreads();
-    for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L.part[0], r);
+      __syncthreads();
+      for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L.part[0], r);
+    } else {
+      dist_mv(Md, n, L.ev, L.Ae);
+    }
     __syncthreads();
     const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
     const dd t2 = red64_dd(n, [&](int i) { return two_prod(L.dl[i], L.Ae[i]); });
@@ -1352,6 +1545,16 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     first = 4;
     perturb = 0.0;
   }
+  // release the workers, advance the launch counter
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t tag = xtag(xcount, L.iv[6]);
+    __hip_atomic_store(xb + kXHead + (L.iv[6] & 1) * kXCmd, ((uint64_t)tag << 32) | ((uint64_t)XCMD_STOP << 16),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(xb, (uint64_t)(xcount + 1u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (L.iv[7]) st->error = ERR_SYNC;
+  }
+  if (L.iv[7]) return;
   if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
   eig_finish(st, L, k, n, normal, y1, rq, res2, dx, tabx, slot_stride, pslot, mode);
 }
@@ -1553,16 +1756,18 @@ void launch_ls(hipStream_t s, DevState* st, const double* Hs, const double* px, 
 }
 
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
-                const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode) {
+                const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode,
+                uint64_t* xb) {
   // Up to n = k + 1 = kEigLdsN the row-owner form, one wave per 64 rows;
   // above, the class form on eight waves (mat_partials assumes two row
-  // blocks per class there).
+  // blocks per class there) with the products on ceil(k / kWRows) worker
+  // workgroups.
   if (k + 1 <= kEigLdsN)
     hipLaunchKernelGGL(k_eig1, dim3(1), dim3(k + 1 <= 64 ? 64 : 128), 0, s, st, k, ldh, Hs, px, pz, dx, tabx,
                        slot_stride, pslot, mode);
   else
-    hipLaunchKernelGGL(k_eig, dim3(1), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px, pz, dx, tabx, slot_stride,
-                       pslot, mode);
+    hipLaunchKernelGGL(k_eig, dim3(1 + (k + kWRows - 1) / kWRows), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px,
+                       pz, dx, tabx, slot_stride, pslot, mode, xb);
 }
 
 }  // namespace mc

@@ -10,8 +10,13 @@ namespace mc {
 
 void launch_submatrix(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2* dets, int k,
                       double* Hs, int ldh);
+// Subspace update.  xb: the exchange buffer of the solver's worker
+// workgroups (kXchgWords words, zeroed once when allocated; used for
+// k >= kEigLdsN).
+constexpr int kXchgWords = 2752;
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
-                const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode);
+                const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode,
+                uint64_t* xb);
 // ept: column elements per thread of the scatter (1, 2, 4, 8 or 16); any value
 // gives the same result.  eslot (k columns of vis_words * 64 entries) gets,
 // per column element, the slot where the element's probe ended -- its
