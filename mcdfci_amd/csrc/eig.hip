@@ -77,6 +77,8 @@ constexpr int kLzAttempts = 6;
 // sends round r + 2 after reading round r + 1, which every other block
 // answered after reading round r.
 constexpr int kWRows = 16;       // H rows per worker (one wave: 16 rows x 4 column classes)
+constexpr int kWLd = kMaxK + 4;  // LDS row stride of a worker's rows: 4 mod 32 doubles spreads the
+                                 // wave's 16 rows x 4 classes over all banks
 constexpr int kXHead = 64;
 constexpr int kXCmd = 544;       // 1 + 2 * kNMax, padded
 constexpr int kXRep = 544;       // 2 * kNMax, padded
@@ -334,6 +336,7 @@ struct MatDist {
   static constexpr bool kSums = true;
   uint64_t* xb;
   uint32_t counter;
+  int nx;            // vector entries sent each round (k + 1; zeros past n)
   int* round;        // LDS: the next exchange round (block 0's threads agree on it)
   int* bad;          // LDS: an exchanged word was lost
   int o;
@@ -362,7 +365,7 @@ __device__ void dist_mv(const MatDist& D, int n, const double* v, double* out) {
   const int cmd = D.o ? XCMD_A : XCMD_H;
   uint64_t* cw = D.xb + kXHead + (rnd & 1) * kXCmd;
   uint64_t* rw = D.xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
-  for (int i = tid; i < n; i += bs) xput(cw + 1 + 2 * i, v[i], tag);
+  for (int i = tid; i < D.nx; i += bs) xput(cw + 1 + 2 * i, i < n ? v[i] : 0.0, tag);
   if (tid == 0)
     __hip_atomic_store(cw, ((uint64_t)tag << 32) | ((uint64_t)cmd << 16) | (uint64_t)n, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
@@ -393,19 +396,22 @@ __device__ void eig_worker(uint64_t* xb, int k, int ldh, const double* Hs, doubl
   const int tid = threadIdx.x, bs = blockDim.x;
   const int h0 = (blockIdx.x - 1) * kWRows;
   const int nr = k - h0 < kWRows ? k - h0 : kWRows;
-  double* Hl = lds;                   // [kWRows][kMaxK]
-  double* v = Hl + kWRows * kMaxK;    // [kNMax]
+  double* Hl = lds;                   // [kWRows][kWLd]
+  double* v = Hl + kWRows * kWLd;     // [kNMax]
   double* bl = v + kNMax;             // [kWRows]
   const uint32_t counter = (uint32_t)__hip_atomic_load(xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int i = tid; i < nr * k; i += bs) {
     const int r = i / k, c = i - r * k;
-    Hl[r * kMaxK + c] = Hs[(size_t)(h0 + r) * ldh + c];
+    Hl[r * kWLd + c] = Hs[(size_t)(h0 + r) * ldh + c];
   }
   int have_b = 0;
   for (int rnd = 0;; ++rnd) {
     const uint32_t tag = xtag(counter, rnd);
     uint64_t* cw = xb + kXHead + (rnd & 1) * kXCmd;
     uint64_t* rw = xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
+    // every round carries k + 1 vector words (the header and the words
+    // are polled together: one round trip)
+    int bad = 0;
     if (tid == 0) {
       uint64_t hd;
       int p = 0;
@@ -416,11 +422,10 @@ __device__ void eig_worker(uint64_t* xb, int k, int ldh, const double* Hs, doubl
       sh[1] = (int)(hd & 0xffff);
       sh[2] = 0;
     }
+    for (int i = tid; i <= k; i += bs) v[i] = xget(cw + 1 + 2 * i, tag, bad);
     __syncthreads();
     const int cmd = sh[0], nv = sh[1];
     if (cmd == XCMD_STOP) break;
-    int bad = 0;
-    for (int i = tid; i < nv; i += bs) v[i] = xget(cw + 1 + 2 * i, tag, bad);
     if (cmd == XCMD_A && !have_b) {
       uint64_t* bw = xb + kXHead + 2 * kXCmd + 2 * kXRep;
       for (int i = tid; i < nr; i += bs) bl[i] = xget(bw + 2 * (h0 + i), xtag_border(counter), bad);
@@ -433,17 +438,26 @@ __device__ void eig_worker(uint64_t* xb, int k, int ldh, const double* Hs, doubl
       const int i = tid >> 2, g = tid & 3;
       double S = 0.0;
       if (i < nr) {
-        const double* hr = Hl + i * kMaxK;
-        if (cmd == XCMD_H) {
-          for (int c = g; c < nv; c += 4) S = __builtin_fma(hr[c], v[c], S);
-        } else {
-          int c = g;
-          if (g == 0) {
-            S = __builtin_fma(bl[i], v[0], S);
-            c = 4;
-          }
-          for (; c < nv; c += 4) S = __builtin_fma(hr[c - 1], v[c], S);
+        // H(h, c - s) pairs with v_c (s = 1 in the bordered indices, where
+        // column 0 is b_h); eight entries loaded per batch
+        const int sft = cmd == XCMD_H ? 0 : 1;
+        const double* hr = Hl + i * kWLd;
+        int c = g;
+        if (sft && g == 0) {
+          S = __builtin_fma(bl[i], v[0], S);
+          c = 4;
         }
+        for (; c + 28 < nv; c += 32) {
+          double a[8], b[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            a[u] = hr[c + 4 * u - sft];
+            b[u] = v[c + 4 * u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) S = __builtin_fma(a[u], b[u], S);
+        }
+        for (; c < nv; c += 4) S = __builtin_fma(hr[c - sft], v[c], S);
       }
       const double s01 = S + lane_xor<1>(S);
       const double sum = s01 + lane_xor<2>(s01);
@@ -1454,7 +1468,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   const int o = normal ? 1 : 0;
   const bool in_lds = n <= kEigLdsN;
   // H x on the workers
-  dist_mv(MatDist{xb, xcount, &L.iv[6], &L.iv[7], 0, 0.0, nullptr}, k, L.x, L.Hx);
+  dist_mv(MatDist{xb, xcount, k + 1, &L.iv[6], &L.iv[7], 0, 0.0, nullptr}, k, L.x, L.Hx);
   if (normal) {
     if (tid == 0) {
       double xTz = 0.0, xTHx = 0.0;
@@ -1478,7 +1492,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   }
   __syncthreads();
   const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
-  const MatDist Md{xb, xcount, &L.iv[6], &L.iv[7], o, normal ? L.sc[6] : 0.0, L.b0};
+  const MatDist Md{xb, xcount, k + 1, &L.iv[6], &L.iv[7], o, normal ? L.sc[6] : 0.0, L.b0};
   if (in_lds) {
     // Eight loads in flight per thread (H_sub comes from another kernel's
     // writes, so each load waits on L2 or the last-level cache).
@@ -1545,12 +1559,19 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     first = 4;
     perturb = 0.0;
   }
-  // release the workers, advance the launch counter
+  // release the workers (a round of zeros with the STOP header), advance
+  // the launch counter
+  __syncthreads();
+  {
+    const uint32_t tag = xtag(xcount, L.iv[6]);
+    uint64_t* cw = xb + kXHead + (L.iv[6] & 1) * kXCmd;
+    for (int i = tid; i <= k; i += bs) xput(cw + 1 + 2 * i, 0.0, tag);
+    if (tid == 0)
+      __hip_atomic_store(cw, ((uint64_t)tag << 32) | ((uint64_t)XCMD_STOP << 16), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
   __syncthreads();
   if (tid == 0) {
-    const uint32_t tag = xtag(xcount, L.iv[6]);
-    __hip_atomic_store(xb + kXHead + (L.iv[6] & 1) * kXCmd, ((uint64_t)tag << 32) | ((uint64_t)XCMD_STOP << 16),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(xb, (uint64_t)(xcount + 1u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (L.iv[7]) st->error = ERR_SYNC;
   }
