@@ -9,10 +9,11 @@ iteration (pick, H_sub, subspace update, column scatter, candidate update).
 Phases:
   0. random-slot access ceiling of the same 64 GB table (insert, update and
      read rates the scatter and pass 2 are measured against).
-  1. throughput: the mixed+batched form (k = 64 coordinates per iteration,
-     FP32 integrals and H elements, FP64 z / energy / eigensolve): launch
-     forms autotuned on real iterations, iterations replayed from HIP graphs
-     of G captured iterations, W warmup iterations, then exactly K timed
+  1. throughput: the mixed+batched form (k = 256 coordinates per iteration,
+     the fastest of the configuration's sweep k in {8, 64, 256}; FP32
+     integrals and H elements, FP64 z / energy / eigensolve): launch forms
+     autotuned on real iterations, iterations replayed from HIP graphs of G
+     captured iterations, W warmup iterations, then exactly K timed
      iterations; value = k*K / seconds.
   2. roofline: a short segment with HIP events around every kernel on the
      solver stream gives the average k_scatter duration; algorithmic bytes =
@@ -21,7 +22,7 @@ Phases:
   3. sweep: k in {8, 64, 256} in both precisions, same machinery.
   4. time to |E - E_ref| < 1e-5 Ha, three paired repetitions, each running
      from the same start until its table fills: plain FP64 at k = 8, mixed
-     at k = 64, FP64 at k = 64 (batching alone) and mixed at k = 256.
+     at k = 256, FP64 at k = 256 (batching alone) and mixed at k = 64.
      E_ref of a repetition is the plain run's energy at its last report
      before its 64 GB table filled (not a converged energy; its last-report
      change is recorded).  Ratios are medians over the repetitions.  A
@@ -382,7 +383,7 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="c4")
-    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--k", type=int, default=256)
     ap.add_argument("--plain-k", type=int, default=8)
     ap.add_argument("--ttt-seconds", type=float, default=25.0)
     ap.add_argument("--ttt-reps", type=int, default=3)
@@ -394,7 +395,7 @@ def main():
     ap.add_argument("--graph", type=int, default=20, help="iterations per captured HIP graph (0: off)")
     ap.add_argument("--sweep-k", default="8,64,256",
                     help="num_of_coordinate values for the throughput sweep, both precisions (empty: none)")
-    ap.add_argument("--ttt-extra-k", default="256",
+    ap.add_argument("--ttt-extra-k", default="64",
                     help="further mixed num_of_coordinate values timed to the threshold (empty: none)")
     ap.add_argument("--tune-rounds", type=int, default=2, help="autotuner rounds (0: fixed launch forms)")
     ap.add_argument("--footprint-sweep", default="",

@@ -113,6 +113,9 @@ def test_bench_reads_the_committed_traffic_summary():
     t = bench.measured_traffic("c4", 64, "mixed")
     assert isinstance(t, int)
     assert t > 1903168 * 32
+    t = bench.measured_traffic("c4", 256, "mixed")  # the bench's default batch size
+    assert isinstance(t, int)
+    assert t > 7612672 * 32
     assert bench.measured_traffic("c4", 63, "mixed") is None
 
 
