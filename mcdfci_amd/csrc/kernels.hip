@@ -1080,8 +1080,12 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
   __syncthreads();
   if (!last) return;
   __threadfence();
-  // Iteration counters: wave 0 sums the stripes and clears them.
+  // Iteration counters: wave 0 sums the stripes and clears them; the
+  // columns' x.z increments (exact integers) and max |dx| over k <= 256
+  // columns, four per lane of wave 0 (order-free sum and maximum).
   long long c_new = 0, c_upd = 0, c_cols = 0;
+  i128 xzc = 0;
+  double dm = 0.0;
   if (threadIdx.x < 64) {
     long long* cs = ctr + threadIdx.x * kCtrStride;
     c_new = cs[0];
@@ -1090,13 +1094,19 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
     cs[0] = 0;
     cs[1] = 0;
     cs[2] = 0;
+    for (int c = threadIdx.x; c < k; c += 64) {
+      xzc += (((i128)col_xz[2 * c]) << 64) | (i128)(uint64_t)col_xz[2 * c + 1];
+      dm = fmax(dm, __builtin_fabs(dx[c]));
+    }
     c_new = wave_sum_ll(c_new);
     c_upd = wave_sum_ll(c_upd);
     c_cols = wave_sum_ll(c_cols);
+    xzc = wave_sum_i128(xzc);
+    for (int off = 32; off >= 1; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off, 64));
   }
   if (threadIdx.x != 0) return;
   i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
-  for (int c = 0; c < k; ++c) xz += (((i128)col_xz[2 * c]) << 64) | (i128)(uint64_t)col_xz[2 * c + 1];
+  xz += xzc;
   st->xz_hi = (int64_t)(xz >> 64);
   st->xz_lo = (uint64_t)xz;
   st->z_size += c_new;
@@ -1104,8 +1114,6 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
   st->cols_size = c_cols;
   st->ticket = 0;
   st->k_prev = k;
-  double dm = 0.0;
-  for (int c = 0; c < k; ++c) dm = fmax(dm, __builtin_fabs(dx[c]));
   st->dx_max = dm;
   if (count_iteration) st->iterations += 1;
   if ((double)st->z_size > 0.79 * st->max_size) st->error = ERR_TABLE_FULL;
