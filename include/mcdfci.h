@@ -227,7 +227,7 @@ int mcdfci_eig_update(mcdfci_ctx* ctx, int k, const double* H, const double* x, 
  * hipMalloc).  No counterpart in the reference.  Same arguments, results
  * and errors as mcdfci_eig_update for k <= 256; k above 256 returns
  * MCDFCI_E_INPUT.  Calls no hipMalloc / hipFree: H_sub, x, z, dx and the
- * solver state live in a fixed 540 KiB device array that each GPU holds
+ * solver state live in a fixed 528 KiB device array that each GPU holds
  * until the process ends.  Calls that overlap on one GPU (any context)
  * race on that array. */
 int mcdfci_eig_update_noalloc(mcdfci_ctx* ctx, int k, const double* H, const double* x, const double* z,

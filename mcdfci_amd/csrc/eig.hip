@@ -1,5 +1,5 @@
 // Subspace update of the CDFCI iteration (src/solver.h:333-609) and
-// update_x (src/wavefunction.h:425-451) in one workgroup.
+// update_x (src/wavefunction.h:425-451).
 //
 // The (k+1)-dimensional problem A = [alpha b^T; b H_sub] is solved for its
 // lowest eigenpair by Lanczos from the current state (sqrt(yy), x_I),
@@ -14,18 +14,20 @@
 // the Ritz vector are fused multiply-adds.  Two forms compute exactly these sums:
 //  * n <= kEigLdsN (k_eig1, below): A in LDS, one wave per 64 rows, each
 //    lane owning one row;
-//  * larger n (k_eig): A read in place from the symmetric H_sub in HBM/L2
-//    (k_submatrix fills both triangles) with the border alpha, b in LDS;
-//    eight waves, wave w forming the class w mod 4 sums of row block w / 4,
-//    then waves 0-3 combine all rows redundantly, so the dot products and
-//    the next Lanczos vector are known to each of them; the other waves
-//    join the products and the Gram-Schmidt coefficients.
+//  * larger n (k_eig): the rows of A spread over ceil(n / kRRows)
+//    workgroups (kRRows rows each in LDS, from the symmetric H_sub that
+//    k_submatrix stores in both triangles); every product A v is formed by
+//    each workgroup for its rows and all-gathered, and every workgroup runs
+//    the same solve on eight waves (waves 0-3 form every row of the step
+//    redundantly, so the dot products and the next Lanczos vector are known
+//    to each of them; the others join the Gram-Schmidt coefficients);
+//    workgroup 0 writes the results.
 // After the alpha step one classical Gram-Schmidt pass against all previous
 // Lanczos vectors (held in LDS; thread i forms q_i.w as four interleaved
 // sums over the rows) keeps them orthogonal, so a convergence test made
 // late costs steps but not accuracy.  The LDS holds lz_capacity(n) vectors
-// (12288 doubles shared with A when A is resident); a solve reaching that
-// many stops and restarts from its Ritz vector.  At scheduled steps (from
+// (12288 doubles shared with A up to n = kEigLdsN, 8192 above); a solve
+// reaching that many stops and restarts from its Ritz vector.  At scheduled steps (from
 // one less than the previous solve's count, then every second step) wave 0
 // finds the lowest eigenvalue of the Lanczos matrix T_m by multisection on
 // Sturm counts (two counts per lane, 128 points per round, warm-started from
@@ -50,45 +52,39 @@ constexpr int kEigLdsN = 90;    // A lives in LDS up to this dimension
 constexpr int kQBudget = 12288;  // LDS doubles for A (n <= kEigLdsN) and the Lanczos basis
 constexpr int kNMax = kMaxK + 1;
 constexpr int kRows = (kNMax + 63) / 64;  // rows per lane
-constexpr int kEigThreads = 512;          // k_eig: eight waves, two per column class
-constexpr int kRowsW = (kNMax + 127) / 128;  // rows per lane in mat_partials (two row blocks)
+constexpr int kEigThreads = 512;          // k_eig: eight waves per workgroup
 constexpr double kLzTol = 1e-15;
 constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accepts a solve
 constexpr double kLzPerturb = 0x1p-16;
 constexpr int kLzAttempts = 6;
 
-// ---------------------------------------------------------------- worker exchange
-// Above n = kEigLdsN the products A v run on worker workgroups (blocks 1..W
-// of k_eig), each holding kWRows rows of H_sub in LDS, while block 0 runs the
-// solve: it sends each vector and gathers the row sums.  A word crosses
-// workgroups as a relaxed device-scope atomic carrying its round's tag in its
-// upper half ((tag << 32) | one half of a double); the reader polls each word
-// until its tag arrives, so no fence orders the words.  Exchange buffer
-// (kXchgWords 64-bit words, zeroed when allocated):
-//   [0]                          launch counter (block 0 advances it at the end)
-//   cmd[p],   p = round & 1      header (tag << 32 | command << 16 | entries),
-//                                then two words per vector entry
-//   reply[p]                     two words per row sum
-//   border                       two words per b_h (the bordered matrix's
-//                                first column), once per launch
-// Round r of a launch with counter c has tag c * 1024 + r + 2 (the border c *
-// 1024 + 1), so a word left from an earlier round never matches.  A slot is
-// rewritten two rounds later only after every block has read it: a block
-// sends round r + 2 after reading round r + 1, which every other block
-// answered after reading round r.
-constexpr int kWRows = 16;       // H rows per worker (one wave: 16 rows x 4 column classes)
-constexpr int kWLd = kMaxK + 4;  // LDS row stride of a worker's rows: 4 mod 32 doubles spreads the
-                                 // wave's 16 rows x 4 classes over all banks
+// ---------------------------------------------------------------- exchange between workgroups
+// Above n = kEigLdsN the rows of A are spread over the workgroups of k_eig
+// (kRRows per block, in LDS), every block runs the same solve, and each
+// product A v is all-gathered: every block publishes its row sums and reads
+// everybody's.  A word crosses workgroups as a relaxed device-scope atomic
+// carrying its round's tag in its upper half ((tag << 32) | one half of a
+// double); the reader polls each word until its tag arrives, so no fence
+// orders the words (a device-scope fence writes back and invalidates the
+// XCD's L2).  Exchange buffer (kXchgWords 64-bit words, zeroed when
+// allocated): [0] the launch counter (block 0 advances it at the end), then
+// two slots of two words per row, slot p for the rounds r with r & 1 = p.
+// Round r of the launch with counter c carries tag c * 1024 + r + 2, so a
+// word left from an earlier round never matches; a slot is rewritten two
+// rounds later only after every block has read it (a block writes round
+// r + 2 after reading all of round r + 1, which every other block wrote after
+// reading all of round r).
+constexpr int kRRows = 12;        // rows of A per workgroup (wave 0: 12 rows x 4 column classes)
+constexpr int kRLd = kMaxK + 4;   // LDS row stride, 4 mod 32 doubles: the wave's rows x classes
+                                  // spread over all banks
+constexpr int kQBudgetRep = 8192; // LDS doubles of the Lanczos basis above n = kEigLdsN
 constexpr int kXHead = 64;
-constexpr int kXCmd = 544;       // 1 + 2 * kNMax, padded
-constexpr int kXRep = 544;       // 2 * kNMax, padded
-constexpr int kXPoll = 1 << 22;  // polls before a word counts as lost
-static_assert(kXHead + 2 * kXCmd + 2 * kXRep + 2 * kMaxK == kXchgWords, "exchange buffer layout");
-static_assert(1 + 2 * (kMaxK + 1) <= kXCmd && 2 * (kMaxK + 1) <= kXRep, "exchange slots");
-enum { XCMD_STOP = 0, XCMD_H = 1, XCMD_A = 2 };
+constexpr int kXRep = 544;        // 2 * kNMax, padded
+constexpr int kXPoll = 1 << 22;   // polls before a word counts as lost
+static_assert(kXHead + 2 * kXRep <= kXchgWords, "exchange buffer layout");
+static_assert(2 * kNMax <= kXRep, "exchange slots");
 
 __device__ __forceinline__ uint32_t xtag(uint32_t counter, int round) { return counter * 1024u + (uint32_t)round + 2u; }
-__device__ __forceinline__ uint32_t xtag_border(uint32_t counter) { return counter * 1024u + 1u; }
 
 __device__ __forceinline__ void xput(uint64_t* p, double v, uint32_t tag) {
   const uint64_t b = (uint64_t)__double_as_longlong(v), t = (uint64_t)tag << 32;
@@ -109,12 +105,10 @@ __device__ __forceinline__ double xget(uint64_t* p, uint32_t tag, int& bad) {
   return __longlong_as_double((long long)((lo & 0xffffffffu) | (hi << 32)));
 }
 
-__device__ __forceinline__ int eig_workers(int k) { return (k + kWRows - 1) / kWRows; }
-
 // Lanczos vectors the LDS holds for dimension n; a solve that reaches this
 // many stops and restarts from its Ritz vector.
 __device__ __forceinline__ int lz_capacity(int n) {
-  const int avail = kQBudget - (n <= kEigLdsN ? n * n : 0);
+  const int avail = n <= kEigLdsN ? kQBudget - n * n : kQBudgetRep;
   const int c = avail / n;
   return c < n ? c : n;
 }
@@ -177,7 +171,6 @@ struct EigLds {
   double x[kMaxK], z[kMaxK], Hx[kMaxK], b0[kMaxK];
   double v0[kNMax];
   double part[2][4][kNMax];
-  double qw[4][(kNMax + 3) / 4];  // wave w's copy of q_j[4i + w]
   double alpha[kNMax], beta[kNMax], e2[kNMax], hbuf[kNMax], wv[kNMax];
   double dg[kNMax], du[kNMax], du2[kNMax], mul[kNMax], rdg[kNMax], y[kNMax];
   int sw[kNMax];
@@ -188,283 +181,70 @@ struct EigLds {
   int iv[8];
 };
 
-__device__ __forceinline__ double dot4_combine(double (*part)[kNMax], int r) {
-  return (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
-}
-
-// Subspace matrix entries A(r, c).
-struct MatLds {  // symmetric, column-major in LDS
-  static constexpr bool kSums = false;
-  static constexpr int kBatch = 6;  // columns whose loads mat_partials issues together
-  const double* a;
-  int n;
-  __device__ __forceinline__ double operator()(int r, int c) const { return a[c * n + r]; }
-};
-
-// Four-way partial row sums of A v for the class form (k_eig, eight
-// waves, two per class: wave w takes class w mod 4 and the row block w / 4) accumulates the columns c = g mod 4 (ascending) of its rows
-// lane + 64 (h + H t), held in registers, and stores S_g(r) in part[g][r];
-// the class sums are the same whichever wave forms
-// them.  vw(i) returns v[4i + g].
-template <class Mat, class V>
-__device__ __forceinline__ void mat_partials(const Mat& A, int n, V vw, double (*part)[kNMax]) {
-  constexpr int CB = Mat::kBatch;  // columns whose loads are issued together
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int w = wv & 3, hb = wv >> 2, nhb = blockDim.x >> 8;
-  const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
-  // Loads are unconditional (indices clamped into A); values outside the
-  // matrix are never accumulated.
-  int rcl[kRowsW];
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) {
-    const int r = lane + 64 * (hb + nhb * t);
-    rcl[t] = r < n ? r : n - 1;
-  }
-  double acc[kRowsW];
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) acc[t] = 0.0;
-  for (int i0 = 0; i0 < nc; i0 += CB) {
-    double a[CB][kRowsW], v[CB];
-#pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      const int i = i0 + b, c = w + 4 * i;
-      v[b] = i < nc ? vw(i) : 0.0;
-      const int cc = c < n ? c : n - 1;
-#pragma unroll
-      for (int t = 0; t < kRowsW; ++t) a[b][t] = 64 * (hb + nhb * t) < n ? A(rcl[t], cc) : 0.0;
-    }
-#pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      if (i0 + b < nc) {
-#pragma unroll
-        for (int t = 0; t < kRowsW; ++t)
-          if (lane + 64 * (hb + nhb * t) < n) acc[t] = __builtin_fma(a[b][t], v[b], acc[t]);
-      }
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) {
-    const int r = lane + 64 * (hb + nhb * t);
-    if (r < n) part[w][r] = acc[t];
-  }
-}
-
-struct MatSub {  // [a00 b^T; b H] (o = 1) or H (o = 0), H read in place
-  static constexpr bool kSums = false;
-  const double* hs;
-  int ldh;
-  const double* b0;
-  double a00;
-  int o;
-  __device__ __forceinline__ double operator()(int r, int c) const {
-    if (o) {
-      if (c == 0) return r == 0 ? a00 : b0[r - 1];
-      if (r == 0) return b0[c - 1];
-    }
-    return hs[(size_t)(c - o) * ldh + (r - o)];
-  }
-};
-
-// mat_partials for A read in place from global memory: the same sums in
-// the same order, with every global load unconditional (row and column
-// indices clamped into H; out-of-range values are never accumulated) and
-// the border entries (row 0, column 0; LDS) patched in afterwards, so the
-// loads stay plain global loads without per-load masking.
-template <class V>
-__device__ __forceinline__ void mat_partials(const MatSub& A, int n, V vw, double (*part)[kNMax]) {
-  constexpr int CB = 8;  // columns whose loads are issued together
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int w = wv & 3, hb = wv >> 2, nhb = blockDim.x >> 8;
-  const int nc = (n - w + 3) >> 2;  // columns c = w + 4i, i < nc
-  const int o = A.o, kh = n - o;    // H is kh x kh
-  int roff[kRowsW];
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) {
-    int rr = lane + 64 * (hb + nhb * t) - o;
-    rr = rr < 0 ? 0 : (rr > kh - 1 ? kh - 1 : rr);
-    roff[t] = rr;
-  }
-  double acc[kRowsW];
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) acc[t] = 0.0;
-  for (int i0 = 0; i0 < nc; i0 += CB) {
-    double a[CB][kRowsW], v[CB];
-#pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      const int i = i0 + b, c = w + 4 * i;
-      v[b] = i < nc ? vw(i) : 0.0;
-      int cc = c - o;
-      cc = cc < 0 ? 0 : (cc > kh - 1 ? kh - 1 : cc);
-      const double* col = A.hs + (size_t)cc * A.ldh;
-#pragma unroll
-      for (int t = 0; t < kRowsW; ++t) a[b][t] = col[roff[t]];
-    }
-    if (o) {
-#pragma unroll
-      for (int b = 0; b < CB; ++b) {
-        const int c = w + 4 * (i0 + b);
-        if (c == 0) {
-#pragma unroll
-          for (int t = 0; t < kRowsW; ++t) {
-            const int r = lane + 64 * (hb + nhb * t);
-            if (r < n) a[b][t] = r == 0 ? A.a00 : A.b0[r - 1];
-          }
-        } else if (lane == 0 && hb == 0 && c < n) {
-          a[b][0] = A.b0[c - 1];  // row 0: lane 0, block 0, t = 0
-        }
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      if (i0 + b < nc) {
-#pragma unroll
-        for (int t = 0; t < kRowsW; ++t)
-          if (lane + 64 * (hb + nhb * t) < n) acc[t] = __builtin_fma(a[b][t], v[b], acc[t]);
-      }
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < kRowsW; ++t) {
-    const int r = lane + 64 * (hb + nhb * t);
-    if (r < n) part[w][r] = acc[t];
-  }
-}
-
-// [a00 b^T; b H] (o = 1) or H (o = 0) with the products on the workers:
-// dist_mv gives whole row sums (kSums) instead of class partials.
-struct MatDist {
+// The bordered matrix A = [a00 b^T; b H] (or H itself in the first
+// iteration) spread over the workgroups of k_eig: block g holds rows
+// r0 .. r0 + nr - 1 in LDS (row stride kRLd).  Every product A v is formed
+// by each block for its rows and all-gathered (rep_mv), so every block holds
+// every row sum and runs the same solve on the same numbers.
+struct MatRep {
   static constexpr bool kSums = true;
   uint64_t* xb;
   uint32_t counter;
-  int nx;            // vector entries sent each round (k + 1; zeros past n)
-  int* round;        // LDS: the next exchange round (block 0's threads agree on it)
-  int* bad;          // LDS: an exchanged word was lost
-  int o;
-  double a00;
-  const double* b0;  // LDS
+  int* round;          // LDS: the next exchange round (the block's threads agree on it)
+  int* bad;            // LDS: an exchanged word was lost
+  const double* rows;  // LDS: this block's rows
+  int r0, nr;
 };
 
-// out[r] = sum_c A(r, c) v[c] for r < n (v has nv = n entries; with o = 1
-// the vector is in the bordered matrix's indices), each row in the canonical
-// order: four fused multiply-add chains over the column classes c mod 4,
-// combined (S0 + S1) + (S2 + S3).  Rows of H come from the workers; row 0
-// of the bordered matrix is formed here by wave 0 while they work.  Every
-// thread of block 0 calls it; it starts and ends with a barrier.
-__device__ void dist_mv(const MatDist& D, int n, const double* v, double* out) {
+// out[r] = sum_c rows(r, c + sft) v[c], c < nv, for every row r < nout of the
+// matrix (sft = 1: the H part of the bordered rows, whose H row r - 1 the
+// block's row r holds from column 1; the output index is then r - 1), each
+// row in the canonical order: four fused multiply-add chains over the column
+// classes c mod 4, combined (S0 + S1) + (S2 + S3).  Lane 4 i + g of wave 0
+// forms class g of the block's row i; the row sums are exchanged through
+// the reply slots of the exchange buffer.  Every thread of every block calls
+// it with the same arguments; it starts and ends with a barrier.
+__device__ void rep_mv(const MatRep& D, int nout, int nv, const double* v, int sft, double* out) {
   const int tid = threadIdx.x, bs = blockDim.x;
   __syncthreads();
   if (*D.bad) {
     // an earlier exchange failed: no more rounds (the solve then breaks
     // down at once and the kernel reports the failure)
-    for (int r = tid; r < n; r += bs) out[r] = 0.0;
+    for (int r = tid; r < nout; r += bs) out[r] = 0.0;
     __syncthreads();
     return;
   }
   const int rnd = *D.round;
   const uint32_t tag = xtag(D.counter, rnd);
-  const int cmd = D.o ? XCMD_A : XCMD_H;
-  uint64_t* cw = D.xb + kXHead + (rnd & 1) * kXCmd;
-  uint64_t* rw = D.xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
-  for (int i = tid; i < D.nx; i += bs) xput(cw + 1 + 2 * i, i < n ? v[i] : 0.0, tag);
-  if (tid == 0)
-    __hip_atomic_store(cw, ((uint64_t)tag << 32) | ((uint64_t)cmd << 16) | (uint64_t)n, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  if (D.o && tid < 64) {
-    // row 0: [a00, b]; lanes 0..3 form the class chains, the quad combines
-    const int g = tid & 3;
+  uint64_t* rw = D.xb + kXHead + (rnd & 1) * kXRep;
+  if (tid < 64) {
+    const int i = tid >> 2, g = tid & 3;
+    const bool own = i < D.nr && D.r0 + i >= sft;
     double S = 0.0;
-    if (tid < 4)
-      for (int c = g; c < n; c += 4) S = __builtin_fma(c == 0 ? D.a00 : D.b0[c - 1], v[c], S);
+    if (own) {
+      const double* ar = D.rows + i * kRLd + sft;
+      int c = g;
+      for (; c + 28 < nv; c += 32) {
+        double a[8], x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          a[u] = ar[c + 4 * u];
+          x[u] = v[c + 4 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) S = __builtin_fma(a[u], x[u], S);
+      }
+      for (; c < nv; c += 4) S = __builtin_fma(ar[c], v[c], S);
+    }
     const double s01 = S + lane_xor<1>(S);
-    const double s = s01 + lane_xor<2>(s01);
-    if (tid == 0) out[0] = s;
+    const double sum = s01 + lane_xor<2>(s01);
+    if (g == 0 && own) xput(rw + 2 * (D.r0 + i - sft), sum, tag);
   }
   int bad = 0;
-  for (int r = D.o + tid; r < n; r += bs) out[r] = xget(rw + 2 * r, tag, bad);
+  for (int r = tid; r < nout; r += bs) out[r] = xget(rw + 2 * r, tag, bad);
   if (bad) *D.bad = 1;
   __syncthreads();
   if (tid == 0) *D.round = rnd + 1;
-}
-
-// Blocks 1..W of k_eig: rows h0 .. h0 + kWRows - 1 of H (rows of the
-// symmetric H_sub are its columns) in LDS; answer each command with the row
-// sums of H v (XCMD_H) or of the bordered matrix's rows h + 1 (XCMD_A: the
-// class-0 chain starts with b_h v_0, H(h, c - 1) pairs with v_c), until
-// XCMD_STOP.  A lost word ends the loop (block 0 then finds its own polls
-// failing and reports the error).
-__device__ void eig_worker(uint64_t* xb, int k, int ldh, const double* Hs, double* lds, int* sh) {
-  const int tid = threadIdx.x, bs = blockDim.x;
-  const int h0 = (blockIdx.x - 1) * kWRows;
-  const int nr = k - h0 < kWRows ? k - h0 : kWRows;
-  double* Hl = lds;                   // [kWRows][kWLd]
-  double* v = Hl + kWRows * kWLd;     // [kNMax]
-  double* bl = v + kNMax;             // [kWRows]
-  const uint32_t counter = (uint32_t)__hip_atomic_load(xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int i = tid; i < nr * k; i += bs) {
-    const int r = i / k, c = i - r * k;
-    Hl[r * kWLd + c] = Hs[(size_t)(h0 + r) * ldh + c];
-  }
-  int have_b = 0;
-  for (int rnd = 0;; ++rnd) {
-    const uint32_t tag = xtag(counter, rnd);
-    uint64_t* cw = xb + kXHead + (rnd & 1) * kXCmd;
-    uint64_t* rw = xb + kXHead + 2 * kXCmd + (rnd & 1) * kXRep;
-    // every round carries k + 1 vector words (the header and the words
-    // are polled together: one round trip)
-    int bad = 0;
-    if (tid == 0) {
-      uint64_t hd;
-      int p = 0;
-      do {
-        hd = __hip_atomic_load(cw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } while ((uint32_t)(hd >> 32) != tag && ++p < kXPoll);
-      sh[0] = (uint32_t)(hd >> 32) == tag ? (int)((hd >> 16) & 0xffff) : XCMD_STOP;
-      sh[1] = (int)(hd & 0xffff);
-      sh[2] = 0;
-    }
-    for (int i = tid; i <= k; i += bs) v[i] = xget(cw + 1 + 2 * i, tag, bad);
-    __syncthreads();
-    const int cmd = sh[0], nv = sh[1];
-    if (cmd == XCMD_STOP) break;
-    if (cmd == XCMD_A && !have_b) {
-      uint64_t* bw = xb + kXHead + 2 * kXCmd + 2 * kXRep;
-      for (int i = tid; i < nr; i += bs) bl[i] = xget(bw + 2 * (h0 + i), xtag_border(counter), bad);
-      have_b = 1;
-    }
-    if (bad) sh[2] = 1;
-    __syncthreads();
-    if (sh[2]) break;
-    if (tid < 64) {  // wave 0: lane 4 i + g forms class g of row h0 + i
-      const int i = tid >> 2, g = tid & 3;
-      double S = 0.0;
-      if (i < nr) {
-        // H(h, c - s) pairs with v_c (s = 1 in the bordered indices, where
-        // column 0 is b_h); eight entries loaded per batch
-        const int sft = cmd == XCMD_H ? 0 : 1;
-        const double* hr = Hl + i * kWLd;
-        int c = g;
-        if (sft && g == 0) {
-          S = __builtin_fma(bl[i], v[0], S);
-          c = 4;
-        }
-        for (; c + 28 < nv; c += 32) {
-          double a[8], b[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            a[u] = hr[c + 4 * u - sft];
-            b[u] = v[c + 4 * u];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) S = __builtin_fma(a[u], b[u], S);
-        }
-        for (; c < nv; c += 4) S = __builtin_fma(hr[c - sft], v[c], S);
-      }
-      const double s01 = S + lane_xor<1>(S);
-      const double sum = s01 + lane_xor<2>(s01);
-      if (g == 0 && i < nr) xput(rw + 2 * (h0 + i + (cmd == XCMD_A ? 1 : 0)), sum, tag);
-    }
-    __syncthreads();
-  }
 }
 
 // A condition of the one-thread tridiagonal code as a wave-uniform value
@@ -874,22 +654,16 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       const int r = lane + 64 * t;
       if (r < n) {
         QC[t] = QC[t] / s1;
-        if ((lane & 3) == w) L.qw[w][r >> 2] = QC[t];
         if ((t & 3) == w) Q[r] = QC[t];
       }
     }
   }
-  __syncthreads();  // q_0 (L.qw) visible to every wave
+  __syncthreads();  // q_0 visible to every wave
   double aprev = 0.0, bprev = 0.0, bprev2 = 0.0, tn_done = 0.0, tn = 0.0;
   double hint = -__builtin_huge_val(), prev = 0.0;  // warm start of the next check (every thread)
   int m = 0;
   for (int j = 0; j < n; ++j) {
-    if constexpr (Mat::kSums) {
-      dist_mv(A, n, Q + (size_t)j * n, L.part[j & 1][0]);
-    } else {
-      mat_partials(A, n, [&](int i) { return L.qw[w & 3][i]; }, L.part[j & 1]);
-      __syncthreads();
-    }
+    rep_mv(A, n, n, Q + (size_t)j * n, 0, L.part[j & 1][0]);
     double alpha = 0.0;
     if (cw) {
       double(*P)[kNMax] = L.part[j & 1];
@@ -899,7 +673,7 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
         const int r = lane + 64 * t;
         W[t] = 0.0;
         if (r < n) {
-          double s = Mat::kSums ? P[0][r] : (P[0][r] + P[1][r]) + (P[2][r] + P[3][r]);
+          double s = P[0][r];
           if (j > 0) s = __builtin_fma(-bprev, QP[t], s);
           W[t] = s;
           pa = pa + QC[t] * s;
@@ -1030,12 +804,11 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
         if (r < n) {
           const double q = W[t] / beta;
           QC[t] = q;
-          if ((lane & 3) == w) L.qw[w][r >> 2] = q;
           if ((t & 3) == w) Q[(size_t)(j + 1) * n + r] = q;
         }
       }
     }
-    __syncthreads();  // q_{j+1} (L.qw) visible to every wave
+    __syncthreads();  // q_{j+1} visible to every wave
   }
   // Ritz vector v = sum_i y_i q_i (waves 0-3, all rows), normalised in
   // double-double.
@@ -1436,13 +1209,11 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
                                              const double* px, const double* pz, double* dx, double* tabx,
                                              int slot_stride, const uint32_t* pslot, int mode, uint64_t* xb) {
   __shared__ EigLds L;
-  __shared__ double Qlds[kQBudget];  // A (n <= kEigLdsN), then the Lanczos basis
+  __shared__ double Qlds[kQBudgetRep];        // the Lanczos basis
+  __shared__ double Arows[kRRows * kRLd];     // this block's rows of A
   if (st->error) return;
-  if (blockIdx.x > 0) {
-    eig_worker(xb, k, ldh, Hs, Qlds, L.iv);
-    return;
-  }
   const int tid = threadIdx.x, bs = blockDim.x;
+  const bool lead = blockIdx.x == 0;  // writes the results
   const uint32_t xcount = (uint32_t)__hip_atomic_load(xb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (tid == 0) {
     L.iv[6] = 0;  // exchange round
@@ -1466,10 +1237,18 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   const bool normal = __builtin_fabs(yy) > 1e-14;
   const int n = normal ? k + 1 : k;
   const int o = normal ? 1 : 0;
-  const bool in_lds = n <= kEigLdsN;
-  // H x on the workers
-  dist_mv(MatDist{xb, xcount, k + 1, &L.iv[6], &L.iv[7], 0, 0.0, nullptr}, k, L.x, L.Hx);
+  // This block's rows of A: H_sub rows (its columns; k_submatrix stores both
+  // triangles) from column o; the border column is filled after H x.
+  const int r0 = blockIdx.x * kRRows;
+  const int nr = n - r0 < 0 ? 0 : (n - r0 < kRRows ? n - r0 : kRRows);
+  for (int idx = tid; idx < nr * k; idx += bs) {
+    const int i = idx / k, c = idx - i * k, r = r0 + i;
+    if (r >= o) Arows[i * kRLd + c + o] = Hs[(size_t)(r - o) * ldh + c];
+  }
+  const MatRep D{xb, xcount, &L.iv[6], &L.iv[7], Arows, r0, nr};
   if (normal) {
+    // H x, all-gathered (the H part of the rows)
+    rep_mv(D, k, k, L.x, 1, L.Hx);
     if (tid == 0) {
       double xTz = 0.0, xTHx = 0.0;
       for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
@@ -1480,40 +1259,23 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
       L.v0[0] = y1;
     }
     for (int i = tid; i < k; i += bs) {
-      const double b = (L.z[i] - L.Hx[i]) / y1;
-      L.b0[i] = b;
+      L.b0[i] = (L.z[i] - L.Hx[i]) / y1;
       L.v0[i + 1] = L.x[i];
-      xput(xb + kXHead + 2 * kXCmd + 2 * kXRep + 2 * i, b, xtag_border(xcount));
     }
+    __syncthreads();
+    for (int i = tid; i < nr; i += bs) {
+      const int r = r0 + i;
+      Arows[i * kRLd] = r == 0 ? L.sc[6] : L.b0[r - 1];
+    }
+    if (r0 == 0)
+      for (int c = 1 + tid; c < n; c += bs) Arows[c] = L.b0[c - 1];
   } else {
     // x is the reference determinant alone, which a symmetry can hide from
     // the lowest eigenvector: start from the fixed pseudo-random vector.
     for (int i = tid; i < k; i += bs) L.v0[i] = 0.0;
   }
   __syncthreads();
-  const MatSub Ms{Hs, ldh, L.b0, normal ? L.sc[6] : 0.0, o};
-  const MatDist Md{xb, xcount, k + 1, &L.iv[6], &L.iv[7], o, normal ? L.sc[6] : 0.0, L.b0};
-  if (in_lds) {
-    // Eight loads in flight per thread (H_sub comes from another kernel's
-    // writes, so each load waits on L2 or the last-level cache).
-    constexpr int U = 8;
-    for (int i0 = tid; i0 < n * n; i0 += U * bs) {
-      double v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int idx = i0 + u * bs;
-        const int c = idx / n, r = idx - c * n;
-        v[u] = idx < n * n ? Ms(r, c) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (i0 + u * bs < n * n) Qlds[i0 + u * bs] = v[u];
-    }
-    __syncthreads();
-  }
   const int first0 = st->lz_prev - 1 > 4 ? st->lz_prev - 1 : 4;
-  const MatLds Ml{Qlds, n};
-  double* const Qb = Qlds + (in_lds ? n * n : 0);
   // Solve, then accept when |A v - rq v| <= kLzAccept |T|_inf; otherwise
   // restart from the Ritz vector (checks from 4 vectors on).
   int first = first0, attempts = 0;
@@ -1521,24 +1283,12 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   dd rq{0.0, 0.0};
   double res2 = 0.0;
   for (;;) {
-    // Separate inlined calls keep the LDS case on ds_* instructions.
-    if (in_lds)
-      lanczos_lowest(L, Ml, n, first, perturb, Qb);
-    else
-      lanczos_lowest(L, Md, n, first, perturb, Qb);
+    lanczos_lowest(L, D, n, first, perturb, Qlds);
     for (int i = tid; i < n; i += bs) {
       L.ev[i] = L.q[i].hi;
       L.dl[i] = L.q[i].lo;
     }
-    __syncthreads();
-    if (in_lds) {
-      mat_partials(Ml, n, [&](int i) { return L.ev[4 * i + ((tid >> 6) & 3)]; }, L.part[0]);
-      __syncthreads();
-      for (int r = tid; r < n; r += bs) L.Ae[r] = dot4_combine(L.part[0], r);
-    } else {
-      dist_mv(Md, n, L.ev, L.Ae);
-    }
-    __syncthreads();
+    rep_mv(D, n, n, L.ev, 0, L.Ae);
     const dd t1 = red64_dd(n, [&](int i) { return two_prod(L.ev[i], L.Ae[i]); });
     const dd t2 = red64_dd(n, [&](int i) { return two_prod(L.dl[i], L.Ae[i]); });
     rq = dd_add(t1, dd_mul_d(t2, 2.0));
@@ -1551,7 +1301,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     if (__builtin_sqrt(res2) <= kLzAccept * L.sc[1]) break;
     if (attempts == kLzAttempts) {
       // accepted without passing the residual test: counted, reported
-      if (tid == 0) st->eig_unconverged += 1;
+      if (lead && tid == 0) st->eig_unconverged += 1;
       break;
     }
     for (int i = tid; i < n; i += bs) L.v0[i] = L.ev[i];
@@ -1559,24 +1309,14 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     first = 4;
     perturb = 0.0;
   }
-  // release the workers (a round of zeros with the STOP header), advance
-  // the launch counter
   __syncthreads();
-  {
-    const uint32_t tag = xtag(xcount, L.iv[6]);
-    uint64_t* cw = xb + kXHead + (L.iv[6] & 1) * kXCmd;
-    for (int i = tid; i <= k; i += bs) xput(cw + 1 + 2 * i, 0.0, tag);
-    if (tid == 0)
-      __hip_atomic_store(cw, ((uint64_t)tag << 32) | ((uint64_t)XCMD_STOP << 16), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
+  if (L.iv[7] && tid == 0) st->error = ERR_SYNC;
+  if (!lead || L.iv[7]) return;
+  // advance the launch counter (every block read it before its first round)
   if (tid == 0) {
     __hip_atomic_store(xb, (uint64_t)(xcount + 1u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (L.iv[7]) st->error = ERR_SYNC;
+    st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
   }
-  if (L.iv[7]) return;
-  if (tid == 0) st->lz_prev = attempts > 1 ? (first0 / 2 > 4 ? first0 / 2 : 4) : L.iv[2];
   eig_finish(st, L, k, n, normal, y1, rq, res2, dx, tabx, slot_stride, pslot, mode);
 }
 
@@ -1780,14 +1520,13 @@ void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, c
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode,
                 uint64_t* xb) {
   // Up to n = k + 1 = kEigLdsN the row-owner form, one wave per 64 rows;
-  // above, the class form on eight waves (mat_partials assumes two row
-  // blocks per class there) with the products on ceil(k / kWRows) worker
-  // workgroups.
+  // above, the rows of A spread over ceil((k + 1) / kRRows) workgroups that
+  // each run the solve on eight waves.
   if (k + 1 <= kEigLdsN)
     hipLaunchKernelGGL(k_eig1, dim3(1), dim3(k + 1 <= 64 ? 64 : 128), 0, s, st, k, ldh, Hs, px, pz, dx, tabx,
                        slot_stride, pslot, mode);
   else
-    hipLaunchKernelGGL(k_eig, dim3(1 + (k + kWRows - 1) / kWRows), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px,
+    hipLaunchKernelGGL(k_eig, dim3((k + 1 + kRRows - 1) / kRRows), dim3(kEigThreads), 0, s, st, k, ldh, Hs, px,
                        pz, dx, tabx, slot_stride, pslot, mode, xb);
 }
 

@@ -13,7 +13,7 @@ void launch_submatrix(hipStream_t s, int nw, const DevHam& H, DevState* st, cons
 // Subspace update.  xb: the exchange buffer of the solver's worker
 // workgroups (kXchgWords words, zeroed once when allocated; used for
 // k >= kEigLdsN).
-constexpr int kXchgWords = 2752;
+constexpr int kXchgWords = 1152;
 void launch_eig(hipStream_t s, DevState* st, int k, int ldh, const double* Hs, const double* px,
                 const double* pz, double* dx, double* tabx, int slot_stride, const uint32_t* pslot, int mode,
                 uint64_t* xb);

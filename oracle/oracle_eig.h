@@ -57,13 +57,16 @@ constexpr double kLzTol = 1e-15;
 constexpr double kLzAccept = 8e-15;  // |A v - rq v| <= kLzAccept |T|_inf accepts a solve
 constexpr double kLzPerturb = 0x1p-16;
 constexpr int kLzAttempts = 6;
-// The device keeps the Lanczos basis (and A, up to n = 90) in 12288 doubles
-// of on-chip memory; a solve stops at that many vectors and restarts.
+// The device keeps the Lanczos basis in on-chip memory: with A in 12288
+// doubles up to n = 90, in 8192 doubles above (where the rows of A are
+// spread over several workgroups that each hold the whole basis); a solve
+// stops at that many vectors and restarts.
 constexpr int kEigLdsN = 90;
 constexpr int kQBudget = 12288;
+constexpr int kQBudgetRep = 8192;
 
 static inline int lz_capacity(int n) {
-  const int avail = kQBudget - (n <= kEigLdsN ? n * n : 0);
+  const int avail = n <= kEigLdsN ? kQBudget - n * n : kQBudgetRep;
   return std::min(n, avail / n);
 }
 
