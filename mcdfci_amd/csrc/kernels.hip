@@ -565,8 +565,10 @@ __global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2*
   __shared__ uint32_t pl[kSel];
   __shared__ i128 red[kWG / 64][2];
   if (st->error) return;
-  const int t = blockIdx.y;
-  const int chunk = blockIdx.x;
+  // Columns and chunks in the reverse of the scatter's order: the lines the
+  // scatter touched last are the likeliest still in the last-level cache.
+  const int t = gridDim.y - 1 - blockIdx.y;
+  const int chunk = gridDim.x - 1 - blockIdx.x;
   setup_ctx<W>(C, H, pdet[t]);
   const double dxt = dx[t];
   const dd s = load_xss(st);
