@@ -1273,7 +1273,12 @@ __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, int
   if (threadIdx.x == 0) out_cnt[t] = keep;
 }
 
-template <int E, int T, int W>
+// One node of the pick's merge tree: the first K distinct entries of the
+// union of F sorted lists (input lists F j .. F j + F - 1).  The window of T * E
+// entries holds list q in its q-th part of T * E / F (>= K) entries, reversed
+// for odd q, so that each pair of parts is bitonic: the sort's levels from
+// T * E / (F / 2) up merge them.  Any fan-in gives the same first K distinct.
+template <int E, int T, int W, int F>
 __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int K, const PickEnt* in,
                                                   const int* in_cnt, int nin, PickEnt* out, int* out_cnt) {
   __shared__ uint64_t ka[T * E];
@@ -1283,32 +1288,26 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int K, con
   __shared__ int wtot[T / 64 + 1];
   if (st->error) return;
   const int j = blockIdx.x;
-  const int la = 2 * j, lb = 2 * j + 1;
-  const int na = in_cnt[la], nb = lb < nin ? in_cnt[lb] : 0;
-  const int half = T * E / 2;  // >= K
+  constexpr int Q = T * E / F;  // >= K
   uint64_t A[E], B[E], B1[E];
   uint32_t P[E];
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     const int i = threadIdx.x * E + r;
+    const int q = i / Q, pos = i - q * Q;
+    const int li = F * j + q;
+    const int nq = li < nin ? in_cnt[li] : 0;
+    const int e = (q & 1) ? Q - 1 - pos : pos;
     A[r] = 0;
     B[r] = 0;
     B1[r] = 0;
     P[r] = 0xffffffffu;
-    if (i < half) {
-      if (i < na) {
-        const PickEnt e = in[(size_t)la * K + i];
-        A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
-      }
-    } else {
-      const int q = 2 * half - 1 - i;
-      if (q < nb) {
-        const PickEnt e = in[(size_t)lb * K + q];
-        A[r] = e.key, B[r] = e.det, B1[r] = e.det1, P[r] = e.slot;
-      }
+    if (e < nq) {
+      const PickEnt en = in[(size_t)li * K + e];
+      A[r] = en.key, B[r] = en.det, B1[r] = en.det1, P[r] = en.slot;
     }
   }
-  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc}, T * E);
+  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc}, T * E / (F / 2));
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < E; ++r) {
@@ -1853,16 +1852,23 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
         default: MC_COLS(4, 256); break;
       }
 #undef MC_COLS
-      // Merge levels (lists of up to k): window 2 * next power of two >= k.
+      // Merge levels (lists of up to k): four lists per node while the window
+      // (the next power of two >= fan-in x k) stays within 512 entries (half
+      // as many levels; measured faster at k = 64), else two (a 1024-entry
+      // sort per node costs more than the levels it saves: k = 256).
+      const int fan = 4 * k <= 512 ? 4 : 2;
       int wm = 64;
-      while (wm < 2 * k) wm <<= 1;
+      while (wm < fan * k) wm <<= 1;
       PickEnt *in = pk0, *out = pk1;
       int *ic = pc0, *oc = pc1;
       int nin = k;
       while (nin > 1) {
-        const int nout = (nin + 1) / 2;
-#define MC_PM(E_, T_) \
-  hipLaunchKernelGGL((k_pick_merge<E_, T_, W>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc)
+        const int nout = (nin + fan - 1) / fan;
+#define MC_PM(E_, T_)                                                                                      \
+  if (fan == 4)                                                                                            \
+    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 4>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc); \
+  else                                                                                                     \
+    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 2>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc)
         switch (wm) {
           case 64: MC_PM(1, 64); break;
           case 128: MC_PM(1, 128); break;
