@@ -1796,6 +1796,10 @@ void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, vo
         case 2048: MC_MERGE(4, 512, false); break;
         default: MC_MERGE(4, 1024, false); break;
       }
+    } else if (mw <= 256 && nchunks > 8) {
+      // Staging-buffer tournament: one wave per pair of lists, so sixteen
+      // waves merge the first level of up to 32 lists in one round.
+      MC_MERGE(1, 1024, true);
     } else {
       int w = kWG;
       while (w < 2 * mw) w <<= 1;
