@@ -826,6 +826,172 @@ __device__ void column_tournament(StageEnt* lists, int nchunks, int mw, const in
   }
 }
 
+// Two-level selection for columns of many chunk lists (mw <= 256, at most
+// T*E/kHead lists): the first kHead entries of every list (its head) are
+// reduced in LDS by the in-place pairwise tournament of column_tournament
+// (list i absorbing list i + s, one wave per pair, each pair kept whole up to
+// mw entries), giving S, the top mw of all heads.  A list's later entries
+// (its tail) all rank below its last head entry, so the column's top mw lies
+// in S and the tails of the lists whose last head entry is not below S's
+// last (all tails when S holds fewer than mw entries): the tails of up to
+// T*E/L - 1 such lists (L = the power of two >= mw) are loaded into LDS
+// beside S and reduced with it by the same tournament.  List 0 ends as the
+// first mw entries of a full sort of the column's entries, returned in
+// blocked registers (entry threadIdx.x*E + r).  With more such lists it
+// returns false without output and the caller runs column_tournament, which
+// reads the staging buffer only.
+constexpr int kHead = 64;
+template <int W, int E, int T>
+__device__ bool column_heads(const StageEnt* lists, int nchunks, int mw, const int* offs, int* lcnt,
+                             const SortScratch& x, uint64_t (&A)[E], uint64_t (&B)[E], uint64_t (&B1)[E],
+                             uint32_t (&P)[E]) {
+  constexpr bool C3 = W == 2;
+  constexpr int kLists = T * E / kHead;
+  __shared__ uint64_t lha[kLists], lhb[kLists], lhc[kLists];
+  __shared__ int need;
+  __shared__ int tail_of[kLists];
+  for (int g = threadIdx.x; g < nchunks * kHead; g += T) {
+    const int c = g / kHead, j = g - c * kHead;
+    uint64_t a = 0, b = 0, b1 = 0;
+    uint32_t p = 0xffffffffu;
+    if (j < offs[c + 1] - offs[c]) {
+      const StageEnt& e = lists[(size_t)c * mw + j];
+      a = e.val;
+      b = e.det;
+      b1 = e.det1;
+      p = e.slot;
+    }
+    x.a[g] = a;
+    x.b[g] = b;
+    x.p[g] = p;
+    if constexpr (C3) x.c[g] = b1;
+    if (j == kHead - 1) {
+      lha[c] = a;
+      lhb[c] = b;
+      lhc[c] = b1;
+    }
+  }
+  for (int c = threadIdx.x; c < nchunks; c += T) {
+    const int cnt = offs[c + 1] - offs[c];
+    lcnt[c] = cnt < kHead ? cnt : kHead;
+  }
+  if (threadIdx.x == 0) need = 0;
+  __syncthreads();
+  int L = 1;
+  while (L < mw) L <<= 1;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The lists at xo (count lcnt[i]) and yo (count lcnt[j]): the top Lk =
+  // 64*E2 of both (Lk at least either count), written back from xo truncated
+  // to mw entries; lcnt[i] becomes the kept count.
+  auto merge_pair = [&](auto e2tag, int xo, int yo, int i, int j) {
+    constexpr int E2 = decltype(e2tag)::value;
+    constexpr int Lk = 64 * E2;
+    uint64_t a[E2], b[E2], c[E2];
+    uint32_t pp[E2];
+    const int ni = lcnt[i], nj = lcnt[j];
+#pragma unroll
+    for (int r = 0; r < E2; ++r) {
+      const int q = lane * E2 + r;
+      uint64_t xa = 0, xb = 0, xc = 0, ya = 0, yb = 0, yc = 0;
+      uint32_t xp = 0xffffffffu, yp = 0xffffffffu;
+      if (q < ni) {
+        xa = x.a[xo + q], xb = x.b[xo + q], xp = x.p[xo + q];
+        if constexpr (C3) xc = x.c[xo + q];
+      }
+      if (Lk - 1 - q < nj) {
+        const int yq = yo + Lk - 1 - q;
+        ya = x.a[yq], yb = x.b[yq], yp = x.p[yq];
+        if constexpr (C3) yc = x.c[yq];
+      }
+      const bool ty = key_gt3<C3>(ya, yb, yc, xa, xb, xc);
+      a[r] = ty ? ya : xa;
+      b[r] = ty ? yb : xb;
+      c[r] = ty ? yc : xc;
+      pp[r] = ty ? yp : xp;
+    }
+    wave_bitonic_merge<E2, C3>(a, b, c, pp, Lk);
+    const int keep = ni + nj < mw ? ni + nj : mw;
+#pragma unroll
+    for (int r = 0; r < E2; ++r) {
+      const int q = lane * E2 + r;
+      if (q < keep) {
+        x.a[xo + q] = a[r];
+        x.b[xo + q] = b[r];
+        x.p[xo + q] = pp[r];
+        if constexpr (C3) x.c[xo + q] = c[r];
+      }
+    }
+    if (lane == 0) lcnt[i] = keep;
+  };
+  auto merge_lk = [&](int Lk, int xo, int yo, int i, int j) {
+    if (Lk <= 64) merge_pair(LaneCount<1>{}, xo, yo, i, j);
+    else if (Lk == 128) merge_pair(LaneCount<2>{}, xo, yo, i, j);
+    else merge_pair(LaneCount<4>{}, xo, yo, i, j);
+  };
+  // Heads: list c at c*kHead; a list at stride sd holds at most
+  // min(sd*kHead, mw) entries, so Lk = min(2*sd*kHead, L) covers either
+  // count and every kept entry (in the pair's own regions).
+  for (int sd = 1; sd < nchunks; sd <<= 1) {
+    const int Lk = 2 * sd * kHead < L ? 2 * sd * kHead : L;
+    for (int i = 2 * sd * wv; i + sd < nchunks; i += 2 * sd * (T / 64)) merge_lk(Lk, i * kHead, (i + sd) * kHead, i, i + sd);
+    __syncthreads();
+  }
+  const int n0 = lcnt[0];
+  for (int c = threadIdx.x; c < nchunks; c += T) {
+    if (offs[c + 1] - offs[c] > kHead &&
+        (n0 < mw || !key_gt3<C3>(x.a[mw - 1], x.b[mw - 1], C3 ? x.c[mw - 1] : 0, lha[c], lhb[c], lhc[c]))) {
+      const int r = atomicAdd(&need, 1);
+      if (r < kLists) tail_of[r] = c;
+    }
+  }
+  __syncthreads();
+  const int m = need;
+  if (m > T * E / L - 1) return false;
+  if (m > 0) {
+    // S stays at [0, n0); tail r (list tail_of[r], entries kHead..count-1,
+    // at most mw - kHead < L) goes to region r + 1 of L entries.
+    for (int g = threadIdx.x; g < m * L; g += T) {
+      const int r = g / L, j = g - r * L;
+      const int c = tail_of[r];
+      uint64_t a = 0, b = 0, b1 = 0;
+      uint32_t p = 0xffffffffu;
+      if (j < offs[c + 1] - offs[c] - kHead) {
+        const StageEnt& e = lists[(size_t)c * mw + kHead + j];
+        a = e.val;
+        b = e.det;
+        b1 = e.det1;
+        p = e.slot;
+      }
+      const int o = (r + 1) * L + j;
+      x.a[o] = a;
+      x.b[o] = b;
+      x.p[o] = p;
+      if constexpr (C3) x.c[o] = b1;
+    }
+    // lcnt is free again: region counts (lcnt[0] = n0 already)
+    if ((int)threadIdx.x < m) {
+      const int c = tail_of[threadIdx.x];
+      lcnt[threadIdx.x + 1] = offs[c + 1] - offs[c] - kHead;
+    }
+    __syncthreads();
+    for (int sd = 1; sd < m + 1; sd <<= 1) {
+      for (int i = 2 * sd * wv; i + sd < m + 1; i += 2 * sd * (T / 64)) merge_lk(L, i * L, (i + sd) * L, i, i + sd);
+      __syncthreads();
+    }
+  }
+  const int nf = lcnt[0];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int g = threadIdx.x * E + r;
+    const bool in = g < nf;
+    A[r] = in ? x.a[g] : 0;
+    B[r] = in ? x.b[g] : 0;
+    B1[r] = (C3 && in) ? x.c[g] : 0;
+    P[r] = in ? x.p[g] : 0xffffffffu;
+  }
+  return true;
+}
+
 // ------------------------------------------------------------------ k_merge
 // One workgroup per column: merge chunk lists into the column's top-M, keep
 // the determinants present in the table (the refetch of
@@ -993,7 +1159,13 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
     }
   } else if (mw <= 256) {
     __shared__ int lcnt[1024];
-    column_tournament<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, A, B, B1, P);
+    bool done = false;
+    if constexpr (T * E >= 2048) {
+      if (nchunks * kHead <= T * E)
+        done = column_heads<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, scr, A, B, B1, P);
+    }
+    if (!done)
+      column_tournament<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, A, B, B1, P);
   } else {
     const int half = T * E / 2;  // >= mw
     auto fetch = [&](int chunk, int j, uint64_t& a, uint64_t& b, uint64_t& b1, uint32_t& p) {
@@ -1085,20 +1257,48 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
   // Iteration counters: wave 0 sums the stripes and clears them; the
   // columns' x.z increments (exact integers) and max |dx| over k <= 256
   // columns, four per lane of wave 0 (order-free sum and maximum).
+  // Every load is issued before the first sum (one memory round trip), the
+  // state thread 0 updates included.
   long long c_new = 0, c_upd = 0, c_cols = 0;
   i128 xzc = 0;
   double dm = 0.0;
+  int64_t s_xh = 0;
+  uint64_t s_xl = 0;
+  long long s_zs = 0, s_it = 0;
+  double s_max = 0.0;
   if (threadIdx.x < 64) {
     long long* cs = ctr + threadIdx.x * kCtrStride;
     c_new = cs[0];
     c_upd = cs[1];
     c_cols = cs[2];
+    int64_t xh[kMaxK / 64], xl[kMaxK / 64];
+    double dd[kMaxK / 64];
+#pragma unroll
+    for (int j = 0; j < kMaxK / 64; ++j) {
+      const int c = threadIdx.x + 64 * j;
+      xh[j] = 0;
+      xl[j] = 0;
+      dd[j] = 0.0;
+      if (c < k) {
+        xh[j] = col_xz[2 * c];
+        xl[j] = col_xz[2 * c + 1];
+        dd[j] = dx[c];
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_xh = st->xz_hi;
+      s_xl = st->xz_lo;
+      s_zs = st->z_size;
+      s_it = st->iterations;
+      s_max = st->max_size;
+    }
     cs[0] = 0;
     cs[1] = 0;
     cs[2] = 0;
-    for (int c = threadIdx.x; c < k; c += 64) {
-      xzc += (((i128)col_xz[2 * c]) << 64) | (i128)(uint64_t)col_xz[2 * c + 1];
-      dm = fmax(dm, __builtin_fabs(dx[c]));
+#pragma unroll
+    for (int j = 0; j < kMaxK / 64; ++j) {
+      xzc += (((i128)xh[j]) << 64) | (i128)(uint64_t)xl[j];
+      dm = fmax(dm, __builtin_fabs(dd[j]));
     }
     c_new = wave_sum_ll(c_new);
     c_upd = wave_sum_ll(c_upd);
@@ -1107,18 +1307,18 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
     for (int off = 32; off >= 1; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off, 64));
   }
   if (threadIdx.x != 0) return;
-  i128 xz = (((i128)st->xz_hi) << 64) | (i128)st->xz_lo;
+  i128 xz = (((i128)s_xh) << 64) | (i128)s_xl;
   xz += xzc;
   st->xz_hi = (int64_t)(xz >> 64);
   st->xz_lo = (uint64_t)xz;
-  st->z_size += c_new;
+  st->z_size = s_zs + c_new;
   st->updates = c_upd;
   st->cols_size = c_cols;
   st->ticket = 0;
   st->k_prev = k;
   st->dx_max = dm;
-  if (count_iteration) st->iterations += 1;
-  if ((double)st->z_size > 0.79 * st->max_size) st->error = ERR_TABLE_FULL;
+  if (count_iteration) st->iterations = s_it + 1;
+  if ((double)(s_zs + c_new) > 0.79 * s_max) st->error = ERR_TABLE_FULL;
 }
 
 // ------------------------------------------------------------------ k_pick
@@ -1797,9 +1997,12 @@ void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, vo
         default: MC_MERGE(4, 1024, false); break;
       }
     } else if (mw <= 256 && nchunks > 8) {
-      // Staging-buffer tournament: one wave per pair of lists, so sixteen
-      // waves merge the first level of up to 32 lists in one round.
-      MC_MERGE(1, 1024, true);
+      // Sixteen waves, one per pair of lists: the first tournament level of
+      // up to 32 lists in one round; at one word the heads of up to 32 lists
+      // fit the 2048-entry LDS window (two words keep E = 1 within the
+      // register budget of 1024 threads).
+      if constexpr (W == 1) MC_MERGE(2, 1024, true);
+      else MC_MERGE(1, 1024, true);
     } else {
       int w = kWG;
       while (w < 2 * mw) w <<= 1;

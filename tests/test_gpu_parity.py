@@ -138,6 +138,9 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10), up
     ("c2", 16, 1024, 1e-7, 12),
     # the norb-24 molecule at k = M = 256
     ("c4", 256, 256, 1e-6, 4),
+    # 29 chunk lists of 256 per column: the column merge's head selection,
+    # with and without the tails of the lists whose heads reach the top M
+    ("c4", 32, 256, 1e-6, 30),
     # 42 orbitals: two-word determinants
     ("c5", 8, 16, 1e-5, 12),
     ("c5", 64, 64, 1e-5, 4),
