@@ -196,6 +196,36 @@ struct MatRep {
   int r0, nr;
 };
 
+// The fused multiply-add chain S = a[r] b[r] + S over r = r0, r0 + 4, ...
+// < n in ascending r (one column class of a dot product), its loads issued
+// sixteen pairs at a time ahead of the chain.
+__device__ __forceinline__ double class_chain(const double* a, const double* b, int r0, int n) {
+  double S = 0.0;
+  int r = r0;
+  for (; r + 60 < n; r += 64) {
+    double x[16], y[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      x[u] = a[r + 4 * u];
+      y[u] = b[r + 4 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) S = __builtin_fma(x[u], y[u], S);
+  }
+  for (; r + 12 < n; r += 16) {
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = a[r + 4 * u];
+      y[u] = b[r + 4 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) S = __builtin_fma(x[u], y[u], S);
+  }
+  for (; r < n; r += 4) S = __builtin_fma(a[r], b[r], S);
+  return S;
+}
+
 // out[r] = sum_c rows(r, c + sft) v[c], c < nv, for every row r < nout of the
 // matrix (sft = 1: the H part of the bordered rows, whose H row r - 1 the
 // block's row r holds from column 1; the output index is then r - 1), each
@@ -221,21 +251,7 @@ __device__ void rep_mv(const MatRep& D, int nout, int nv, const double* v, int s
     const int i = tid >> 2, g = tid & 3;
     const bool own = i < D.nr && D.r0 + i >= sft;
     double S = 0.0;
-    if (own) {
-      const double* ar = D.rows + i * kRLd + sft;
-      int c = g;
-      for (; c + 28 < nv; c += 32) {
-        double a[8], x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          a[u] = ar[c + 4 * u];
-          x[u] = v[c + 4 * u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) S = __builtin_fma(a[u], x[u], S);
-      }
-      for (; c < nv; c += 4) S = __builtin_fma(ar[c], v[c], S);
-    }
+    if (own) S = class_chain(D.rows + i * kRLd + sft, v, g, nv);
     const double s01 = S + lane_xor<1>(S);
     const double sum = s01 + lane_xor<2>(s01);
     if (g == 0 && own) xput(rw + 2 * (D.r0 + i - sft), sum, tag);
@@ -696,47 +712,47 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     for (int i0 = 0; i0 <= j; i0 += (int)blockDim.x >> 2) {
       const int i = i0 + (tid >> 2), g = tid & 3;
       double S = 0.0;
-      if (i <= j) {
-        const double* qi = Q + (size_t)i * n;
-        int r = g;
-        for (; r + 12 < n; r += 16) {
-          double qa[4], wa[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            qa[u] = qi[r + 4 * u];
-            wa[u] = L.wv[r + 4 * u];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) S = __builtin_fma(qa[u], wa[u], S);
-        }
-        for (; r < n; r += 4) S = __builtin_fma(qi[r], L.wv[r], S);
-      }
+      if (i <= j) S = class_chain(Q + (size_t)i * n, L.wv, g, n);
       const double s01 = S + lane_xor<1>(S);
       const double h = s01 + lane_xor<2>(s01);
       if (g == 0 && i <= j) L.hbuf[i] = h;
     }
     __syncthreads();
     if (cw) {
+      // every row's chain in ascending i, the lane's rows interleaved (rows
+      // at or past n read row 0 and are discarded)
+      double acc[kRows];
+      int rc[kRows];
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) {
+        acc[t] = W[t];
+        rc[t] = lane + 64 * t < n ? lane + 64 * t : 0;
+      }
+      int i = 0;
+      for (; i + 7 <= j; i += 8) {
+        double ha[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ha[u] = L.hbuf[i + u];
+#pragma unroll
+        for (int t = 0; t < kRows; ++t) {
+          double qa[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[t]];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[t] = __builtin_fma(-ha[u], qa[u], acc[t]);
+        }
+      }
+      for (; i <= j; ++i) {
+        const double h = L.hbuf[i];
+#pragma unroll
+        for (int t = 0; t < kRows; ++t) acc[t] = __builtin_fma(-h, Q[(size_t)i * n + rc[t]], acc[t]);
+      }
       double pb = 0.0;
 #pragma unroll
       for (int t = 0; t < kRows; ++t) {
-        const int r = lane + 64 * t;
-        if (r < n) {
-          double acc = W[t];
-          int i = 0;
-          for (; i + 7 <= j; i += 8) {
-            double qa[8], ha[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              qa[u] = Q[(size_t)(i + u) * n + r];
-              ha[u] = L.hbuf[i + u];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) acc = __builtin_fma(-ha[u], qa[u], acc);
-          }
-          for (; i <= j; ++i) acc = __builtin_fma(-L.hbuf[i], Q[(size_t)i * n + r], acc);
-          W[t] = acc;
-          pb = pb + acc * acc;
+        if (lane + 64 * t < n) {
+          W[t] = acc[t];
+          pb = pb + acc[t] * acc[t];
         }
       }
       const double beta = __builtin_sqrt(bfly_sum(pb));
@@ -813,29 +829,38 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
   // Ritz vector v = sum_i y_i q_i (waves 0-3, all rows), normalised in
   // double-double.
   if (cw) {
+    // every row's chain in ascending i, the lane's rows interleaved
     double V[kRows];
+    int rc[kRows];
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      V[t] = 0.0;
+      rc[t] = lane + 64 * t < n ? lane + 64 * t : 0;
+    }
+    int i = 0;
+    for (; i + 7 < m; i += 8) {
+      double ya[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) ya[u] = L.y[i + u];
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) {
+        double qa[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[t]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) V[t] = __builtin_fma(ya[u], qa[u], V[t]);
+      }
+    }
+    for (; i < m; ++i) {
+      const double y = L.y[i];
+#pragma unroll
+      for (int t = 0; t < kRows; ++t) V[t] = __builtin_fma(y, Q[(size_t)i * n + rc[t]], V[t]);
+    }
     dd pn{0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
-      const int r = lane + 64 * t;
-      V[t] = 0.0;
-      if (r < n) {
-        double acc = 0.0;
-        int i = 0;
-        for (; i + 7 < m; i += 8) {
-          double qa[8], ya[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            qa[u] = Q[(size_t)(i + u) * n + r];
-            ya[u] = L.y[i + u];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc = __builtin_fma(ya[u], qa[u], acc);
-        }
-        for (; i < m; ++i) acc = __builtin_fma(L.y[i], Q[(size_t)i * n + r], acc);
-        V[t] = acc;
-        pn = dd_add(pn, two_prod(acc, acc));
-      }
+      if (lane + 64 * t < n) pn = dd_add(pn, two_prod(V[t], V[t]));
+      else V[t] = 0.0;
     }
     const dd nrm = dd_sqrt(bfly_sum_dd(pn));
 #pragma unroll
