@@ -284,6 +284,9 @@ def time_to_threshold(fcid, args, z_thr, max_mem):
         "precision_only_speedup_median": med([
             (r["fp64_batched"]["seconds"] / r["mixed"]["seconds"])
             if (r["fp64_batched"]["seconds"] and r["mixed"]["seconds"]) else None for r in reps]),
+        # energy of the mixed batched run at its last report minus the plain
+        # FP64 run's energy at the same wall time (negative: lower, better)
+        "mixed_minus_plain_at_equal_time_median": med([r["mixed_minus_plain_at_equal_time"] for r in reps]),
         "plain_k": args.plain_k,
         "batched_k": args.k,
         "per_rep": reps,

@@ -899,12 +899,22 @@ constexpr int kR1 = 2;  // entries per lane of the setup vectors (k < 128)
 // Row r of A v: (S0 + S1) + (S2 + S3), S_g the ascending sum over c = g
 // mod 4 of A(r, c) v_c.  A is symmetric (k_submatrix writes both triangles
 // from one value, the border is copied), so row r is read along its
-// contiguous storage a[r * lda + c], eight columns per batch; v in LDS.
+// contiguous storage a[r * lda + c], sixteen columns per batch; v in LDS.
 __device__ __forceinline__ double matvec_row(const double* a, int lda, int n, int r, const double* v) {
   const double* rowp = a + (r < n ? r : n - 1) * lda;
   double S[4] = {0.0, 0.0, 0.0, 0.0};
   int c0 = 0;
-  for (; c0 + 8 <= n; c0 += 8) {
+  for (; c0 + 16 <= n; c0 += 16) {
+    double vv[16], aa[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      vv[u] = v[c0 + u];
+      aa[u] = rowp[c0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) S[u & 3] = __builtin_fma(aa[u], vv[u], S[u & 3]);
+  }
+  if (c0 + 8 <= n) {
     double vv[8], aa[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -913,6 +923,7 @@ __device__ __forceinline__ double matvec_row(const double* a, int lda, int n, in
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) S[u & 3] = __builtin_fma(aa[u], vv[u], S[u & 3]);
+    c0 += 8;
   }
   if (c0 + 4 <= n) {
     double vv[4], aa[4];
