@@ -718,15 +718,19 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
       if (g == 0 && i <= j) L.hbuf[i] = h;
     }
     __syncthreads();
+    // w -= sum_i h_i q_i, each row's chain in ascending i: wave w forms rows
+    // lane + 64 t of t = w and t = w + 4 (interleaved; rows at or past n
+    // read row 0 and are discarded) and publishes them in L.wv, free since
+    // the coefficients were formed; then every row wave reads all rows back.
+    constexpr int kOwn = (kRows + 3) / 4;
     if (cw) {
-      // every row's chain in ascending i, the lane's rows interleaved (rows
-      // at or past n read row 0 and are discarded)
-      double acc[kRows];
-      int rc[kRows];
+      double acc[kOwn];
+      int rc[kOwn];
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) {
-        acc[t] = W[t];
-        rc[t] = lane + 64 * t < n ? lane + 64 * t : 0;
+      for (int s = 0; s < kOwn; ++s) {
+        const int r = lane + 64 * (w + 4 * s);
+        rc[s] = r < n ? r : 0;
+        acc[s] = r < n ? L.wv[r] : 0.0;
       }
       int i = 0;
       for (; i + 7 <= j; i += 8) {
@@ -734,25 +738,34 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
 #pragma unroll
         for (int u = 0; u < 8; ++u) ha[u] = L.hbuf[i + u];
 #pragma unroll
-        for (int t = 0; t < kRows; ++t) {
+        for (int s = 0; s < kOwn; ++s) {
           double qa[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[t]];
+          for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[s]];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc[t] = __builtin_fma(-ha[u], qa[u], acc[t]);
+          for (int u = 0; u < 8; ++u) acc[s] = __builtin_fma(-ha[u], qa[u], acc[s]);
         }
       }
       for (; i <= j; ++i) {
         const double h = L.hbuf[i];
 #pragma unroll
-        for (int t = 0; t < kRows; ++t) acc[t] = __builtin_fma(-h, Q[(size_t)i * n + rc[t]], acc[t]);
+        for (int s = 0; s < kOwn; ++s) acc[s] = __builtin_fma(-h, Q[(size_t)i * n + rc[s]], acc[s]);
       }
+#pragma unroll
+      for (int s = 0; s < kOwn; ++s) {
+        const int r = lane + 64 * (w + 4 * s);
+        if (r < n) L.wv[r] = acc[s];
+      }
+    }
+    __syncthreads();
+    if (cw) {
       double pb = 0.0;
 #pragma unroll
       for (int t = 0; t < kRows; ++t) {
-        if (lane + 64 * t < n) {
-          W[t] = acc[t];
-          pb = pb + acc[t] * acc[t];
+        const int r = lane + 64 * t;
+        if (r < n) {
+          W[t] = L.wv[r];
+          pb = pb + W[t] * W[t];
         }
       }
       const double beta = __builtin_sqrt(bfly_sum(pb));
@@ -826,16 +839,18 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     }
     __syncthreads();  // q_{j+1} visible to every wave
   }
-  // Ritz vector v = sum_i y_i q_i (waves 0-3, all rows), normalised in
-  // double-double.
+  // Ritz vector v = sum_i y_i q_i, each row's chain in ascending i, the rows
+  // spread over waves 0-3 as in the Gram-Schmidt update (through L.wv);
+  // wave 0 normalises it in double-double.
+  constexpr int kOwnR = (kRows + 3) / 4;
   if (cw) {
-    // every row's chain in ascending i, the lane's rows interleaved
-    double V[kRows];
-    int rc[kRows];
+    double V[kOwnR];
+    int rc[kOwnR];
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      V[t] = 0.0;
-      rc[t] = lane + 64 * t < n ? lane + 64 * t : 0;
+    for (int s = 0; s < kOwnR; ++s) {
+      const int r = lane + 64 * (w + 4 * s);
+      V[s] = 0.0;
+      rc[s] = r < n ? r : 0;
     }
     int i = 0;
     for (; i + 7 < m; i += 8) {
@@ -843,30 +858,43 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
 #pragma unroll
       for (int u = 0; u < 8; ++u) ya[u] = L.y[i + u];
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) {
+      for (int s = 0; s < kOwnR; ++s) {
         double qa[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[t]];
+        for (int u = 0; u < 8; ++u) qa[u] = Q[(size_t)(i + u) * n + rc[s]];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) V[t] = __builtin_fma(ya[u], qa[u], V[t]);
+        for (int u = 0; u < 8; ++u) V[s] = __builtin_fma(ya[u], qa[u], V[s]);
       }
     }
     for (; i < m; ++i) {
       const double y = L.y[i];
 #pragma unroll
-      for (int t = 0; t < kRows; ++t) V[t] = __builtin_fma(y, Q[(size_t)i * n + rc[t]], V[t]);
+      for (int s = 0; s < kOwnR; ++s) V[s] = __builtin_fma(y, Q[(size_t)i * n + rc[s]], V[s]);
     }
+#pragma unroll
+    for (int s = 0; s < kOwnR; ++s) {
+      const int r = lane + 64 * (w + 4 * s);
+      if (r < n) L.wv[r] = V[s];
+    }
+  }
+  __syncthreads();
+  if (w == 0) {
+    double V[kRows];
     dd pn{0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
-      if (lane + 64 * t < n) pn = dd_add(pn, two_prod(V[t], V[t]));
-      else V[t] = 0.0;
+      const int r = lane + 64 * t;
+      V[t] = 0.0;
+      if (r < n) {
+        V[t] = L.wv[r];
+        pn = dd_add(pn, two_prod(V[t], V[t]));
+      }
     }
     const dd nrm = dd_sqrt(bfly_sum_dd(pn));
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
       const int r = lane + 64 * t;
-      if (r < n && w == 0) L.q[r] = dd_div(mkdd(V[t]), nrm);
+      if (r < n) L.q[r] = dd_div(mkdd(V[t]), nrm);
     }
   }
   __syncthreads();
