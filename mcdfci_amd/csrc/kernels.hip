@@ -1478,9 +1478,12 @@ __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, int
 // entries holds list q in its q-th part of T * E / F (>= K) entries, reversed
 // for odd q, so that each pair of parts is bitonic: the sort's levels from
 // T * E / (F / 2) up merge them.  Any fan-in gives the same first K distinct.
+// The root (emit = 1) writes the picked coordinates as k_pick_emit does.
 template <int E, int T, int W, int F>
-__global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int K, const PickEnt* in,
-                                                  const int* in_cnt, int nin, PickEnt* out, int* out_cnt) {
+__global__ __launch_bounds__(T) void k_pick_merge(DevState* st, int K, const PickEnt* in, const int* in_cnt,
+                                                  int nin, PickEnt* out, int* out_cnt, int emit,
+                                                  const SlotW<W>* tab, D2* pdet, uint32_t* pslot, double* px,
+                                                  double* pz) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
   __shared__ uint64_t kc[W == 2 ? T * E : 1];
@@ -1528,6 +1531,25 @@ __global__ __launch_bounds__(T) void k_pick_merge(const DevState* st, int K, con
   }
   int distinct;
   int pos = block_excl_scan(cnt, wtot, distinct);
+  if (emit) {
+    if (distinct < K) {
+      if (threadIdx.x == 0) st->error = ERR_CANDIDATES;
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      if (!first[r]) continue;
+      if (pos < K) {
+        pdet[pos] = D2{B[r], B1[r]};
+        pslot[pos] = P[r];
+        const SlotW<W>& sl = tab[P[r]];
+        px[pos] = sl.x;
+        pz[pos] = fix_to_d(tz_join(sl.za, sl.zb));
+      }
+      ++pos;
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     if (!first[r]) continue;
@@ -2071,11 +2093,14 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
       int nin = k;
       while (nin > 1) {
         const int nout = (nin + fan - 1) / fan;
+        const int root = nout == 1 ? 1 : 0;  // the root emits the picked coordinates
 #define MC_PM(E_, T_)                                                                                      \
   if (fan == 4)                                                                                            \
-    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 4>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc); \
+    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 4>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc, \
+                       root, tb, pdet, pslot, px, pz);                                                     \
   else                                                                                                     \
-    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 2>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc)
+    hipLaunchKernelGGL((k_pick_merge<E_, T_, W, 2>), dim3(nout), dim3(T_), 0, s, st, k, in, ic, nin, out, oc, \
+                       root, tb, pdet, pslot, px, pz)
         switch (wm) {
           case 64: MC_PM(1, 64); break;
           case 128: MC_PM(1, 128); break;
@@ -2088,7 +2113,8 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
         std::swap(ic, oc);
         nin = nout;
       }
-      hipLaunchKernelGGL(k_pick_emit<W>, dim3(1), dim3(256), 0, s, st, k, in, ic, tb, pdet, pslot, px, pz);
+      if (k == 1)  // a single list: no merge level, nothing emitted yet
+        hipLaunchKernelGGL(k_pick_emit<W>, dim3(1), dim3(256), 0, s, st, k, in, ic, tb, pdet, pslot, px, pz);
     }
   }
 }
