@@ -115,6 +115,8 @@ struct mcdfci_ctx {
   uint32_t* znorm = nullptr;  // slots whose z word B pass 2 found near its range (kZNormCap)
   uint64_t* xbuf = nullptr;   // exchange buffer of the subspace solver's workgroups (kXchgWords)
   int znorm_bits = 61;        // fold threshold (MCDFCI_ZNORM_BITS lowers it to exercise the fold)
+  int merge_tails = 7;        // chunk lists whose tails the column merge takes in LDS (MCDFCI_MERGE_TAILS
+                              // lowers it to exercise the full tournament it falls back to)
   size_t iter_bytes = 0;
   // launch choices (result-neutral; set by mcdfci_autotune)
   int scatter_ept = 4;
@@ -457,7 +459,8 @@ void launch_update_z(mcdfci_ctx* c, int k, int count_iteration) {
                c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part, c->eslot, c->znorm);
   if (c->timing && count_iteration) (void)hipEventRecord(c->ev[5], s);
   launch_merge(s, c->nw, c->st, c->pslot, c->tab, c->M, k, c->nchunks, c->stage, c->stage_cnt, c->part,
-               c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration, c->ctr, c->nslots, c->znorm);
+               c->col_xz, c->cand, c->cand_cnt, c->dx, count_iteration, c->ctr, c->nslots, c->znorm,
+               c->merge_tails);
 }
 
 // One iteration of src/solver.h:701-759.  With timing enabled every kernel
@@ -568,6 +571,12 @@ int create_ctx(SpatialHam&& ham_in, const mcdfci_opts* opts, mcdfci_ctx** out) {
     // test hook: fold z word B into A from 2^bits on (53..61) instead of 2^61
     const int v = std::atoi(zb);
     if (v >= 53 && v <= 61) c->znorm_bits = v;
+  }
+  if (const char* mt = std::getenv("MCDFCI_MERGE_TAILS")) {
+    // test hook: the column merge falls back to the full tournament when more
+    // than this many (0..7) chunk lists can reach the column's top M
+    const int v = std::atoi(mt);
+    if (v >= 0 && v <= 7) c->merge_tails = v;
   }
   c->k = opts->num_of_coordinate;
   c->M = opts->estimate;

@@ -38,7 +38,7 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr,
-                  uint64_t nslots, const uint32_t* znorm_list);
+                  uint64_t nslots, const uint32_t* znorm_list, int merge_tails);
 // tree: per-column sorts and a merge tree even where one workgroup could
 // sort all k * M candidates (same result either way).
 void launch_pick(hipStream_t s, int nw, DevState* st, int k, int M, const CandEnt* cand, const int* cand_cnt,

@@ -837,14 +837,14 @@ __device__ void column_tournament(StageEnt* lists, int nchunks, int mw, const in
 // T*E/L - 1 such lists (L = the power of two >= mw) are loaded into LDS
 // beside S and reduced with it by the same tournament.  List 0 ends as the
 // first mw entries of a full sort of the column's entries, returned in
-// blocked registers (entry threadIdx.x*E + r).  With more such lists it
-// returns false without output and the caller runs column_tournament, which
-// reads the staging buffer only.
+// blocked registers (entry threadIdx.x*E + r).  With more such lists (or
+// more than tails_cap) it returns false without output and the caller runs
+// column_tournament, which reads the staging buffer only.
 constexpr int kHead = 64;
 template <int W, int E, int T>
 __device__ bool column_heads(const StageEnt* lists, int nchunks, int mw, const int* offs, int* lcnt,
-                             const SortScratch& x, uint64_t (&A)[E], uint64_t (&B)[E], uint64_t (&B1)[E],
-                             uint32_t (&P)[E]) {
+                             const SortScratch& x, int tails_cap, uint64_t (&A)[E], uint64_t (&B)[E],
+                             uint64_t (&B1)[E], uint32_t (&P)[E]) {
   constexpr bool C3 = W == 2;
   constexpr int kLists = T * E / kHead;
   __shared__ uint64_t lha[kLists], lhb[kLists], lhc[kLists];
@@ -946,7 +946,7 @@ __device__ bool column_heads(const StageEnt* lists, int nchunks, int mw, const i
   }
   __syncthreads();
   const int m = need;
-  if (m > T * E / L - 1) return false;
+  if (m > T * E / L - 1 || m > tails_cap) return false;
   if (m > 0) {
     // S stays at [0, n0); tail r (list tail_of[r], entries kHead..count-1,
     // at most mw - kHead < L) goes to region r + 1 of L entries.
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
                                                const int* stage_cnt, const int64_t* part,
                                                int64_t* col_xz, CandEnt* cand, int* cand_cnt,
                                                const double* dx, int count_iteration, long long* ctr,
-                                               uint64_t nslots, const uint32_t* znorm_list) {
+                                               uint64_t nslots, const uint32_t* znorm_list, int merge_tails) {
   __shared__ uint64_t ka[T * E];
   __shared__ uint64_t kb[T * E];
   __shared__ uint64_t kc[W == 2 ? T * E : 1];
@@ -1162,7 +1162,8 @@ __global__ __launch_bounds__(T) void k_merge(DevState* st, const uint32_t* pslot
     bool done = false;
     if constexpr (T * E >= 2048) {
       if (nchunks * kHead <= T * E)
-        done = column_heads<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, scr, A, B, B1, P);
+        done = column_heads<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, scr, merge_tails,
+                                     A, B, B1, P);
     }
     if (!done)
       column_tournament<W, E, T>(stage + (size_t)t * nchunks * mw, nchunks, mw, offs, lcnt, A, B, B1, P);
@@ -1999,14 +2000,15 @@ void launch_pass2(hipStream_t s, int nw, const DevHam& H, DevState* st, const D2
 void launch_merge(hipStream_t s, int nw, DevState* st, const uint32_t* pslot, void* tab, int M, int k,
                   int nchunks, StageEnt* stage, const int* stage_cnt, const int64_t* part, int64_t* col_xz,
                   CandEnt* cand, int* cand_cnt, const double* dx, int count_iteration, long long* ctr,
-                  uint64_t nslots, const uint32_t* znorm_list) {
+                  uint64_t nslots, const uint32_t* znorm_list, int merge_tails) {
   // Up to 4096 entries: one sort of all chunk lists (256 threads up to 1024
   // entries, then four entries per thread).  Beyond: the streaming merge
   // with a window of 2 * Mp entries.
   const int mw = M < kChunk ? M : kChunk;
 #define MC_MERGE(E_, T_, S_)                                                                                 \
   hipLaunchKernelGGL((k_merge<E_, T_, S_, W>), dim3(k), dim3(T_), 0, s, st, pslot, (SlotW<W>*)tab, M, nchunks, \
-                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration, ctr, nslots, znorm_list)
+                     stage, stage_cnt, part, col_xz, cand, cand_cnt, dx, count_iteration, ctr, nslots, znorm_list, \
+                     merge_tails)
   MC_W(nw, {
     if ((long long)nchunks * mw <= kSortCap) {
       int w = kWG;

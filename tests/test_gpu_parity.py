@@ -411,3 +411,17 @@ def test_z_word_fold_keeps_the_trajectory(name, k, M, z_thr, iters, bits, worklo
     monkeypatch.setenv("MCDFCI_ZNORM_BITS", str(bits))
     mem = 8.0 if name == "c4" else 0.05
     _run_pair(workloads(name)["fcidump"], k, M, z_thr, iters, max_memory=mem, checkpoints=(1, 3))
+
+
+@pytest.mark.parametrize("tails", [0, 1])
+def test_merge_fallback_keeps_the_trajectory(tails, workloads, monkeypatch):
+    """The column merge at M = 256 (29 chunk lists of 256 per column) selects
+    from the lists' first 64 entries in LDS and merges in the rest of the
+    lists that can still reach the column's top M, up to seven of them; with
+    more it falls back to the full tournament over the staging buffer.  On
+    this workload no column needs more than two, so MCDFCI_MERGE_TAILS lowers
+    the seven (0: every column with such a list falls back, 1: those with two)
+    and the trajectory and the table stay bit-exact with the oracle."""
+    print("case merge fallback", tails, flush=True)
+    monkeypatch.setenv("MCDFCI_MERGE_TAILS", str(tails))
+    _run_pair(workloads("c4")["fcidump"], 32, 256, 1e-6, 12, max_memory=8.0)
