@@ -18,9 +18,10 @@
 //    workgroups (kRRows rows each in LDS, from the symmetric H_sub that
 //    k_submatrix stores in both triangles); every product A v is formed by
 //    each workgroup for its rows and all-gathered, and every workgroup runs
-//    the same solve on eight waves (waves 0-3 form every row of the step
-//    redundantly, so the dot products and the next Lanczos vector are known
-//    to each of them; the others join the Gram-Schmidt coefficients);
+//    the same solve on eight waves (waves 0-3 hold every row of the step,
+//    so the dot products and the next Lanczos vector are known to each of
+//    them; they share the Gram-Schmidt update and the Ritz vector by rows
+//    through LDS; the others join the Gram-Schmidt coefficients);
 //    workgroup 0 writes the results.
 // After the alpha step one classical Gram-Schmidt pass against all previous
 // Lanczos vectors (held in LDS; thread i forms q_i.w as four interleaved
@@ -639,8 +640,9 @@ __device__ void tri_vector_w(EigLds& L, int m, double lam, double& ss, double& y
 template <class Mat>
 __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, double perturb, double* Q) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  // Waves 0-3 hold every row and form the step; the others (if any) join
-  // the products A q_j and the Gram-Schmidt coefficients only.
+  // Waves 0-3 hold every row and form the step (the Gram-Schmidt update
+  // and the Ritz vector split by rows between them); the others (if any)
+  // join the products A q_j and the Gram-Schmidt coefficients only.
   const bool cw = w < 4;
   const int mcap = lz_capacity(n);
   double QC[kRows], QP[kRows], W[kRows];
