@@ -6,12 +6,15 @@
 // normalised and perturbed by a fixed 2^-16 pseudo-random vector:
 //   w = A q_j - beta_{j-1} q_{j-1};  alpha_j = q_j.w;  w -= alpha_j q_j;
 //   w -= sum_i (q_i.w) q_i;  beta_j = |w|;  q_{j+1} = w / beta_j.
-// Row sums of A v are four interleaved partial sums over the column classes
-// c = g mod 4 (ascending fused multiply-add chains from 0.0), combined
-// (S0 + S1) + (S2 + S3); dot products are 64 strided per-lane partials
-// (rows l, l + 64, ... added to 0.0 in order) reduced by an xor butterfly;
-// w - beta q, w - alpha q, the Gram-Schmidt subtraction (ascending i) and
-// the Ritz vector are fused multiply-adds.  Two forms compute exactly these sums:
+// Row sums of A v and the Gram-Schmidt coefficients are interleaved partial
+// sums over column (row) classes: four, c = g mod 4 (ascending fused
+// multiply-add chains from 0.0), combined (S0 + S1) + (S2 + S3), in the
+// row-owner form, sixteen combined pairwise over four levels in the form
+// that spreads the rows over workgroups; dot products are 64 strided
+// per-lane partials (rows l, l + 64, ... added to 0.0 in order) reduced by
+// an xor butterfly; w - beta q, w - alpha q, the Gram-Schmidt subtraction
+// (ascending i) and the Ritz vector are fused multiply-adds.  The oracle
+// restates both forms (dot4, dot16):
 //  * n <= kEigLdsN (k_eig1, below): A in LDS, one wave per 64 rows, each
 //    lane owning one row;
 //  * larger n (k_eig): the rows of A spread over ceil(n / kRRows)
@@ -24,8 +27,8 @@
 //    through LDS; the others join the Gram-Schmidt coefficients);
 //    workgroup 0 writes the results.
 // After the alpha step one classical Gram-Schmidt pass against all previous
-// Lanczos vectors (held in LDS; thread i forms q_i.w as four interleaved
-// sums over the rows) keeps them orthogonal, so a convergence test made
+// Lanczos vectors (held in LDS; q_i.w as four or sixteen interleaved sums
+// over the rows, as above) keeps them orthogonal, so a convergence test made
 // late costs steps but not accuracy.  The LDS holds lz_capacity(n) vectors
 // (12288 doubles shared with A up to n = kEigLdsN, 8192 above); a solve
 // reaching that many stops and restarts from its Ritz vector.  At scheduled steps (from
@@ -197,43 +200,53 @@ struct MatRep {
   int r0, nr;
 };
 
-// The fused multiply-add chain S = a[r] b[r] + S over r = r0, r0 + 4, ...
-// < n in ascending r (one column class of a dot product), its loads issued
-// sixteen pairs at a time ahead of the chain.
-__device__ __forceinline__ double class_chain(const double* a, const double* b, int r0, int n) {
+// The chain of one of sixteen column classes, r = r0, r0 + 16, ... < n in
+// ascending r, loads issued sixteen pairs at a time; and the sum of the
+// sixteen class sums held by lanes 16 i + g (g = 0..15, all lanes active),
+// combined pairwise over four levels, (((S0 + S1) + (S2 + S3)) + ...): each
+// level adds the partner lane's value, and a + b = b + a exactly, so every
+// lane of the sixteen gets the same bits.
+__device__ __forceinline__ double class_chain16(const double* a, const double* b, int r0, int n) {
   double S = 0.0;
   int r = r0;
-  for (; r + 60 < n; r += 64) {
+  for (; r + 240 < n; r += 256) {
     double x[16], y[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      x[u] = a[r + 4 * u];
-      y[u] = b[r + 4 * u];
+      x[u] = a[r + 16 * u];
+      y[u] = b[r + 16 * u];
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) S = __builtin_fma(x[u], y[u], S);
   }
-  for (; r + 12 < n; r += 16) {
+  for (; r + 48 < n; r += 64) {
     double x[4], y[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      x[u] = a[r + 4 * u];
-      y[u] = b[r + 4 * u];
+      x[u] = a[r + 16 * u];
+      y[u] = b[r + 16 * u];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) S = __builtin_fma(x[u], y[u], S);
   }
-  for (; r < n; r += 4) S = __builtin_fma(a[r], b[r], S);
+  for (; r < n; r += 16) S = __builtin_fma(a[r], b[r], S);
   return S;
+}
+
+__device__ __forceinline__ double tree16(double S) {
+  const double s1 = S + lane_xor<1>(S);
+  const double s2 = s1 + lane_xor<2>(s1);
+  const double s3 = s2 + lane_xor<4>(s2);
+  return s3 + lane_xor<8>(s3);
 }
 
 // out[r] = sum_c rows(r, c + sft) v[c], c < nv, for every row r < nout of the
 // matrix (sft = 1: the H part of the bordered rows, whose H row r - 1 the
 // block's row r holds from column 1; the output index is then r - 1), each
-// row in the canonical order: four fused multiply-add chains over the column
-// classes c mod 4, combined (S0 + S1) + (S2 + S3).  Lane 4 i + g of wave 0
-// forms class g of the block's row i; the row sums are exchanged through
-// the reply slots of the exchange buffer.  Every thread of every block calls
+// row in the canonical order of this form: sixteen fused multiply-add chains
+// over the column classes c mod 16, combined pairwise (tree16).  Lane 16 i +
+// g of waves 0-2 forms class g of the block's row i; the row sums are
+// exchanged through the reply slots of the exchange buffer.  Every thread of every block calls
 // it with the same arguments; it starts and ends with a barrier.
 __device__ void rep_mv(const MatRep& D, int nout, int nv, const double* v, int sft, double* out) {
   const int tid = threadIdx.x, bs = blockDim.x;
@@ -248,13 +261,12 @@ __device__ void rep_mv(const MatRep& D, int nout, int nv, const double* v, int s
   const int rnd = *D.round;
   const uint32_t tag = xtag(D.counter, rnd);
   uint64_t* rw = D.xb + kXHead + (rnd & 1) * kXRep;
-  if (tid < 64) {
-    const int i = tid >> 2, g = tid & 3;
+  if (tid < 16 * kRRows) {
+    const int i = tid >> 4, g = tid & 15;
     const bool own = i < D.nr && D.r0 + i >= sft;
     double S = 0.0;
-    if (own) S = class_chain(D.rows + i * kRLd + sft, v, g, nv);
-    const double s01 = S + lane_xor<1>(S);
-    const double sum = s01 + lane_xor<2>(s01);
+    if (own) S = class_chain16(D.rows + i * kRLd + sft, v, g, nv);
+    const double sum = tree16(S);
     if (g == 0 && own) xput(rw + 2 * (D.r0 + i - sft), sum, tag);
   }
   int bad = 0;
@@ -707,16 +719,14 @@ __device__ void lanczos_lowest(EigLds& L, const Mat& A, int n, int first_check, 
     }
     __syncthreads();
     // Full reorthogonalisation, one classical Gram-Schmidt pass:
-    // h_i = q_i.w as four interleaved sums S_g over the rows r = g mod 4
-    // (ascending), combined (S0 + S1) + (S2 + S3); then waves 0-3 subtract
-    // sum_i h_i q_i in ascending i.  Thread 4i + g forms S_g and the four
-    // lanes of i combine by two exchanges.
-    for (int i0 = 0; i0 <= j; i0 += (int)blockDim.x >> 2) {
-      const int i = i0 + (tid >> 2), g = tid & 3;
+    // h_i = q_i.w as sixteen interleaved sums S_g over the rows r = g mod 16
+    // (ascending), combined pairwise (tree16); then waves 0-3 subtract
+    // sum_i h_i q_i in ascending i.  Thread 16 i + g forms S_g.
+    for (int i0 = 0; i0 <= j; i0 += (int)blockDim.x >> 4) {
+      const int i = i0 + (tid >> 4), g = tid & 15;
       double S = 0.0;
-      if (i <= j) S = class_chain(Q + (size_t)i * n, L.wv, g, n);
-      const double s01 = S + lane_xor<1>(S);
-      const double h = s01 + lane_xor<2>(s01);
+      if (i <= j) S = class_chain16(Q + (size_t)i * n, L.wv, g, n);
+      const double h = tree16(S);
       if (g == 0 && i <= j) L.hbuf[i] = h;
     }
     __syncthreads();

@@ -828,7 +828,7 @@ void oc_lowest_eig(int n, const double* A, double* lambda, double* q_hi, double*
   std::vector<double> v0(n, 0.0);
   v0[0] = 1.0;
   EigStats st;
-  EigSolve S = solve_lowest(n, [&](int r, int c) { return A[(size_t)r * n + c]; }, v0, st);
+  EigSolve S = solve_lowest(n, [&](int r, int c) { return A[(size_t)r * n + c]; }, v0, st, eig_classes(n - 1));
   const EigPair& ep = S.ep;
   *lambda = S.rq.hi;
   for (int i = 0; i < n; ++i) {
@@ -862,11 +862,12 @@ int oc_eig_update(int k, const double* Hs, const double* x, const double* z, dou
 // attempt's vectors, alpha_0, theta, |T|_inf, residual}.
 void oc_eig_trace(int k, const double* Hs, const double* x, const double* z, const double* s, double* out) {
   auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
+  const int cls = eig_classes(k);
   double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
   for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
   for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
-  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
+  for (int r = 0; r < k; ++r) Hx[r] = dotc(cls, k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
   for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
   dd xx{s[0], s[1]};
   dd xz = fixed_to_dd(to_fixed(s[4]) + to_fixed(s[5]));
@@ -883,7 +884,7 @@ void oc_eig_trace(int k, const double* Hs, const double* x, const double* z, con
   v0[0] = y1;
   for (int i = 0; i < k; ++i) v0[i + 1] = x[i];
   EigStats st;
-  EigSolve S = solve_lowest(n, A, v0, st);
+  EigSolve S = solve_lowest(n, A, v0, st, cls);
   out[0] = S.steps_first;
   out[1] = S.attempts;
   out[2] = S.ep.steps;

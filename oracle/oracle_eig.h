@@ -16,7 +16,9 @@
 //        beta_j = |w|;  q_{j+1} = w / beta_j,
 //      every matrix-vector row sum taken as four interleaved partial sums,
 //      c = j mod 4, each a chain of fused multiply-adds, combined as
-//      (S0 + S1) + (S2 + S3) (dot4), every other dot product in the
+//      (S0 + S1) + (S2 + S3) (dot4) -- sixteen classes combined pairwise
+//      (dot16) where the device spreads the rows over several workgroups
+//      (k + 1 > kEigLdsN) -- every other dot product in the
 //      canonical 64-lane order (red64); the vector updates w - beta q,
 //      w - alpha q, w - sum h_i q_i and the Ritz vector sum_i y_i q_i are
 //      fused multiply-adds too (std::fma here, the device's fma: both
@@ -91,6 +93,32 @@ static inline double dot4(int m, FA a, FB b) {
   }
   return (S[0] + S[1]) + (S[2] + S[3]);
 }
+
+// The same with sixteen classes, c = j mod 16, combined pairwise over four
+// levels: (((S0 + S1) + (S2 + S3)) + ((S4 + S5) + (S6 + S7))) + (... S15).
+template <class FA, class FB>
+static inline double dot16(int m, FA a, FB b) {
+  double S[16];
+  for (int j = 0; j < 16; ++j) {
+    double acc = 0.0;
+    for (int c = j; c < m; c += 16) acc = std::fma(a(c), b(c), acc);
+    S[j] = acc;
+  }
+  for (int w = 16; w > 1; w >>= 1)
+    for (int i = 0; i < w / 2; ++i) S[i] = S[2 * i] + S[2 * i + 1];
+  return S[0];
+}
+
+// dot4 or dot16 (cls = 4 or 16).
+template <class FA, class FB>
+static inline double dotc(int cls, int m, FA a, FB b) {
+  return cls == 16 ? dot16(m, a, b) : dot4(m, a, b);
+}
+
+// Column classes of the subspace solve for k coordinates: the device's
+// row-owner form (k + 1 <= kEigLdsN) sums four, the form that spreads the
+// rows over workgroups sixteen.
+static inline int eig_classes(int k) { return k + 1 > kEigLdsN ? 16 : 4; }
 
 static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
 static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
@@ -288,7 +316,8 @@ static std::vector<double> tri_vector(int m, const double* d, const double* e, d
 // (unnormalised).  first_check: the smallest number of Lanczos vectors at
 // which convergence is tested (then every step).
 template <class Aop>
-static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int first_check, double perturb) {
+static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int first_check, double perturb,
+                              int cls) {
   EigPair out;
   std::vector<double> Q((size_t)(n + 1) * n), w(n), h(n), alpha(n), beta(n), y;
   {
@@ -304,16 +333,16 @@ static EigPair lanczos_lowest(int n, Aop A, const std::vector<double>& v0, int f
   int m = 0;
   for (int j = 0; j < n; ++j) {
     const double* qj = &Q[(size_t)j * n];
-    for (int r = 0; r < n; ++r) w[r] = dot4(n, [&](int c) { return A(r, c); }, [&](int c) { return qj[c]; });
+    for (int r = 0; r < n; ++r) w[r] = dotc(cls, n, [&](int c) { return A(r, c); }, [&](int c) { return qj[c]; });
     if (j > 0)
       for (int r = 0; r < n; ++r) w[r] = std::fma(-beta[j - 1], Q[(size_t)(j - 1) * n + r], w[r]);
     alpha[j] = red64(n, [&](int i) { return qj[i] * w[i]; });
     for (int r = 0; r < n; ++r) w[r] = std::fma(-alpha[j], qj[r], w[r]);
     // Full reorthogonalisation: one classical Gram-Schmidt pass against
-    // q_0..q_j (coefficients as four-way sums over the rows, then
-    // subtracted in ascending order).
+    // q_0..q_j (coefficients as class sums over the rows, then subtracted
+    // in ascending order).
     for (int i = 0; i <= j; ++i)
-      h[i] = dot4(n, [&](int r) { return Q[(size_t)i * n + r]; }, [&](int r) { return w[r]; });
+      h[i] = dotc(cls, n, [&](int r) { return Q[(size_t)i * n + r]; }, [&](int r) { return w[r]; });
     for (int r = 0; r < n; ++r) {
       double acc = w[r];
       for (int i = 0; i <= j; ++i) acc = std::fma(-h[i], Q[(size_t)i * n + r], acc);
@@ -401,13 +430,13 @@ struct EigSolve {
 };
 
 template <class Aop>
-static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st) {
+static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st, int cls) {
   EigSolve S;
   int first = lz_first_check(st.lz_prev);
   const int first0 = first;
   double pert = kLzPerturb;
   for (int attempt = 0; attempt < kLzAttempts; ++attempt) {
-    S.ep = lanczos_lowest(n, A, v0, first, pert);
+    S.ep = lanczos_lowest(n, A, v0, first, pert, cls);
     S.ev.assign(n, 0.0);
     S.dl.assign(n, 0.0);
     S.Ae.assign(n, 0.0);
@@ -415,7 +444,8 @@ static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st)
       S.ev[i] = S.ep.q[i].hi;
       S.dl[i] = S.ep.q[i].lo;
     }
-    for (int r = 0; r < n; ++r) S.Ae[r] = dot4(n, [&](int c) { return A(r, c); }, [&](int c) { return S.ev[c]; });
+    for (int r = 0; r < n; ++r)
+      S.Ae[r] = dotc(cls, n, [&](int c) { return A(r, c); }, [&](int c) { return S.ev[c]; });
     dd t1 = red64_dd(n, [&](int i) { return two_prod(S.ev[i], S.Ae[i]); });
     dd t2 = red64_dd(n, [&](int i) { return two_prod(S.dl[i], S.Ae[i]); });
     S.rq = dd_add(t1, dd_mul_d(t2, 2.0));
@@ -446,11 +476,12 @@ static EigSolve solve_lowest(int n, Aop A, std::vector<double> v0, EigStats& st)
 static int eig_update(int k, const double* Hs, const double* x, const double* z, dd& xx, dd& sc,
                       dd xz, double* dx, EigStats& st) {
   auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
+  const int cls = eig_classes(k);
   double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
   for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
   for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
-  for (int r = 0; r < k; ++r) Hx[r] = dot4(k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
+  for (int r = 0; r < k; ++r) Hx[r] = dotc(cls, k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
   for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
   double yy = dd_add_d(xx, -xTx).hi;
   double y1 = std::sqrt(yy);
@@ -469,7 +500,7 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
     std::vector<double> v0(n);
     v0[0] = y1;
     for (int i = 0; i < k; ++i) v0[i + 1] = x[i];
-    EigSolve S = solve_lowest(n, A, v0, st);
+    EigSolve S = solve_lowest(n, A, v0, st, cls);
     const std::vector<dd>& q = S.ep.q;
     const double lam = S.rq.hi;
     st.max_residual = std::max(st.max_residual, std::sqrt(S.res2));
@@ -500,7 +531,7 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
   // x is the reference determinant alone, which a symmetry can hide from
   // the lowest eigenvector: start from the fixed pseudo-random vector.
   std::vector<double> v0(k, 0.0);
-  EigSolve S = solve_lowest(k, H, v0, st);
+  EigSolve S = solve_lowest(k, H, v0, st, cls);
   if (!(S.rq.hi < 0.0)) return EIG_NONNEG;
   double r = std::sqrt(-S.rq.hi);
   for (int i = 0; i < k; ++i) dx[i] = S.ev[i] * r - x[i];

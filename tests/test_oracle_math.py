@@ -219,7 +219,8 @@ def _reference_update_numpy(Hs, x, z, xx, xz):
     return v * np.sqrt(-w[0]) - x, -w[0], 1.0
 
 
-@pytest.mark.parametrize("k", [1, 2, 8, 33, 64])
+# k + 1 > 90 sums sixteen column classes (the multi-workgroup form), below four
+@pytest.mark.parametrize("k", [1, 2, 8, 33, 64, 128, 256])
 def test_eig_update_vs_numpy_and_identities(k):
     rng = np.random.default_rng(100 + k)
     Hs = rng.normal(size=(k, k)) * 0.1
