@@ -1300,6 +1300,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     L.z[i] = pz[i];
   }
   __syncthreads();
+  // x.x on thread 0 and x.z on thread 64 (two serial chains at once)
   if (tid == 0) {
     double xTx = 0.0;
     for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
@@ -1307,6 +1308,10 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     const double yy = dd_add_d(xx, -xTx).hi;
     L.sc[4] = yy;
     L.sc[5] = __builtin_sqrt(yy);
+  } else if (tid == 64) {
+    double xTz = 0.0;
+    for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
+    L.sc[7] = xTz;
   }
   __syncthreads();
   const double yy = L.sc[4], y1 = L.sc[5];
@@ -1326,8 +1331,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     // H x, all-gathered (the H part of the rows)
     rep_mv(D, k, k, L.x, 1, L.Hx);
     if (tid == 0) {
-      double xTz = 0.0, xTHx = 0.0;
-      for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
+      const double xTz = L.sc[7];
+      double xTHx = 0.0;
       for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
       dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
       dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
@@ -1441,9 +1446,13 @@ __global__ __launch_bounds__(128) void k_eig1(DevState* st, int k, int ldh, cons
       }
     }
   }
-  // x^T x as the ascending sum (every wave, entries broadcast by readlane)
-  double xTx = 0.0;
-  bcast_each2(xr, xr, k, [&](double xi, double) { xTx = xTx + xi * xi; });
+  // x^T x and x^T z as ascending sums, the two chains interleaved (every
+  // wave, entries broadcast by readlane)
+  double xTx = 0.0, xTz = 0.0;
+  bcast_each2(xr, zr, k, [&](double xi, double zi) {
+    xTx = xTx + xi * xi;
+    xTz = xTz + xi * zi;
+  });
   const double yy = dd_add_d(dd{st->xx_hi, st->xx_lo}, -xTx).hi;
   const double y1 = __builtin_sqrt(yy);
   const bool normal = __builtin_fabs(yy) > 1e-14;
@@ -1458,8 +1467,7 @@ __global__ __launch_bounds__(128) void k_eig1(DevState* st, int k, int ldh, cons
     double hx[kR1];
 #pragma unroll
     for (int t = 0; t < kR1; ++t) hx[t] = lane + 64 * t < k ? L.Hx[lane + 64 * t] : 0.0;
-    double xTz = 0.0, xTHx = 0.0;
-    bcast_each2(xr, zr, k, [&](double xi, double zi) { xTz = xTz + xi * zi; });
+    double xTHx = 0.0;
     bcast_each2(xr, hx, k, [&](double xi, double hi) { xTHx = xTHx + xi * hi; });
     const dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
     const dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
