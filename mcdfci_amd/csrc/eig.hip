@@ -1199,10 +1199,14 @@ __device__ __forceinline__ void eig_finish(DevState* st, EigLds& L, int k, int n
     }
     const dd gam = dd_div_d(L.q[0], y1);
     for (int i = tid; i < k; i += bs) dx[i] = dd_add_d(dd_div(L.q[i + 1], gam), -L.x[i]).hi;
+    // max |ev_i|, i >= 1, on wave 0 (a maximum does not depend on the order)
+    double mev = 0.0;
+    if (tid < 64) {
+      for (int i = 1 + tid; i < n; i += 64) mev = fmax(mev, __builtin_fabs(L.ev[i]));
+      for (int off = 32; off >= 1; off >>= 1) mev = fmax(mev, __shfl_xor(mev, off, 64));
+    }
     if (tid == 0) {
       const dd sc_q = dd_mul(gam, dd_sqrt(lam_q));
-      double mev = 0.0;
-      for (int i = 1; i < n; ++i) mev = fmax(mev, __builtin_fabs(L.ev[i]));
       st->max_residual = fmax(st->max_residual, __builtin_sqrt(res2));
       st->curr_max_abs_evec = mev;
       const double sc_d = L.ev[0] / y1 * __builtin_sqrt(-lam);
@@ -1300,18 +1304,19 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
     L.z[i] = pz[i];
   }
   __syncthreads();
-  // x.x on thread 0 and x.z on thread 64 (two serial chains at once)
-  if (tid == 0) {
-    double xTx = 0.0;
-    for (int i = 0; i < k; ++i) xTx = xTx + L.x[i] * L.x[i];
-    dd xx{st->xx_hi, st->xx_lo};
-    const double yy = dd_add_d(xx, -xTx).hi;
-    L.sc[4] = yy;
-    L.sc[5] = __builtin_sqrt(yy);
-  } else if (tid == 64) {
-    double xTz = 0.0;
-    for (int i = 0; i < k; ++i) xTz = xTz + L.x[i] * L.z[i];
-    L.sc[7] = xTz;
+  // x.x on wave 0 and x.z on wave 1, in the 64-lane order (red64)
+  const int wave = tid >> 6;
+  if (wave == 0) {
+    const double xTx = red64(k, [&](int i) { return L.x[i] * L.x[i]; });
+    if (tid == 0) {
+      dd xx{st->xx_hi, st->xx_lo};
+      const double yy = dd_add_d(xx, -xTx).hi;
+      L.sc[4] = yy;
+      L.sc[5] = __builtin_sqrt(yy);
+    }
+  } else if (wave == 1) {
+    const double xTz = red64(k, [&](int i) { return L.x[i] * L.z[i]; });
+    if (tid == 64) L.sc[7] = xTz;
   }
   __syncthreads();
   const double yy = L.sc[4], y1 = L.sc[5];
@@ -1330,14 +1335,15 @@ __global__ __launch_bounds__(kEigThreads) void k_eig(DevState* st, int k, int ld
   if (normal) {
     // H x, all-gathered (the H part of the rows)
     rep_mv(D, k, k, L.x, 1, L.Hx);
-    if (tid == 0) {
+    if (wave == 0) {
       const double xTz = L.sc[7];
-      double xTHx = 0.0;
-      for (int i = 0; i < k; ++i) xTHx = xTHx + L.x[i] * L.Hx[i];
-      dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
-      dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
-      L.sc[6] = dd_div_d(a00, yy).hi;
-      L.v0[0] = y1;
+      const double xTHx = red64(k, [&](int i) { return L.x[i] * L.Hx[i]; });
+      if (tid == 0) {
+        dd xz = fix_to_dd((((i128)st->xz_hi) << 64) | (i128)st->xz_lo);
+        dd a00 = dd_add_d(dd_add_d(xz, -(2.0 * xTz)), xTHx);
+        L.sc[6] = dd_div_d(a00, yy).hi;
+        L.v0[0] = y1;
+      }
     }
     for (int i = tid; i < k; i += bs) {
       L.b0[i] = (L.z[i] - L.Hx[i]) / y1;

@@ -863,12 +863,10 @@ int oc_eig_update(int k, const double* Hs, const double* x, const double* z, dou
 void oc_eig_trace(int k, const double* Hs, const double* x, const double* z, const double* s, double* out) {
   auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
   const int cls = eig_classes(k);
-  double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
-  for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
-  for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
   for (int r = 0; r < k; ++r) Hx[r] = dotc(cls, k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
-  for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
+  double xTx, xTz, xTHx;
+  setup_sums(cls, k, x, z, Hx.data(), xTx, xTz, xTHx);
   dd xx{s[0], s[1]};
   dd xz = fixed_to_dd(to_fixed(s[4]) + to_fixed(s[5]));
   const double yy = dd_add_d(xx, -xTx).hi, y1 = std::sqrt(yy);

@@ -120,6 +120,24 @@ static inline double dotc(int cls, int m, FA a, FB b) {
 // rows over workgroups sixteen.
 static inline int eig_classes(int k) { return k + 1 > kEigLdsN ? 16 : 4; }
 
+// x.x, x.z and x.Hx of the set-up: ascending sums in the row-owner form, the
+// 64-lane order (red64) in the form that spreads the rows over workgroups.
+static inline void setup_sums(int cls, int k, const double* x, const double* z, const double* Hx, double& xTx,
+                              double& xTz, double& xTHx) {
+  if (cls == 16) {
+    xTx = red64(k, [&](int i) { return x[i] * x[i]; });
+    xTz = red64(k, [&](int i) { return x[i] * z[i]; });
+    xTHx = red64(k, [&](int i) { return x[i] * Hx[i]; });
+    return;
+  }
+  xTx = 0.0;
+  xTz = 0.0;
+  xTHx = 0.0;
+  for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
+  for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
+  for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
+}
+
 static inline bool dd_is_zero(dd a) { return a.hi == 0.0 && a.lo == 0.0; }
 static inline dd dd_abs(dd a) { return dd_lt0(a) ? dd_neg(a) : a; }
 static inline bool dd_ge(dd a, dd b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
@@ -477,12 +495,10 @@ static int eig_update(int k, const double* Hs, const double* x, const double* z,
                       dd xz, double* dx, EigStats& st) {
   auto H = [&](int r, int c) { return r <= c ? Hs[r * k + c] : Hs[c * k + r]; };
   const int cls = eig_classes(k);
-  double xTx = 0.0, xTz = 0.0, xTHx = 0.0;
-  for (int i = 0; i < k; ++i) xTx = xTx + x[i] * x[i];
-  for (int i = 0; i < k; ++i) xTz = xTz + x[i] * z[i];
   std::vector<double> Hx(k);
   for (int r = 0; r < k; ++r) Hx[r] = dotc(cls, k, [&](int c) { return H(r, c); }, [&](int c) { return x[c]; });
-  for (int i = 0; i < k; ++i) xTHx = xTHx + x[i] * Hx[i];
+  double xTx, xTz, xTHx;
+  setup_sums(cls, k, x, z, Hx.data(), xTx, xTz, xTHx);
   double yy = dd_add_d(xx, -xTx).hi;
   double y1 = std::sqrt(yy);
 
