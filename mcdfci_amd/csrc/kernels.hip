@@ -549,8 +549,11 @@ __global__ __launch_bounds__(kWG) void k_scatter(DevHam H, DevState* st, const D
 // chunk's top-M list sorted by (value, determinant) descending.
 // SEL 0: full sort of the chunk; 1: top L <= 64 by the per-wave tournament;
 // 2: top 256 (kept entries 65..256).
+// At one word the registers are capped for five waves per SIMD (more random
+// slot reads in flight; a handful of spills): k = 256 pass 2 358 -> 345 us;
+// six waves spill 20 registers and run slower.
 template <bool MIXED, int W, int SEL>
-__global__ __launch_bounds__(kWG) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
+__global__ __launch_bounds__(kWG, W == 1 ? 5 : 1) void k_pass2(DevHam H, DevState* st, const D2* pdet, const double* dx,
                                                const double* Hs, int ldh, SlotW<W>* tab, uint64_t mask,
                                                const uint64_t* vis, int vis_words, int M, int nchunks,
                                                StageEnt* stage, int* stage_cnt, int64_t* part,
