@@ -1477,6 +1477,78 @@ __global__ __launch_bounds__(T) void k_pick_cols(DevState* st, int M, int L, int
   if (threadIdx.x == 0) out_cnt[t] = keep;
 }
 
+// The column sorts fused with the tree's first level: workgroup j refetches
+// the candidates of columns C j .. C j + C - 1 into the C parts of its window
+// (T * E / C >= M entries each), sorts the window and keeps the first K
+// distinct entries -- what a tree node makes of those columns' sorted lists
+// cut to min(M, k): an entry below position K of its own column has K
+// distinct entries above it.  Output as k_pick_merge's (lists K apart).
+template <int E, int T, int W, int C>
+__global__ __launch_bounds__(T) void k_pick_cols_merge(DevState* st, int M, int K, const CandEnt* cand,
+                                                       const int* cand_cnt, const SlotW<W>* tab, PickEnt* out,
+                                                       int* out_cnt) {
+  __shared__ uint64_t ka[T * E];
+  __shared__ uint64_t kb[T * E];
+  __shared__ uint64_t kc[W == 2 ? T * E : 1];
+  __shared__ uint32_t pl[T * E];
+  __shared__ int wtot[T / 64 + 1];
+  if (st->error) return;
+  const int j = blockIdx.x;
+  constexpr int Q = T * E / C;
+  const int kp = st->k_prev;
+  const dd s = load_xss(st);
+  uint64_t A[E], B[E], B1[E];
+  uint32_t P[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    const int c = i / Q, pos = i - c * Q;
+    const int t = C * j + c;
+    const int n = (t < K && t < kp) ? min(cand_cnt[t], M) : 0;
+    A[r] = 0;
+    B[r] = 0;
+    B1[r] = 0;
+    P[r] = 0xffffffffu;
+    if (pos < n) {
+      const CandEnt ce = cand[(size_t)t * M + pos];
+      const SlotW<W>& sl = tab[ce.slot];
+      const double z = fix_to_d(tz_join(sl.za, sl.zb));
+      A[r] = bitsd(grad_key(s, sl.x, z));
+      B[r] = ce.det;
+      B1[r] = ce.det1;
+      P[r] = ce.slot;
+    }
+  }
+  block_sort_desc3<E, W == 2>(A, B, B1, P, SortScratch{ka, kb, pl, kc});
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    kb[threadIdx.x * E + r] = B[r];
+    if constexpr (W == 2) kc[threadIdx.x * E + r] = B1[r];
+  }
+  __syncthreads();
+  bool first[E];
+  int cnt = 0;
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int i = threadIdx.x * E + r;
+    // Fillers are (0, 0, 0); a real determinant is never zero.
+    bool dup = i > 0 && kb[i - 1] == B[r];
+    if constexpr (W == 2) dup = dup && kc[i - 1] == B1[r];
+    first[r] = (B[r] | B1[r]) != 0 && !dup;
+    cnt += first[r] ? 1 : 0;
+  }
+  int distinct;
+  int pos = block_excl_scan(cnt, wtot, distinct);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    if (!first[r]) continue;
+    if (pos < K) out[(size_t)j * K + pos] = PickEnt{A[r], B[r], B1[r], P[r], 0};
+    ++pos;
+  }
+  if (threadIdx.x == 0) out_cnt[j] = distinct < K ? distinct : K;
+}
+
 // One node of the pick's merge tree: the first K distinct entries of the
 // union of F sorted lists (input lists F j .. F j + F - 1).  The window of T * E
 // entries holds list q in its q-th part of T * E / F (>= K) entries, reversed
@@ -2073,19 +2145,6 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
       }
     } else {
       const int L = M < k ? M : k;
-      // Column sorts: width next power of two >= M (<= 1024).
-      int wc = 64;
-      while (wc < M) wc <<= 1;
-#define MC_COLS(E_, T_)                                                                                       \
-  hipLaunchKernelGGL((k_pick_cols<E_, T_, W>), dim3(k), dim3(T_), 0, s, st, M, L, k, cand, cand_cnt, tb, pk0, pc0)
-      switch (wc) {
-        case 64: MC_COLS(1, 64); break;
-        case 128: MC_COLS(1, 128); break;
-        case 256: MC_COLS(1, 256); break;
-        case 512: MC_COLS(2, 256); break;
-        default: MC_COLS(4, 256); break;
-      }
-#undef MC_COLS
       // Merge levels (lists of up to k): four lists per node while the window
       // (the next power of two >= fan-in x k) stays within 512 entries (half
       // as many levels; measured faster at k = 64), else two (a 1024-entry
@@ -2093,9 +2152,41 @@ void launch_pick_w(hipStream_t s, DevState* st, int k, int M, const CandEnt* can
       const int fan = 4 * k <= 512 ? 4 : 2;
       int wm = 64;
       while (wm < fan * k) wm <<= 1;
+      // Column sorts: width next power of two >= M (<= 1024); with four-list
+      // nodes fused with the first merge level (four columns per workgroup)
+      // when the window of 4 x that width holds at most 1024 entries and it
+      // is not the root (k = 64: pick 27.9 -> 26.4 us; with two-list nodes
+      // the 512-entry sorts in half as many workgroups were slower).
+      int wc = 64;
+      while (wc < M) wc <<= 1;
+      const int nfused = (k + fan - 1) / fan;
+      int nin = k;
+      if (fan == 4 && fan * wc <= 1024 && nfused >= 2) {
+        const int win = fan * wc;
+#define MC_CM(E_, T_)                                                                                          \
+  hipLaunchKernelGGL((k_pick_cols_merge<E_, T_, W, 4>), dim3(nfused), dim3(T_), 0, s, st, M, k, cand, cand_cnt, tb, \
+                     pk0, pc0)
+        switch (win) {
+          case 256: MC_CM(1, 256); break;
+          case 512: MC_CM(2, 256); break;
+          default: MC_CM(4, 256); break;
+        }
+#undef MC_CM
+        nin = nfused;
+      } else {
+#define MC_COLS(E_, T_)                                                                                       \
+  hipLaunchKernelGGL((k_pick_cols<E_, T_, W>), dim3(k), dim3(T_), 0, s, st, M, L, k, cand, cand_cnt, tb, pk0, pc0)
+        switch (wc) {
+          case 64: MC_COLS(1, 64); break;
+          case 128: MC_COLS(1, 128); break;
+          case 256: MC_COLS(1, 256); break;
+          case 512: MC_COLS(2, 256); break;
+          default: MC_COLS(4, 256); break;
+        }
+#undef MC_COLS
+      }
       PickEnt *in = pk0, *out = pk1;
       int *ic = pc0, *oc = pc1;
-      int nin = k;
       while (nin > 1) {
         const int nout = (nin + fan - 1) / fan;
         const int root = nout == 1 ? 1 : 0;  // the root emits the picked coordinates
