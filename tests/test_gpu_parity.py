@@ -143,6 +143,8 @@ def _run_pair(path, k, M, z_thr, iters, max_memory=0.05, checkpoints=(1, 10), up
     ("c4", 32, 256, 1e-6, 30),
     # the same with lists kept shorter than the merge window (M = 200 < 256)
     ("c4", 16, 200, 1e-6, 12),
+    # 29 lists of 128: one sort of 3712 entries per column (four per thread)
+    ("c4", 16, 128, 1e-6, 12),
     # 42 orbitals: two-word determinants
     ("c5", 8, 16, 1e-5, 12),
     ("c5", 64, 64, 1e-5, 4),
